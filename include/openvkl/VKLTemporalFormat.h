@@ -1,0 +1,18 @@
+/* Temporal formats; values fixed by openvkl/include/openvkl/VKLTemporalFormat.h.
+ * The HIP device accepts VKL_TEMPORAL_FORMAT_CONSTANT only. */
+#pragma once
+#include "types.h"
+
+enum VKLTemporalFormat
+#if __cplusplus >= 201103L
+    : vkl_uint32
+#endif
+{
+  VKL_TEMPORAL_FORMAT_CONSTANT = 0,
+  VKL_TEMPORAL_FORMAT_STRUCTURED,
+  VKL_TEMPORAL_FORMAT_UNSTRUCTURED,
+  VKL_TEMPORAL_FORMAT_INVALID = 100
+};
+#ifndef __cplusplus
+typedef enum VKLTemporalFormat VKLTemporalFormat;
+#endif

@@ -1,0 +1,54 @@
+/* Extensions of the HIP (MI355X) device. None of these has a counterpart in
+ * the reference API; each one exists because per-ray host calls and
+ * synchronous calls cannot feed a GPU. Results equal those of the reference
+ * calls they batch (see each declaration). */
+#pragma once
+
+#include <stdint.h>
+#include "device/openvkl.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* vklComputeSampleN without the host synchronisation: device pointers only,
+ * enqueued on `stream` (a hipStream_t; NULL = the null stream). Usable under
+ * hipGraph capture. Same results as vklComputeSampleN. */
+OPENVKL_INTERFACE void vklComputeSampleNAsync(const VKLSampler *sampler, unsigned int N,
+                                              const vkl_vec3f *objectCoordinates, float *samples,
+                                              unsigned int attributeIndex, void *stream);
+
+/* vklComputeGradientN without the host synchronisation (device pointers). */
+OPENVKL_INTERFACE void vklComputeGradientNAsync(const VKLSampler *sampler, unsigned int N,
+                                                const vkl_vec3f *objectCoordinates, vkl_vec3f *gradients,
+                                                unsigned int attributeIndex, void *stream);
+
+/* Iterate the intervals of numRays rays in one launch. Ray r writes up to
+ * maxIntervalsPerRay intervals to intervals[r * maxIntervalsPerRay + k] and
+ * their count to numIntervals[r]; the sequence equals calling
+ * vklInitIntervalIterator + vklIterateInterval until it returns 0 (truncated
+ * at maxIntervalsPerRay). tRanges == NULL means [0, inf) for every ray.
+ * Pointers may be host or device memory. */
+OPENVKL_INTERFACE void vklIterateIntervalsBatch(const VKLIntervalIteratorContext *context, unsigned int numRays,
+                                                const vkl_vec3f *origins, const vkl_vec3f *directions,
+                                                const vkl_range1f *tRanges, const float *times,
+                                                VKLInterval *intervals, unsigned int maxIntervalsPerRay,
+                                                unsigned int *numIntervals);
+
+/* Hit iteration for numRays rays in one launch, same contract as above with
+ * vklInitHitIterator + vklIterateHit. */
+OPENVKL_INTERFACE void vklIterateHitsBatch(const VKLHitIteratorContext *context, unsigned int numRays,
+                                           const vkl_vec3f *origins, const vkl_vec3f *directions,
+                                           const vkl_range1f *tRanges, const float *times, VKLHit *hits,
+                                           unsigned int maxHitsPerRay, unsigned int *numHits);
+
+/* Read-only view of a committed volume's node table level (0..2): node
+ * count, voxel codes (numNodes * res^3, res = 64/32/16) and value ranges
+ * (numNodes * res^3 * numAttributes). The same data the reference exposes
+ * through the "InnerNode" observer. Returns 1 on success. */
+OPENVKL_INTERFACE int vklHipGetVolumeLevel(VKLVolume volume, unsigned int level, uint64_t *numNodes,
+                                           const uint64_t **voxels, const vkl_range1f **valueRanges);
+
+#ifdef __cplusplus
+}
+#endif

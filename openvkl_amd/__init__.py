@@ -1,0 +1,546 @@
+"""Python binding of the HIP device's C ABI (libopenvkl_hip.so).
+
+A ctypes mirror of the Open VKL C API (openvkl/include/openvkl/*.h and
+openvkl/devices/cpu/include/openvkl/device/openvkl.h) -- the same calls,
+argument meaning and error behaviour -- plus small helpers that build volumes
+from numpy arrays and pass torch tensors (device memory) or numpy arrays
+(host memory) to the sampling / iteration calls.
+
+The library must be built (openvkl_amd/build.py, or __graft_entry__.build());
+importing this module raises if it is missing -- there is no fallback path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libopenvkl_hip.so")
+
+# ---- enums (values fixed by the reference headers) -----------------------------------
+VKL_DATA, VKL_VOLUME = 0x8000000 + 100, 0x8000000 + 101
+VKL_BOOL, VKL_STRING, VKL_VOID_PTR = 250, 1500, 200
+VKL_UCHAR, VKL_SHORT, VKL_USHORT, VKL_INT, VKL_UINT = 2500, 3000, 3500, 4000, 4500
+VKL_HALF, VKL_FLOAT, VKL_DOUBLE = 5800, 6000, 7000
+VKL_VEC3I, VKL_VEC3F, VKL_BOX1F, VKL_BOX3I, VKL_AFFINE3F = 4002, 6002, 10000, 8002, 12003
+VKL_DATA_DEFAULT, VKL_DATA_SHARED_BUFFER = 0, 1
+VKL_FILTER_NEAREST, VKL_FILTER_LINEAR, VKL_FILTER_CUBIC = 0, 100, 200
+VKL_FORMAT_TILE, VKL_FORMAT_DENSE_ZYX = 0, 1
+VKL_NO_ERROR, VKL_UNKNOWN_ERROR, VKL_INVALID_ARGUMENT = 0, 1, 2
+
+NP_TO_VKL = {
+    np.dtype(np.uint8): VKL_UCHAR,
+    np.dtype(np.int16): VKL_SHORT,
+    np.dtype(np.uint16): VKL_USHORT,
+    np.dtype(np.int32): VKL_INT,
+    np.dtype(np.uint32): VKL_UINT,
+    np.dtype(np.float16): VKL_HALF,
+    np.dtype(np.float32): VKL_FLOAT,
+    np.dtype(np.float64): VKL_DOUBLE,
+}
+
+
+class VKLObject(C.Structure):
+    _fields_ = [("host", C.c_void_p), ("device", C.c_void_p)]
+
+
+class vec3f(C.Structure):
+    _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float)]
+
+
+class range1f(C.Structure):
+    _fields_ = [("lower", C.c_float), ("upper", C.c_float)]
+
+
+class box3f(C.Structure):
+    _fields_ = [("lower", vec3f), ("upper", vec3f)]
+
+
+class VKLInterval(C.Structure):
+    _fields_ = [("tRange", range1f), ("valueRange", range1f), ("nominalDeltaT", C.c_float)]
+
+
+class VKLHit(C.Structure):
+    _fields_ = [("t", C.c_float), ("sample", C.c_float), ("epsilon", C.c_float)]
+
+
+def _wide(w):
+    class VV(C.Structure):
+        _fields_ = [("x", C.c_float * w), ("y", C.c_float * w), ("z", C.c_float * w)]
+
+    class VR(C.Structure):
+        _fields_ = [("lower", C.c_float * w), ("upper", C.c_float * w)]
+
+    class VI(C.Structure):
+        _fields_ = [("tRange", VR), ("valueRange", VR), ("nominalDeltaT", C.c_float * w)]
+
+    class VH(C.Structure):
+        _fields_ = [("t", C.c_float * w), ("sample", C.c_float * w), ("epsilon", C.c_float * w)]
+
+    return VV, VR, VI, VH
+
+
+WIDE = {w: _wide(w) for w in (4, 8, 16)}
+
+_lib = None
+ERROR_CALLBACK = C.CFUNCTYPE(None, C.c_void_p, C.c_uint32, C.c_char_p)
+
+
+def lib():
+    """The loaded C-ABI library (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with openvkl_amd/build.py")
+    L = C.CDLL(LIB_PATH)
+    vp, u32, i32, f32, sz = C.c_void_p, C.c_uint32, C.c_int, C.c_float, C.c_size_t
+    O = VKLObject
+    sig = {
+        "vklInit": (None, []),
+        "vklNewDevice": (vp, [C.c_char_p]),
+        "vklDeviceSetErrorCallback": (None, [vp, ERROR_CALLBACK, vp]),
+        "vklDeviceSetInt": (None, [vp, C.c_char_p, i32]),
+        "vklDeviceSetVoidPtr": (None, [vp, C.c_char_p, vp]),
+        "vklCommitDevice": (None, [vp]),
+        "vklDeviceGetLastErrorCode": (u32, [vp]),
+        "vklDeviceGetLastErrorMsg": (C.c_char_p, [vp]),
+        "vklGetNativeSIMDWidth": (i32, [vp]),
+        "vklReleaseDevice": (None, [vp]),
+        "vklCommit": (None, [O]),
+        "vklRelease": (None, [O]),
+        "vklNewData": (O, [vp, sz, u32, vp, u32, sz]),
+        "vklNewVolume": (O, [vp, C.c_char_p]),
+        "vklGetBoundingBox": (box3f, [O]),
+        "vklGetNumAttributes": (C.c_uint, [O]),
+        "vklGetValueRange": (range1f, [O, C.c_uint]),
+        "vklNewSampler": (O, [O]),
+        "vklNewIntervalIteratorContext": (O, [O]),
+        "vklNewHitIteratorContext": (O, [O]),
+        "vklSetParam": (None, [O, C.c_char_p, u32, vp]),
+        "vklSetBool": (None, [O, C.c_char_p, i32]),
+        "vklSetFloat": (None, [O, C.c_char_p, f32]),
+        "vklSetInt": (None, [O, C.c_char_p, i32]),
+        "vklSetVec3f": (None, [O, C.c_char_p, f32, f32, f32]),
+        "vklSetVec3i": (None, [O, C.c_char_p, i32, i32, i32]),
+        "vklSetData": (None, [O, C.c_char_p, O]),
+        "vklComputeSample": (f32, [C.POINTER(O), C.POINTER(vec3f), C.c_uint, f32]),
+        "vklComputeSampleN": (None, [C.POINTER(O), C.c_uint, vp, vp, C.c_uint, vp]),
+        "vklComputeSampleM": (None, [C.POINTER(O), C.POINTER(vec3f), vp, C.c_uint, vp, f32]),
+        "vklComputeSampleMN": (None, [C.POINTER(O), C.c_uint, vp, vp, C.c_uint, vp, vp]),
+        "vklComputeGradient": (vec3f, [C.POINTER(O), C.POINTER(vec3f), C.c_uint, f32]),
+        "vklComputeGradientN": (None, [C.POINTER(O), C.c_uint, vp, vp, C.c_uint, vp]),
+        "vklComputeSampleNAsync": (None, [C.POINTER(O), C.c_uint, vp, vp, C.c_uint, vp]),
+        "vklComputeGradientNAsync": (None, [C.POINTER(O), C.c_uint, vp, vp, C.c_uint, vp]),
+        "vklGetIntervalIteratorSize": (sz, [C.POINTER(O)]),
+        "vklInitIntervalIterator": (vp, [C.POINTER(O), C.POINTER(vec3f), C.POINTER(vec3f), C.POINTER(range1f), f32, vp]),
+        "vklIterateInterval": (i32, [vp, C.POINTER(VKLInterval)]),
+        "vklGetHitIteratorSize": (sz, [C.POINTER(O)]),
+        "vklInitHitIterator": (vp, [C.POINTER(O), C.POINTER(vec3f), C.POINTER(vec3f), C.POINTER(range1f), f32, vp]),
+        "vklIterateHit": (i32, [vp, C.POINTER(VKLHit)]),
+        "vklIterateIntervalsBatch": (None, [C.POINTER(O), C.c_uint, vp, vp, vp, vp, vp, C.c_uint, vp]),
+        "vklIterateHitsBatch": (None, [C.POINTER(O), C.c_uint, vp, vp, vp, vp, vp, C.c_uint, vp]),
+        "vklHipGetVolumeLevel": (i32, [O, C.c_uint, C.POINTER(C.c_uint64), C.POINTER(C.POINTER(C.c_uint64)),
+                                       C.POINTER(C.POINTER(range1f))]),
+    }
+    for w in (4, 8, 16):
+        VV, VR, VI, VH = WIDE[w]
+        sig[f"vklComputeSample{w}"] = (None, [vp, C.POINTER(O), C.POINTER(VV), vp, C.c_uint, vp])
+        sig[f"vklComputeGradient{w}"] = (None, [vp, C.POINTER(O), C.POINTER(VV), C.POINTER(VV), C.c_uint, vp])
+        sig[f"vklComputeSampleM{w}"] = (None, [vp, C.POINTER(O), C.POINTER(VV), vp, C.c_uint, vp, vp])
+        sig[f"vklGetIntervalIteratorSize{w}"] = (sz, [C.POINTER(O)])
+        sig[f"vklInitIntervalIterator{w}"] = (vp, [vp, C.POINTER(O), C.POINTER(VV), C.POINTER(VV), C.POINTER(VR), vp, vp])
+        sig[f"vklIterateInterval{w}"] = (None, [vp, vp, C.POINTER(VI), vp])
+        sig[f"vklGetHitIteratorSize{w}"] = (sz, [C.POINTER(O)])
+        sig[f"vklInitHitIterator{w}"] = (vp, [vp, C.POINTER(O), C.POINTER(VV), C.POINTER(VV), C.POINTER(VR), vp, vp])
+        sig[f"vklIterateHit{w}"] = (None, [vp, vp, C.POINTER(VH), vp])
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+# ---- pointer helpers ----------------------------------------------------------------------
+
+def _ptr(a):
+    """Address of a numpy array or torch tensor (None -> NULL)."""
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data
+    return a.data_ptr()  # torch.Tensor (host or device)
+
+
+def _n_points(a):
+    return int(a.shape[0])
+
+
+class VklError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"VKL error {code}: {msg}")
+        self.code = code
+        self.msg = msg
+
+
+class Device:
+    """vklNewDevice("hip") + vklCommitDevice. Errors are collected, not printed."""
+
+    def __init__(self, name: str = "hip", stream=None):
+        L = lib()
+        self.h = L.vklNewDevice(name.encode())
+        if not self.h:
+            raise RuntimeError(f"vklNewDevice({name!r}) failed")
+        self.errors = []
+        self._cb = ERROR_CALLBACK(lambda u, c, m: self.errors.append((int(c), m.decode())))
+        L.vklDeviceSetErrorCallback(self.h, self._cb, None)
+        if stream is not None:
+            L.vklDeviceSetVoidPtr(self.h, b"stream", stream)
+        L.vklCommitDevice(self.h)
+        self.check()
+
+    def last_error(self):
+        L = lib()
+        return int(L.vklDeviceGetLastErrorCode(self.h)), L.vklDeviceGetLastErrorMsg(self.h).decode()
+
+    def check(self):
+        if self.errors:
+            code, msg = self.errors[-1]
+            self.errors.clear()
+            raise VklError(code, msg)
+
+    def release(self):
+        if self.h:
+            lib().vklReleaseDevice(self.h)
+            self.h = None
+
+    # -- object creation ------------------------------------------------------------------
+    def data(self, array, dtype=None, shared=False):
+        """vklNewData from a numpy array (copied) or a torch tensor (shared)."""
+        if isinstance(array, np.ndarray):
+            a = np.ascontiguousarray(array)
+            t = dtype if dtype is not None else NP_TO_VKL[a.dtype]
+            n = a.size if t not in (VKL_VEC3I, VKL_VEC3F, VKL_BOX1F) else a.size // (3 if t != VKL_BOX1F else 2)
+            h = lib().vklNewData(self.h, n, t, a.ctypes.data, VKL_DATA_SHARED_BUFFER if shared else 0, 0)
+            obj = Handle(self, h, keep=a if shared else None)
+        else:  # torch tensor, always shared (device memory stays in place)
+            import torch
+
+            tmap = {torch.float32: VKL_FLOAT, torch.float16: VKL_HALF, torch.uint8: VKL_UCHAR,
+                    torch.int16: VKL_SHORT, torch.float64: VKL_DOUBLE, torch.int32: VKL_INT}
+            t = dtype if dtype is not None else tmap[array.dtype]
+            h = lib().vklNewData(self.h, array.numel(), t, array.data_ptr(), VKL_DATA_SHARED_BUFFER, 0)
+            obj = Handle(self, h, keep=array)
+        self.check()
+        return obj
+
+    def data_of_data(self, items):
+        arr = (VKLObject * len(items))(*[i.h for i in items])
+        h = lib().vklNewData(self.h, len(items), VKL_DATA, C.addressof(arr), 0, 0)
+        self.check()
+        return Handle(self, h)
+
+    def new_volume(self, type_name):
+        h = lib().vklNewVolume(self.h, type_name.encode())
+        self.check()
+        return Volume(self, h)
+
+    def structured_regular(self, voxels, origin=(0, 0, 0), spacing=(1, 1, 1), filter=None, cell_centered=None,
+                           background=None, index_origin=None, index_to_object=None):
+        """voxels: array [nz, ny, nx] (x fastest) or a list of them."""
+        arrs = voxels if isinstance(voxels, (list, tuple)) else [voxels]
+        v = self.new_volume("structuredRegular")
+        nz, ny, nx = arrs[0].shape
+        v.set_vec3i("dimensions", nx, ny, nz)
+        v.set_vec3f("gridOrigin", *origin)
+        v.set_vec3f("gridSpacing", *spacing)
+        datas = [self.data(a) if isinstance(a, np.ndarray) else self.data(a.reshape(-1)) for a in arrs]
+        if len(datas) == 1:
+            v.set_data("data", datas[0])
+        else:
+            dd = self.data_of_data(datas)
+            v.set_data("data", dd)
+            dd.release()
+        for d in datas:
+            d.release()
+        if filter is not None:
+            v.set_int("filter", filter)
+        if cell_centered is not None:
+            v.set_bool("cellCentered", cell_centered)
+        if background is not None:
+            v.set_float("background", background)
+        if index_origin is not None:
+            v.set_vec3i("indexOrigin", *index_origin)
+        if index_to_object is not None:
+            d = self.data(np.asarray(index_to_object, np.float32))
+            v.set_data("indexToObject", d)
+            d.release()
+        v.commit()
+        return v
+
+    def vdb(self, level, origin, fmt, packed_dense=None, packed_tile=None, leaf_data=None, background=None,
+            index_to_object=None, filter=None):
+        """Sparse vdb volume from node arrays (packed or per-leaf data)."""
+        v = self.new_volume("vdb")
+        tmp = []
+
+        def d(a, t=None):
+            h = self.data(a, t)
+            tmp.append(h)
+            return h
+
+        v.set_data("node.level", d(np.ascontiguousarray(level, np.uint32)))
+        v.set_data("node.origin", d(np.ascontiguousarray(origin, np.int32).reshape(-1), VKL_VEC3I))
+        v.set_data("node.format", d(np.ascontiguousarray(fmt, np.uint32)))
+        if packed_dense is not None:
+            pd = self.data_of_data([d(a) if isinstance(a, np.ndarray) else d(a.reshape(-1)) for a in packed_dense])
+            tmp.append(pd)
+            v.set_data("nodesPackedDense", pd)
+            if packed_tile is not None:
+                pt = self.data_of_data([d(a) for a in packed_tile])
+                tmp.append(pt)
+                v.set_data("nodesPackedTile", pt)
+        else:
+            per_leaf = []
+            for leaf in leaf_data:
+                if len(leaf) == 1:
+                    per_leaf.append(d(leaf[0]))
+                else:
+                    x = self.data_of_data([d(a) for a in leaf])
+                    tmp.append(x)
+                    per_leaf.append(x)
+            nd = self.data_of_data(per_leaf)
+            tmp.append(nd)
+            v.set_data("node.data", nd)
+        if background is not None:
+            v.set_float("background", background)
+        if filter is not None:
+            v.set_int("filter", filter)
+        if index_to_object is not None:
+            v.set_data("indexToObject", d(np.asarray(index_to_object, np.float32)))
+        v.commit()
+        for h in tmp:
+            h.release()
+        return v
+
+
+class Handle:
+    """A VKLObject handle with typed setters."""
+
+    def __init__(self, device, h, keep=None):
+        self.device = device
+        self.h = h
+        self._keep = keep
+
+    def _set(self, fn, name, *args):
+        getattr(lib(), fn)(self.h, name.encode(), *args)
+        self.device.check()
+
+    def set_int(self, name, x):
+        self._set("vklSetInt", name, int(x))
+
+    def set_float(self, name, x):
+        self._set("vklSetFloat", name, float(x))
+
+    def set_bool(self, name, b):
+        self._set("vklSetBool", name, int(bool(b)))
+
+    def set_vec3f(self, name, x, y, z):
+        self._set("vklSetVec3f", name, float(x), float(y), float(z))
+
+    def set_vec3i(self, name, x, y, z):
+        self._set("vklSetVec3i", name, int(x), int(y), int(z))
+
+    def set_data(self, name, data):
+        self._set("vklSetData", name, data.h)
+
+    def commit(self):
+        lib().vklCommit(self.h)
+        self.device.check()
+
+    def release(self):
+        if self.h is not None and self.h.host:
+            lib().vklRelease(self.h)
+            self.h = VKLObject()
+
+
+class Volume(Handle):
+    def bounding_box(self):
+        b = lib().vklGetBoundingBox(self.h)
+        return np.array([b.lower.x, b.lower.y, b.lower.z, b.upper.x, b.upper.y, b.upper.z], np.float32)
+
+    def value_range(self, attr=0):
+        r = lib().vklGetValueRange(self.h, attr)
+        return np.array([r.lower, r.upper], np.float32)
+
+    def num_attributes(self):
+        return int(lib().vklGetNumAttributes(self.h))
+
+    def level(self, l):
+        n = C.c_uint64()
+        vox = C.POINTER(C.c_uint64)()
+        rng = C.POINTER(range1f)()
+        ok = lib().vklHipGetVolumeLevel(self.h, l, C.byref(n), C.byref(vox), C.byref(rng))
+        self.device.check()
+        assert ok
+        res = [64, 32, 16][l]
+        count = n.value * res ** 3
+        na = self.num_attributes()
+        v = np.ctypeslib.as_array(vox, shape=(count,)).copy() if count else np.zeros(0, np.uint64)
+        r = (np.ctypeslib.as_array(C.cast(rng, C.POINTER(C.c_float)), shape=(count * na * 2,)).reshape(count, na, 2).copy()
+             if count else np.zeros((0, na, 2), np.float32))
+        return n.value, v, r
+
+    def sampler(self, filter=None, gradient_filter=None, max_sampling_depth=None):
+        h = lib().vklNewSampler(self.h)
+        self.device.check()
+        s = Sampler(self.device, h)
+        s.volume = self
+        if filter is not None:
+            s.set_int("filter", filter)
+        if gradient_filter is not None:
+            s.set_int("gradientFilter", gradient_filter)
+        if max_sampling_depth is not None:
+            s.set_int("maxSamplingDepth", max_sampling_depth)
+        s.commit()
+        return s
+
+
+class Sampler(Handle):
+    def _out(self, like, shape, dtype=np.float32):
+        if isinstance(like, np.ndarray):
+            return np.empty(shape, dtype)
+        import torch
+
+        return torch.empty(shape, dtype=torch.float32, device=like.device)
+
+    def sample(self, xyz, attr=0, out=None):
+        """vklComputeSampleN; xyz: [N,3] float32 numpy (host) or torch (device)."""
+        n = _n_points(xyz)
+        out = self._out(xyz, (n,)) if out is None else out
+        lib().vklComputeSampleN(C.byref(self.h), n, _ptr(xyz), _ptr(out), attr, None)
+        self.device.check()
+        return out
+
+    def sample_async(self, xyz, out, attr=0, stream=None):
+        lib().vklComputeSampleNAsync(C.byref(self.h), _n_points(xyz), _ptr(xyz), _ptr(out), attr, stream)
+        self.device.check()
+        return out
+
+    def gradient(self, xyz, attr=0, out=None):
+        n = _n_points(xyz)
+        out = self._out(xyz, (n, 3)) if out is None else out
+        lib().vklComputeGradientN(C.byref(self.h), n, _ptr(xyz), _ptr(out), attr, None)
+        self.device.check()
+        return out
+
+    def sample_m(self, xyz, attrs):
+        n = _n_points(xyz)
+        at = np.ascontiguousarray(attrs, np.uint32)
+        out = self._out(xyz, (n, at.size))
+        lib().vklComputeSampleMN(C.byref(self.h), n, _ptr(xyz), _ptr(out), at.size, at.ctypes.data, None)
+        self.device.check()
+        return out
+
+    def sample1(self, p, attr=0):
+        v = vec3f(*[float(x) for x in p])
+        r = lib().vklComputeSample(C.byref(self.h), C.byref(v), attr, 0.0)
+        self.device.check()
+        return r
+
+    def gradient1(self, p, attr=0):
+        v = vec3f(*[float(x) for x in p])
+        r = lib().vklComputeGradient(C.byref(self.h), C.byref(v), attr, 0.0)
+        self.device.check()
+        return np.array([r.x, r.y, r.z], np.float32)
+
+    def sample_wide(self, pts, attr=0, width=4, valid=None):
+        VV = WIDE[width][0]
+        vv = VV()
+        k = len(pts)
+        for i in range(k):
+            vv.x[i], vv.y[i], vv.z[i] = [float(c) for c in pts[i]]
+        val = (C.c_int * width)(*([1] * k + [0] * (width - k))) if valid is None else (C.c_int * width)(*valid)
+        out = (C.c_float * width)()
+        getattr(lib(), f"vklComputeSample{width}")(C.addressof(val), C.byref(self.h), C.byref(vv), C.addressof(out), attr, None)
+        self.device.check()
+        return np.array(out[:k], np.float32)
+
+    def interval_context(self, attr=0, value_ranges=None, hint=None):
+        h = lib().vklNewIntervalIteratorContext(self.h)
+        self.device.check()
+        c = IteratorContext(self.device, h)
+        c.set_int("attributeIndex", attr)
+        if value_ranges is not None:
+            d = self.device.data(np.asarray(value_ranges, np.float32).reshape(-1), VKL_BOX1F)
+            c.set_data("valueRanges", d)
+            d.release()
+        if hint is not None:
+            c.set_float("intervalResolutionHint", hint)
+        c.commit()
+        c.hit = False
+        return c
+
+    def hit_context(self, values, attr=0):
+        h = lib().vklNewHitIteratorContext(self.h)
+        self.device.check()
+        c = IteratorContext(self.device, h)
+        c.set_int("attributeIndex", attr)
+        d = self.device.data(np.asarray(values, np.float32))
+        c.set_data("values", d)
+        d.release()
+        c.commit()
+        c.hit = True
+        return c
+
+
+class IteratorContext(Handle):
+    def intervals(self, org, dir, t_range=(0.0, np.inf), time=0.0, max_out=100000):
+        """Scalar API: vklInitIntervalIterator + vklIterateInterval loop."""
+        L = lib()
+        buf = C.create_string_buffer(L.vklGetIntervalIteratorSize(C.byref(self.h)))
+        o, d = vec3f(*map(float, org)), vec3f(*map(float, dir))
+        t = range1f(*map(float, t_range))
+        it = L.vklInitIntervalIterator(C.byref(self.h), C.byref(o), C.byref(d), C.byref(t), time, buf)
+        self.device.check()
+        out = []
+        iv = VKLInterval()
+        while len(out) < max_out and L.vklIterateInterval(it, C.byref(iv)):
+            out.append([iv.tRange.lower, iv.tRange.upper, iv.valueRange.lower, iv.valueRange.upper, iv.nominalDeltaT])
+        self.device.check()
+        return np.array(out, np.float32).reshape(-1, 5)
+
+    def hits(self, org, dir, t_range=(0.0, np.inf), time=0.0, max_out=10000):
+        L = lib()
+        buf = C.create_string_buffer(L.vklGetHitIteratorSize(C.byref(self.h)))
+        o, d = vec3f(*map(float, org)), vec3f(*map(float, dir))
+        t = range1f(*map(float, t_range))
+        it = L.vklInitHitIterator(C.byref(self.h), C.byref(o), C.byref(d), C.byref(t), time, buf)
+        self.device.check()
+        out = []
+        h = VKLHit()
+        while len(out) < max_out and L.vklIterateHit(it, C.byref(h)):
+            out.append([h.t, h.sample, h.epsilon])
+        self.device.check()
+        return np.array(out, np.float32).reshape(-1, 3)
+
+    def batch(self, org, dir, t_ranges=None, times=None, max_per_ray=64):
+        """vklIterateIntervalsBatch / vklIterateHitsBatch. org/dir: [R,3] numpy
+        (host) or torch (device) float32. Returns (results, counts)."""
+        r = _n_points(org)
+        width = 3 if self.hit else 5
+        if isinstance(org, np.ndarray):
+            out = np.empty((r, max_per_ray, width), np.float32)
+            cnt = np.empty(r, np.uint32)
+        else:
+            import torch
+
+            out = torch.empty((r, max_per_ray, width), dtype=torch.float32, device=org.device)
+            cnt = torch.empty(r, dtype=torch.int32, device=org.device)
+        fn = lib().vklIterateHitsBatch if self.hit else lib().vklIterateIntervalsBatch
+        fn(C.byref(self.h), r, _ptr(org), _ptr(dir), _ptr(t_ranges), _ptr(times), _ptr(out), max_per_ray, _ptr(cnt))
+        self.device.check()
+        return out, cnt

@@ -1,0 +1,1128 @@
+// C ABI of the HIP device: the object API (openvkl/api/API.cpp:104-525) and
+// the hot-path calls (openvkl/devices/cpu/api/DeviceAPI.cpp:65-533), plus the
+// batch / async extensions declared in openvkl/openvkl_hip.h.
+//
+// Error contract (API.cpp:37-89): every entry catches exceptions and reports
+// them to the owning device (last error code / message + error callback);
+// std::bad_alloc maps to VKL_OUT_OF_MEMORY, everything else to the code the
+// exception carries (VKL_UNKNOWN_ERROR by default). Default returns: NaN
+// samples, null handles / iterators, 0 sizes, 0 from iterate calls.
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+
+#include "kernels.h"
+#include "objects.h"
+#include "openvkl/openvkl_hip.h"
+
+using namespace vklhip;
+
+namespace vklhip {
+
+// ---- Sampler ------------------------------------------------------------------
+
+Sampler::Sampler(Device *d, Volume *v) : ManagedObject(d), volume(v)
+{
+  volume->retain();
+  commit();
+}
+
+Sampler::~Sampler() { volume->release(); }
+
+// VdbSampler::commit, VdbSampler.cpp:37-59.
+void Sampler::commit()
+{
+  if (!volume->committed) throw VklError("the volume of this sampler is not committed");
+  const int f  = getInt("filter", volume->filter);
+  const int gf = getInt("gradientFilter", hasParam("filter") ? f : volume->gradientFilter);
+  const int md = getInt("maxSamplingDepth", volume->maxSamplingDepth);
+  for (int x : {f, gf})
+    if (x != kFilterNearest && x != kFilterLinear && x != kFilterCubic) throw VklError("invalid filter");
+  view.grid                  = volume->grid;
+  view.grid.maxSamplingDepth = (uint32_t)std::max(md, 0);
+  view.filter                = f;
+  view.gradientFilter        = gf;
+  device->makeCurrent();
+  if (!deviceView.ptr) deviceView.allocate(sizeof(SamplerView));
+  deviceView.upload(&view, sizeof(view), device->stream);
+  committed = true;
+}
+
+const AttrView &Sampler::attr(uint32_t a) const
+{
+  if (a >= volume->attrViews.size()) throw VklError("illegal attributeIndex requested on volume");
+  return volume->attrViews[a];
+}
+
+// ---- Iterator contexts --------------------------------------------------------------
+
+IteratorContext::IteratorContext(Device *d, Sampler *s, bool hit) : ManagedObject(d), sampler(s), isHit(hit)
+{
+  sampler->retain();
+  commit();
+}
+
+IteratorContext::~IteratorContext() { sampler->release(); }
+
+namespace {
+// mapToMaxIteratorDepth for VDB volumes, IteratorContext.cpp:49-174.
+int mapToDepth(float hint)
+{
+  hint = std::max(std::min(1.f, hint), 0.f);
+  static const std::pair<float, int> table[4] = {{0.f, 0}, {0.2f, 1}, {0.4f, 2}, {0.8f, 3}};
+  int depth = 0;
+  for (const auto &m : table) {
+    if (hint >= m.first)
+      depth = m.second;
+    else
+      break;
+  }
+  return depth;
+}
+}  // namespace
+
+// IntervalIteratorContext::commit / HitIteratorContext::commit,
+// IteratorContext.cpp:189-336.
+void IteratorContext::commit()
+{
+  const int attrIndex = getInt("attributeIndex", 0);
+  if (attrIndex < 0 || (uint32_t)attrIndex >= sampler->volume->numAttributes())
+    throw VklError("illegal attributeIndex requested on volume");
+  view.sampler        = sampler->view;
+  view.attributeIndex = (uint32_t)attrIndex;
+  std::vector<Range1f> ranges;
+  std::vector<float> values;
+  if (isHit) {
+    if (Data *d = getData("values")) values = d->hostArray<float>();
+    for (float v : values) ranges.push_back({v, v});
+    // Sparse vdb: elementary cells (depth 3); structuredRegular: depth 2.
+    view.maxIteratorDepth = (uint32_t)(sampler->volume->grid.dense ? mapToDepth(0.5f) : mapToDepth(1.f));
+  } else {
+    if (Data *d = getData("valueRanges")) {
+      std::vector<uint8_t> b = d->hostCopy();
+      ranges.resize(d->numItems);
+      std::memcpy(ranges.data(), b.data(), d->numItems * sizeof(Range1f));
+    }
+    view.maxIteratorDepth = (uint32_t)mapToDepth(getFloat("intervalResolutionHint", 0.5f));
+  }
+  device->makeCurrent();
+  view.numRanges    = (int32_t)ranges.size();
+  view.rangesMinMax = {-std::numeric_limits<float>::infinity(), std::numeric_limits<float>::infinity()};
+  view.ranges       = nullptr;
+  if (!ranges.empty()) {
+    view.rangesMinMax = {std::numeric_limits<float>::infinity(), -std::numeric_limits<float>::infinity()};
+    for (const Range1f &r : ranges) {
+      view.rangesMinMax.lower = r.lower < view.rangesMinMax.lower ? r.lower : view.rangesMinMax.lower;
+      view.rangesMinMax.upper = r.upper > view.rangesMinMax.upper ? r.upper : view.rangesMinMax.upper;
+    }
+    rangesBuf.allocate(ranges.size() * sizeof(Range1f));
+    rangesBuf.upload(ranges.data(), ranges.size() * sizeof(Range1f), device->stream);
+    view.ranges = rangesBuf.as<Range1f>();
+  }
+  view.numValues = (int32_t)values.size();
+  view.values    = nullptr;
+  if (!values.empty()) {
+    valuesBuf.allocate(values.size() * sizeof(float));
+    valuesBuf.upload(values.data(), values.size() * sizeof(float), device->stream);
+    view.values = valuesBuf.as<float>();
+  }
+  committed = true;
+}
+
+}  // namespace vklhip
+
+// ---- error plumbing ------------------------------------------------------------------
+
+namespace {
+
+void report(Device *d, VKLError e, const char *msg)
+{
+  if (d)
+    d->handleError(e, msg);
+  else
+    std::fprintf(stderr, "OPENVKL ERROR (no device): %s\n", msg);
+}
+
+#define VKL_TRY(devExpr)   \
+  Device *dev_ = nullptr;  \
+  try {                    \
+    dev_ = (devExpr);
+#define VKL_CATCH(ret)                                                      \
+  }                                                                         \
+  catch (const VklError &e)                                                 \
+  {                                                                         \
+    report(dev_, e.code, e.what());                                         \
+    return ret;                                                             \
+  }                                                                         \
+  catch (const std::bad_alloc &)                                            \
+  {                                                                         \
+    report(dev_, VKL_OUT_OF_MEMORY, "Open VKL was unable to allocate memory"); \
+    return ret;                                                             \
+  }                                                                         \
+  catch (const std::exception &e)                                           \
+  {                                                                         \
+    report(dev_, VKL_UNKNOWN_ERROR, e.what());                              \
+    return ret;                                                             \
+  }
+
+ManagedObject *objectOf(VKLObject o)
+{
+  if (!o.host) throw VklError("null object handle");
+  return static_cast<ManagedObject *>(o.host);
+}
+
+Device *deviceOf(VKLObject o) { return objectOf(o)->device; }
+
+template <typename T>
+T *as(VKLObject o, const char *what)
+{
+  T *p = dynamic_cast<T *>(objectOf(o));
+  if (!p) throw VklError(std::string("object is not a ") + what);
+  return p;
+}
+
+VKLObject handle(ManagedObject *o, void *devicePtr = nullptr)
+{
+  VKLObject h;
+  h.host   = o;
+  h.device = devicePtr;
+  return h;
+}
+
+const float kNaN = std::numeric_limits<float>::quiet_NaN();
+
+// Device-side view of a caller pointer: device memory is used in place; host
+// memory is staged through a stream-ordered allocation.
+struct Staged
+{
+  void *dev        = nullptr;
+  bool owned       = false;
+  const void *host = nullptr;
+  size_t bytes     = 0;
+  hipStream_t st   = nullptr;
+
+  Staged(const void *p, size_t n, hipStream_t s, bool copyIn) : host(p), bytes(n), st(s)
+  {
+    if (!p || n == 0) return;
+    if (isDevicePointer(p)) {
+      dev = const_cast<void *>(p);
+      return;
+    }
+    checkHip(hipMallocAsync(&dev, n, st), "hipMallocAsync");
+    owned = true;
+    if (copyIn) checkHip(hipMemcpyAsync(dev, p, n, hipMemcpyHostToDevice, st), "hipMemcpyAsync");
+  }
+  void copyOut()
+  {
+    if (owned) checkHip(hipMemcpyAsync(const_cast<void *>(host), dev, bytes, hipMemcpyDeviceToHost, st), "hipMemcpyAsync");
+  }
+  ~Staged()
+  {
+    if (owned) (void)hipFreeAsync(dev, st);
+  }
+  template <typename T>
+  T *as() const { return static_cast<T *>(dev); }
+};
+
+void syncDevice(Device *d) { checkHip(hipStreamSynchronize(d->stream), "hipStreamSynchronize"); }
+
+Sampler *samplerOf(const VKLSampler *s)
+{
+  if (!s || !s->host) throw VklError("null sampler");
+  Sampler *sp = static_cast<Sampler *>(s->host);
+  if (!sp->committed) throw VklError("sampler is not committed");
+  return sp;
+}
+
+void sampleN(Sampler *s, uint64_t n, const vkl_vec3f *oc, float *out, uint32_t attr)
+{
+  Device *d = s->device;
+  d->makeCurrent();
+  const AttrView &a = s->attr(attr);
+  if (n == 0) return;
+  Staged in(oc, n * 12, d->stream, true);
+  Staged o(out, n * 4, d->stream, false);
+  checkHip(launchSample(s->view, a, attr, n, in.as<float>(), o.as<float>(), d->stream), "sample kernel");
+  o.copyOut();
+  syncDevice(d);
+}
+
+void gradientN(Sampler *s, uint64_t n, const vkl_vec3f *oc, vkl_vec3f *out, uint32_t attr)
+{
+  Device *d = s->device;
+  d->makeCurrent();
+  const AttrView &a = s->attr(attr);
+  if (n == 0) return;
+  Staged in(oc, n * 12, d->stream, true);
+  Staged o(out, n * 12, d->stream, false);
+  checkHip(launchGradient(s->view, a, attr, n, in.as<float>(), o.as<float>(), d->stream), "gradient kernel");
+  o.copyOut();
+  syncDevice(d);
+}
+
+void sampleMN(Sampler *s, uint64_t n, const vkl_vec3f *oc, float *out, uint32_t m, const unsigned int *attrs)
+{
+  Device *d = s->device;
+  d->makeCurrent();
+  if (n == 0 || m == 0) return;
+  for (uint32_t k = 0; k < m; k++) (void)s->attr(attrs[k]);
+  Staged in(oc, n * 12, d->stream, true);
+  Staged at(attrs, m * 4, d->stream, true);
+  Staged o(out, n * m * 4, d->stream, false);
+  checkHip(launchSampleM(s->view, m, at.as<uint32_t>(), n, in.as<float>(), o.as<float>(), d->stream), "sampleM kernel");
+  o.copyOut();
+  syncDevice(d);
+}
+
+// SoA (width W) -> AoS for the valid lanes.
+template <int W, typename VV>
+std::vector<vkl_vec3f> gatherLanes(const int *valid, const VV *v, std::vector<int> &lanes)
+{
+  std::vector<vkl_vec3f> p;
+  for (int i = 0; i < W; i++)
+    if (valid[i]) {
+      lanes.push_back(i);
+      p.push_back({v->x[i], v->y[i], v->z[i]});
+    }
+  return p;
+}
+
+template <int W, typename VV>
+void sampleW(const int *valid, const VKLSampler *sampler, const VV *oc, float *samples, unsigned attr)
+{
+  Sampler *s = samplerOf(sampler);
+  std::vector<int> lanes;
+  std::vector<vkl_vec3f> p = gatherLanes<W>(valid, oc, lanes);
+  std::vector<float> r(p.size());
+  sampleN(s, p.size(), p.data(), r.data(), attr);
+  for (size_t k = 0; k < lanes.size(); k++) samples[lanes[k]] = r[k];
+}
+
+template <int W, typename VV>
+void sampleMW(const int *valid, const VKLSampler *sampler, const VV *oc, float *samples, unsigned m,
+              const unsigned *attrs)
+{
+  Sampler *s = samplerOf(sampler);
+  std::vector<int> lanes;
+  std::vector<vkl_vec3f> p = gatherLanes<W>(valid, oc, lanes);
+  std::vector<float> r(p.size() * m);
+  sampleMN(s, p.size(), p.data(), r.data(), m, attrs);
+  // Sampler.h:226-244: samples[a * W + lane] for the vector calls.
+  for (size_t k = 0; k < lanes.size(); k++)
+    for (unsigned a = 0; a < m; a++) samples[a * W + lanes[k]] = r[k * m + a];
+}
+
+template <int W, typename VV>
+void gradientW(const int *valid, const VKLSampler *sampler, const VV *oc, VV *out, unsigned attr)
+{
+  Sampler *s = samplerOf(sampler);
+  std::vector<int> lanes;
+  std::vector<vkl_vec3f> p = gatherLanes<W>(valid, oc, lanes);
+  std::vector<vkl_vec3f> r(p.size());
+  gradientN(s, p.size(), p.data(), r.data(), attr);
+  for (size_t k = 0; k < lanes.size(); k++) {
+    out->x[lanes[k]] = r[k].x;
+    out->y[lanes[k]] = r[k].y;
+    out->z[lanes[k]] = r[k].z;
+  }
+}
+
+// ---- scalar iterators: state in the caller's buffer, stepped on the GPU ----
+
+constexpr uint32_t kIntervalMagic = 0x564b4c49u;  // "VKLI"
+constexpr uint32_t kHitMagic      = 0x564b4c48u;  // "VKLH"
+
+struct IntervalBuffer
+{
+  IteratorContext *ctx;
+  uint32_t magic;
+  uint32_t pad;
+  IntervalState state;
+};
+
+struct HitBuffer
+{
+  IteratorContext *ctx;
+  uint32_t magic;
+  uint32_t pad;
+  HitState state;
+};
+
+static_assert(sizeof(IntervalBuffer) <= VKL_MAX_INTERVAL_ITERATOR_SIZE, "interval iterator too large");
+static_assert(sizeof(HitBuffer) <= VKL_MAX_HIT_ITERATOR_SIZE, "hit iterator too large");
+
+// Per-thread device scratch for single-ray steps (4 KiB, reused).
+struct StepScratch
+{
+  int device = -1;
+  void *ptr  = nullptr;
+};
+thread_local StepScratch t_scratch;
+
+uint8_t *stepScratch(Device *d)
+{
+  if (t_scratch.device != d->hipDevice || !t_scratch.ptr) {
+    checkHip(hipMalloc(&t_scratch.ptr, 4096), "hipMalloc");
+    t_scratch.device = d->hipDevice;
+  }
+  return static_cast<uint8_t *>(t_scratch.ptr);
+}
+
+IteratorContext *contextOf(const VKLObject *c, bool hit)
+{
+  if (!c || !c->host) throw VklError("null iterator context");
+  IteratorContext *ctx = static_cast<IteratorContext *>(c->host);
+  if (ctx->isHit != hit) throw VklError("wrong iterator context type");
+  if (!ctx->committed) throw VklError("iterator context is not committed");
+  return ctx;
+}
+
+void initInterval(IteratorContext *ctx, const vkl_vec3f *o, const vkl_vec3f *dir, const vkl_range1f *t, float time,
+                  IntervalBuffer *buf)
+{
+  Device *d = ctx->device;
+  d->makeCurrent();
+  uint8_t *s  = stepScratch(d);
+  float ray[9] = {o->x, o->y, o->z, dir->x, dir->y, dir->z, t ? t->lower : 0.f,
+                  t ? t->upper : std::numeric_limits<float>::infinity(), time};
+  checkHip(hipMemcpyAsync(s, ray, sizeof(ray), hipMemcpyHostToDevice, d->stream), "hipMemcpyAsync");
+  checkHip(launchIntervalInit(ctx->view, reinterpret_cast<IntervalState *>(s + 64), reinterpret_cast<float *>(s),
+                              d->stream),
+           "interval init");
+  checkHip(hipMemcpyAsync(&buf->state, s + 64, sizeof(IntervalState), hipMemcpyDeviceToHost, d->stream), "copy");
+  syncDevice(d);
+  buf->ctx   = ctx;
+  buf->magic = kIntervalMagic;
+}
+
+int iterateInterval(IntervalBuffer *buf, VKLInterval *out)
+{
+  if (buf->magic != kIntervalMagic) throw VklError("invalid interval iterator");
+  IteratorContext *ctx = buf->ctx;
+  Device *d            = ctx->device;
+  d->makeCurrent();
+  uint8_t *s = stepScratch(d);
+  // Layout: [0,64) result, [64, ...) state, [1024, ...) interval.
+  checkHip(hipMemcpyAsync(s + 64, &buf->state, sizeof(IntervalState), hipMemcpyHostToDevice, d->stream), "copy");
+  checkHip(launchIntervalNext(ctx->view, reinterpret_cast<IntervalState *>(s + 64),
+                              reinterpret_cast<IntervalOut *>(s + 1024), reinterpret_cast<int *>(s), d->stream),
+           "interval step");
+  struct
+  {
+    int result;
+    IntervalOut out;
+  } r;
+  checkHip(hipMemcpyAsync(&buf->state, s + 64, sizeof(IntervalState), hipMemcpyDeviceToHost, d->stream), "copy");
+  checkHip(hipMemcpyAsync(&r.result, s, 4, hipMemcpyDeviceToHost, d->stream), "copy");
+  checkHip(hipMemcpyAsync(&r.out, s + 1024, sizeof(IntervalOut), hipMemcpyDeviceToHost, d->stream), "copy");
+  syncDevice(d);
+  if (r.result) std::memcpy(out, &r.out, sizeof(VKLInterval));
+  return r.result;
+}
+
+void initHit(IteratorContext *ctx, const vkl_vec3f *o, const vkl_vec3f *dir, const vkl_range1f *t, float time,
+             HitBuffer *buf)
+{
+  Device *d = ctx->device;
+  d->makeCurrent();
+  uint8_t *s   = stepScratch(d);
+  float ray[9] = {o->x, o->y, o->z, dir->x, dir->y, dir->z, t ? t->lower : 0.f,
+                  t ? t->upper : std::numeric_limits<float>::infinity(), time};
+  checkHip(hipMemcpyAsync(s, ray, sizeof(ray), hipMemcpyHostToDevice, d->stream), "hipMemcpyAsync");
+  checkHip(launchHitInit(ctx->view, reinterpret_cast<HitState *>(s + 64), reinterpret_cast<float *>(s), d->stream),
+           "hit init");
+  checkHip(hipMemcpyAsync(&buf->state, s + 64, sizeof(HitState), hipMemcpyDeviceToHost, d->stream), "copy");
+  syncDevice(d);
+  buf->ctx   = ctx;
+  buf->magic = kHitMagic;
+}
+
+int iterateHit(HitBuffer *buf, VKLHit *out)
+{
+  if (buf->magic != kHitMagic) throw VklError("invalid hit iterator");
+  IteratorContext *ctx = buf->ctx;
+  Device *d            = ctx->device;
+  d->makeCurrent();
+  uint8_t *s = stepScratch(d);
+  checkHip(hipMemcpyAsync(s + 64, &buf->state, sizeof(HitState), hipMemcpyHostToDevice, d->stream), "copy");
+  checkHip(launchHitNext(ctx->view, reinterpret_cast<HitState *>(s + 64), reinterpret_cast<HitOut *>(s + 1024),
+                         reinterpret_cast<int *>(s), d->stream),
+           "hit step");
+  int result = 0;
+  HitOut h;
+  checkHip(hipMemcpyAsync(&buf->state, s + 64, sizeof(HitState), hipMemcpyDeviceToHost, d->stream), "copy");
+  checkHip(hipMemcpyAsync(&result, s, 4, hipMemcpyDeviceToHost, d->stream), "copy");
+  checkHip(hipMemcpyAsync(&h, s + 1024, sizeof(HitOut), hipMemcpyDeviceToHost, d->stream), "copy");
+  syncDevice(d);
+  if (result) std::memcpy(out, &h, sizeof(VKLHit));
+  return result;
+}
+
+template <int W, typename VV, typename VR>
+void initIntervalW(const int *valid, const VKLIntervalIteratorContext *c, const VV *o, const VV *dir, const VR *t,
+                   const float *times, void *buffer)
+{
+  IteratorContext *ctx = contextOf(c, false);
+  IntervalBuffer *b    = static_cast<IntervalBuffer *>(buffer);
+  for (int i = 0; i < W; i++) {
+    if (!valid[i]) {
+      b[i].magic = 0;
+      continue;
+    }
+    const vkl_vec3f oi = {o->x[i], o->y[i], o->z[i]}, di = {dir->x[i], dir->y[i], dir->z[i]};
+    const vkl_range1f ti = {t->lower[i], t->upper[i]};
+    initInterval(ctx, &oi, &di, &ti, times ? times[i] : 0.f, &b[i]);
+  }
+}
+
+template <int W, typename VI>
+void iterateIntervalW(const int *valid, void *it, VI *out, int *result)
+{
+  IntervalBuffer *b = static_cast<IntervalBuffer *>(it);
+  for (int i = 0; i < W; i++) {
+    result[i] = 0;
+    if (!valid[i] || b[i].magic != kIntervalMagic) continue;
+    VKLInterval iv;
+    result[i] = iterateInterval(&b[i], &iv);
+    if (result[i]) {
+      out->tRange.lower[i]     = iv.tRange.lower;
+      out->tRange.upper[i]     = iv.tRange.upper;
+      out->valueRange.lower[i] = iv.valueRange.lower;
+      out->valueRange.upper[i] = iv.valueRange.upper;
+      out->nominalDeltaT[i]    = iv.nominalDeltaT;
+    }
+  }
+}
+
+template <int W, typename VV, typename VR>
+void initHitW(const int *valid, const VKLHitIteratorContext *c, const VV *o, const VV *dir, const VR *t,
+              const float *times, void *buffer)
+{
+  IteratorContext *ctx = contextOf(c, true);
+  HitBuffer *b         = static_cast<HitBuffer *>(buffer);
+  for (int i = 0; i < W; i++) {
+    if (!valid[i]) {
+      b[i].magic = 0;
+      continue;
+    }
+    const vkl_vec3f oi = {o->x[i], o->y[i], o->z[i]}, di = {dir->x[i], dir->y[i], dir->z[i]};
+    const vkl_range1f ti = {t->lower[i], t->upper[i]};
+    initHit(ctx, &oi, &di, &ti, times ? times[i] : 0.f, &b[i]);
+  }
+}
+
+template <int W, typename VH>
+void iterateHitW(const int *valid, void *it, VH *out, int *result)
+{
+  HitBuffer *b = static_cast<HitBuffer *>(it);
+  for (int i = 0; i < W; i++) {
+    result[i] = 0;
+    if (!valid[i] || b[i].magic != kHitMagic) continue;
+    VKLHit h;
+    result[i] = iterateHit(&b[i], &h);
+    if (result[i]) {
+      out->t[i]       = h.t;
+      out->sample[i]  = h.sample;
+      out->epsilon[i] = h.epsilon;
+    }
+  }
+}
+
+}  // namespace
+
+// =====================================================================================
+// C ABI
+// =====================================================================================
+
+extern "C" {
+
+// ---- device -------------------------------------------------------------------------
+
+OPENVKL_INTERFACE void vklInit(void) {}
+
+OPENVKL_INTERFACE VKLDevice vklNewDevice(const char *name)
+{
+  VKL_TRY(nullptr)
+  if (!name) throw VklError("null deviceName provided to vklNewDevice");
+  const std::string n(name);
+  if (n != "hip" && n != "default")
+    throw VklError("unknown device '" + n + "' (this library provides the \"hip\" device)");
+  return reinterpret_cast<VKLDevice>(new Device());
+  VKL_CATCH(nullptr)
+}
+
+#define DEV(d) reinterpret_cast<Device *>(d)
+
+OPENVKL_INTERFACE void vklDeviceSetLogCallback(VKLDevice d, VKLLogCallback cb, void *user)
+{
+  if (!d) return;
+  DEV(d)->logCallback = cb;
+  DEV(d)->logUserData = user;
+}
+
+OPENVKL_INTERFACE void vklDeviceSetErrorCallback(VKLDevice d, VKLErrorCallback cb, void *user)
+{
+  if (!d) return;
+  DEV(d)->errorCallback = cb;
+  DEV(d)->errorUserData = user;
+}
+
+OPENVKL_INTERFACE void vklDeviceSetInt(VKLDevice d, const char *name, int x)
+{
+  if (!d || !name) return;
+  ManagedObject::Param p;
+  p.type = VKL_INT;
+  p.bytes.assign(reinterpret_cast<uint8_t *>(&x), reinterpret_cast<uint8_t *>(&x) + 4);
+  DEV(d)->params[name] = p;
+}
+
+OPENVKL_INTERFACE void vklDeviceSetString(VKLDevice d, const char *name, const char *s)
+{
+  if (!d || !name || !s) return;
+  ManagedObject::Param p;
+  p.type = VKL_STRING;
+  p.str  = s;
+  DEV(d)->params[name] = p;
+}
+
+OPENVKL_INTERFACE void vklDeviceSetVoidPtr(VKLDevice d, const char *name, const void *a)
+{
+  if (!d || !name) return;
+  ManagedObject::Param p;
+  p.type = VKL_VOID_PTR;
+  p.bytes.assign(reinterpret_cast<const uint8_t *>(&a), reinterpret_cast<const uint8_t *>(&a) + sizeof(a));
+  DEV(d)->params[name] = p;
+}
+
+OPENVKL_INTERFACE void vklCommitDevice(VKLDevice d)
+{
+  VKL_TRY(DEV(d))
+  if (!dev_) throw VklError("null device");
+  dev_->commit();
+  VKL_CATCH()
+}
+
+OPENVKL_INTERFACE VKLError vklDeviceGetLastErrorCode(VKLDevice d) { return d ? DEV(d)->lastErrorCode : VKL_INVALID_ARGUMENT; }
+
+OPENVKL_INTERFACE const char *vklDeviceGetLastErrorMsg(VKLDevice d)
+{
+  return d ? DEV(d)->lastErrorMessage.c_str() : "null device";
+}
+
+OPENVKL_INTERFACE int vklGetNativeSIMDWidth(VKLDevice d) { return d ? 64 : 0; }
+
+OPENVKL_INTERFACE void vklReleaseDevice(VKLDevice d)
+{
+  if (d && --DEV(d)->refs == 0) delete DEV(d);
+}
+
+// ---- objects --------------------------------------------------------------------------
+
+OPENVKL_INTERFACE void vklCommit(VKLObject o)
+{
+  VKL_TRY(deviceOf(o))
+  objectOf(o)->commit();
+  VKL_CATCH()
+}
+
+OPENVKL_INTERFACE void vklRelease(VKLObject o)
+{
+  VKL_TRY(deviceOf(o))
+  objectOf(o)->release();
+  VKL_CATCH()
+}
+
+OPENVKL_INTERFACE VKLData vklNewData(VKLDevice d, size_t numItems, VKLDataType type, const void *source,
+                                     VKLDataCreationFlags flags, size_t byteStride)
+{
+  VKL_TRY(DEV(d))
+  if (!dev_) throw VklError("null device provided to vklNewData");
+  if (!dev_->committed) throw VklError("You must commit the device before using it!");
+  dev_->makeCurrent();
+  Data *data = new Data(dev_, numItems, type, source, flags, byteStride);
+  VKLData h;
+  h.host   = data;
+  h.device = nullptr;
+  return h;
+  VKL_CATCH(VKLData())
+}
+
+OPENVKL_INTERFACE VKLVolume vklNewVolume(VKLDevice d, const char *type)
+{
+  VKL_TRY(DEV(d))
+  if (!dev_) throw VklError("null device provided to vklNewVolume");
+  if (!dev_->committed) throw VklError("You must commit the device before using it!");
+  if (!type) throw VklError("null type provided to vklNewVolume");
+  const std::string t(type);
+  Volume *v = nullptr;
+  if (t == "structuredRegular")
+    v = new DenseVolume(dev_);
+  else if (t == "vdb")
+    v = new VdbVolume(dev_);
+  else
+    throw VklError("could not create volume of type '" + t + "' (the hip device supports structuredRegular, vdb)");
+  VKLVolume h;
+  h.host   = v;
+  h.device = nullptr;
+  return h;
+  VKL_CATCH(VKLVolume())
+}
+
+OPENVKL_INTERFACE vkl_box3f vklGetBoundingBox(VKLVolume v)
+{
+  vkl_box3f b;
+  std::memset(&b, 0, sizeof(b));
+  VKL_TRY(deviceOf(v))
+  Volume *vol = as<Volume>(v, "volume");
+  std::memcpy(&b, vol->bbox, sizeof(b));
+  return b;
+  VKL_CATCH(b)
+}
+
+OPENVKL_INTERFACE void vklGetBoundingBoxRef(const VKLVolume *v, vkl_box3f *b) { *b = vklGetBoundingBox(*v); }
+
+OPENVKL_INTERFACE unsigned int vklGetNumAttributes(VKLVolume v)
+{
+  VKL_TRY(deviceOf(v))
+  return as<Volume>(v, "volume")->numAttributes();
+  VKL_CATCH(0)
+}
+
+OPENVKL_INTERFACE unsigned int vklGetNumAttributesRef(const VKLVolume *v) { return vklGetNumAttributes(*v); }
+
+OPENVKL_INTERFACE vkl_range1f vklGetValueRange(VKLVolume v, unsigned int a)
+{
+  vkl_range1f r = {kNaN, kNaN};
+  VKL_TRY(deviceOf(v))
+  Volume *vol = as<Volume>(v, "volume");
+  if (a >= vol->valueRanges.size()) throw VklError("illegal attributeIndex requested on volume");
+  r.lower = vol->valueRanges[a].lower;
+  r.upper = vol->valueRanges[a].upper;
+  return r;
+  VKL_CATCH(r)
+}
+
+OPENVKL_INTERFACE void vklGetValueRangeRef(const VKLVolume *v, unsigned int a, vkl_range1f *r)
+{
+  *r = vklGetValueRange(*v, a);
+}
+
+OPENVKL_INTERFACE VKLSampler vklNewSampler(VKLVolume v)
+{
+  VKL_TRY(deviceOf(v))
+  Volume *vol = as<Volume>(v, "volume");
+  Sampler *s  = new Sampler(dev_, vol);
+  VKLSampler h;
+  h.host   = s;
+  h.device = s->deviceView.ptr;
+  return h;
+  VKL_CATCH(VKLSampler())
+}
+
+OPENVKL_INTERFACE VKLFeatureFlags vklGetFeatureFlags(VKLSampler) { return (VKLFeatureFlags)VKL_FEATURE_FLAGS_DEFAULT; }
+
+OPENVKL_INTERFACE VKLIntervalIteratorContext vklNewIntervalIteratorContext(VKLSampler s)
+{
+  VKL_TRY(deviceOf(s))
+  IteratorContext *c = new IteratorContext(dev_, as<Sampler>(s, "sampler"), false);
+  VKLIntervalIteratorContext h;
+  h.host   = c;
+  h.device = nullptr;
+  return h;
+  VKL_CATCH(VKLIntervalIteratorContext())
+}
+
+OPENVKL_INTERFACE VKLHitIteratorContext vklNewHitIteratorContext(VKLSampler s)
+{
+  VKL_TRY(deviceOf(s))
+  IteratorContext *c = new IteratorContext(dev_, as<Sampler>(s, "sampler"), true);
+  VKLHitIteratorContext h;
+  h.host   = c;
+  h.device = nullptr;
+  return h;
+  VKL_CATCH(VKLHitIteratorContext())
+}
+
+// ---- parameters ------------------------------------------------------------------------
+
+OPENVKL_INTERFACE void vklSetParam(VKLObject o, const char *name, VKLDataType type, const void *mem)
+{
+  VKL_TRY(deviceOf(o))
+  if (!name) throw VklError("null parameter name");
+  objectOf(o)->setParam(name, type, mem);
+  VKL_CATCH()
+}
+
+OPENVKL_INTERFACE void vklSetBool(VKLObject o, const char *name, int b)
+{
+  const bool v = b != 0;
+  vklSetParam(o, name, VKL_BOOL, &v);
+}
+OPENVKL_INTERFACE void vklSetFloat(VKLObject o, const char *name, float x) { vklSetParam(o, name, VKL_FLOAT, &x); }
+OPENVKL_INTERFACE void vklSetInt(VKLObject o, const char *name, int x) { vklSetParam(o, name, VKL_INT, &x); }
+OPENVKL_INTERFACE void vklSetVec3f(VKLObject o, const char *name, float x, float y, float z)
+{
+  const float v[3] = {x, y, z};
+  vklSetParam(o, name, VKL_VEC3F, v);
+}
+OPENVKL_INTERFACE void vklSetVec3i(VKLObject o, const char *name, int x, int y, int z)
+{
+  const int v[3] = {x, y, z};
+  vklSetParam(o, name, VKL_VEC3I, v);
+}
+OPENVKL_INTERFACE void vklSetData(VKLObject o, const char *name, VKLData d) { vklSetParam(o, name, VKL_DATA, &d); }
+OPENVKL_INTERFACE void vklSetString(VKLObject o, const char *name, const char *s) { vklSetParam(o, name, VKL_STRING, s); }
+OPENVKL_INTERFACE void vklSetVoidPtr(VKLObject o, const char *name, void *v) { vklSetParam(o, name, VKL_VOID_PTR, &v); }
+
+// ---- sampling ----------------------------------------------------------------------------
+
+OPENVKL_INTERFACE float vklComputeSample(const VKLSampler *s, const vkl_vec3f *oc, unsigned int a, float)
+{
+  VKL_TRY(samplerOf(s)->device)
+  float r = kNaN;
+  sampleN(samplerOf(s), 1, oc, &r, a);
+  return r;
+  VKL_CATCH(kNaN)
+}
+
+OPENVKL_INTERFACE void vklComputeSample4(const int *valid, const VKLSampler *s, const vkl_vvec3f4 *oc, float *out,
+                                         unsigned int a, const float *)
+{
+  VKL_TRY(samplerOf(s)->device)
+  sampleW<4>(valid, s, oc, out, a);
+  VKL_CATCH()
+}
+OPENVKL_INTERFACE void vklComputeSample8(const int *valid, const VKLSampler *s, const vkl_vvec3f8 *oc, float *out,
+                                         unsigned int a, const float *)
+{
+  VKL_TRY(samplerOf(s)->device)
+  sampleW<8>(valid, s, oc, out, a);
+  VKL_CATCH()
+}
+OPENVKL_INTERFACE void vklComputeSample16(const int *valid, const VKLSampler *s, const vkl_vvec3f16 *oc, float *out,
+                                          unsigned int a, const float *)
+{
+  VKL_TRY(samplerOf(s)->device)
+  sampleW<16>(valid, s, oc, out, a);
+  VKL_CATCH()
+}
+
+OPENVKL_INTERFACE void vklComputeSampleN(const VKLSampler *s, unsigned int N, const vkl_vec3f *oc, float *out,
+                                         unsigned int a, const float *)
+{
+  VKL_TRY(samplerOf(s)->device)
+  sampleN(samplerOf(s), N, oc, out, a);
+  VKL_CATCH()
+}
+
+OPENVKL_INTERFACE void vklComputeSampleM(const VKLSampler *s, const vkl_vec3f *oc, float *out, unsigned int M,
+                                         const unsigned int *attrs, float)
+{
+  VKL_TRY(samplerOf(s)->device)
+  sampleMN(samplerOf(s), 1, oc, out, M, attrs);
+  VKL_CATCH()
+}
+OPENVKL_INTERFACE void vklComputeSampleM4(const int *valid, const VKLSampler *s, const vkl_vvec3f4 *oc, float *out,
+                                          unsigned int M, const unsigned int *attrs, const float *)
+{
+  VKL_TRY(samplerOf(s)->device)
+  sampleMW<4>(valid, s, oc, out, M, attrs);
+  VKL_CATCH()
+}
+OPENVKL_INTERFACE void vklComputeSampleM8(const int *valid, const VKLSampler *s, const vkl_vvec3f8 *oc, float *out,
+                                          unsigned int M, const unsigned int *attrs, const float *)
+{
+  VKL_TRY(samplerOf(s)->device)
+  sampleMW<8>(valid, s, oc, out, M, attrs);
+  VKL_CATCH()
+}
+OPENVKL_INTERFACE void vklComputeSampleM16(const int *valid, const VKLSampler *s, const vkl_vvec3f16 *oc, float *out,
+                                           unsigned int M, const unsigned int *attrs, const float *)
+{
+  VKL_TRY(samplerOf(s)->device)
+  sampleMW<16>(valid, s, oc, out, M, attrs);
+  VKL_CATCH()
+}
+OPENVKL_INTERFACE void vklComputeSampleMN(const VKLSampler *s, unsigned int N, const vkl_vec3f *oc, float *out,
+                                          unsigned int M, const unsigned int *attrs, const float *)
+{
+  VKL_TRY(samplerOf(s)->device)
+  sampleMN(samplerOf(s), N, oc, out, M, attrs);
+  VKL_CATCH()
+}
+
+OPENVKL_INTERFACE vkl_vec3f vklComputeGradient(const VKLSampler *s, const vkl_vec3f *oc, unsigned int a, float)
+{
+  vkl_vec3f r = {kNaN, kNaN, kNaN};
+  VKL_TRY(samplerOf(s)->device)
+  gradientN(samplerOf(s), 1, oc, &r, a);
+  return r;
+  VKL_CATCH(r)
+}
+OPENVKL_INTERFACE void vklComputeGradient4(const int *valid, const VKLSampler *s, const vkl_vvec3f4 *oc,
+                                           vkl_vvec3f4 *g, unsigned int a, const float *)
+{
+  VKL_TRY(samplerOf(s)->device)
+  gradientW<4>(valid, s, oc, g, a);
+  VKL_CATCH()
+}
+OPENVKL_INTERFACE void vklComputeGradient8(const int *valid, const VKLSampler *s, const vkl_vvec3f8 *oc,
+                                           vkl_vvec3f8 *g, unsigned int a, const float *)
+{
+  VKL_TRY(samplerOf(s)->device)
+  gradientW<8>(valid, s, oc, g, a);
+  VKL_CATCH()
+}
+OPENVKL_INTERFACE void vklComputeGradient16(const int *valid, const VKLSampler *s, const vkl_vvec3f16 *oc,
+                                            vkl_vvec3f16 *g, unsigned int a, const float *)
+{
+  VKL_TRY(samplerOf(s)->device)
+  gradientW<16>(valid, s, oc, g, a);
+  VKL_CATCH()
+}
+OPENVKL_INTERFACE void vklComputeGradientN(const VKLSampler *s, unsigned int N, const vkl_vec3f *oc, vkl_vec3f *g,
+                                           unsigned int a, const float *)
+{
+  VKL_TRY(samplerOf(s)->device)
+  gradientN(samplerOf(s), N, oc, g, a);
+  VKL_CATCH()
+}
+
+// ---- interval iterators ---------------------------------------------------------------------
+
+OPENVKL_INTERFACE size_t vklGetIntervalIteratorSize(const VKLIntervalIteratorContext *) { return sizeof(IntervalBuffer); }
+OPENVKL_INTERFACE size_t vklGetIntervalIteratorSize4(const VKLIntervalIteratorContext *) { return 4 * sizeof(IntervalBuffer); }
+OPENVKL_INTERFACE size_t vklGetIntervalIteratorSize8(const VKLIntervalIteratorContext *) { return 8 * sizeof(IntervalBuffer); }
+OPENVKL_INTERFACE size_t vklGetIntervalIteratorSize16(const VKLIntervalIteratorContext *) { return 16 * sizeof(IntervalBuffer); }
+
+OPENVKL_INTERFACE VKLIntervalIterator vklInitIntervalIterator(const VKLIntervalIteratorContext *c, const vkl_vec3f *o,
+                                                              const vkl_vec3f *d, const vkl_range1f *t, float time,
+                                                              void *buffer)
+{
+  VKL_TRY(contextOf(c, false)->device)
+  if (!buffer) throw VklError("null iterator buffer");
+  initInterval(contextOf(c, false), o, d, t, time, static_cast<IntervalBuffer *>(buffer));
+  return reinterpret_cast<VKLIntervalIterator>(buffer);
+  VKL_CATCH(nullptr)
+}
+
+OPENVKL_INTERFACE VKLIntervalIterator4 vklInitIntervalIterator4(const int *valid, const VKLIntervalIteratorContext *c,
+                                                                const vkl_vvec3f4 *o, const vkl_vvec3f4 *d,
+                                                                const vkl_vrange1f4 *t, const float *times, void *buffer)
+{
+  VKL_TRY(contextOf(c, false)->device)
+  initIntervalW<4>(valid, c, o, d, t, times, buffer);
+  return reinterpret_cast<VKLIntervalIterator4>(buffer);
+  VKL_CATCH(nullptr)
+}
+OPENVKL_INTERFACE VKLIntervalIterator8 vklInitIntervalIterator8(const int *valid, const VKLIntervalIteratorContext *c,
+                                                                const vkl_vvec3f8 *o, const vkl_vvec3f8 *d,
+                                                                const vkl_vrange1f8 *t, const float *times, void *buffer)
+{
+  VKL_TRY(contextOf(c, false)->device)
+  initIntervalW<8>(valid, c, o, d, t, times, buffer);
+  return reinterpret_cast<VKLIntervalIterator8>(buffer);
+  VKL_CATCH(nullptr)
+}
+OPENVKL_INTERFACE VKLIntervalIterator16 vklInitIntervalIterator16(const int *valid, const VKLIntervalIteratorContext *c,
+                                                                  const vkl_vvec3f16 *o, const vkl_vvec3f16 *d,
+                                                                  const vkl_vrange1f16 *t, const float *times,
+                                                                  void *buffer)
+{
+  VKL_TRY(contextOf(c, false)->device)
+  initIntervalW<16>(valid, c, o, d, t, times, buffer);
+  return reinterpret_cast<VKLIntervalIterator16>(buffer);
+  VKL_CATCH(nullptr)
+}
+
+OPENVKL_INTERFACE int vklIterateInterval(VKLIntervalIterator it, VKLInterval *out)
+{
+  IntervalBuffer *b = reinterpret_cast<IntervalBuffer *>(it);
+  VKL_TRY(b && b->magic == kIntervalMagic ? b->ctx->device : nullptr)
+  if (!dev_) throw VklError("invalid interval iterator");
+  return iterateInterval(b, out);
+  VKL_CATCH(0)
+}
+OPENVKL_INTERFACE void vklIterateInterval4(const int *valid, VKLIntervalIterator4 it, VKLInterval4 *out, int *result)
+{
+  VKL_TRY(nullptr)
+  iterateIntervalW<4>(valid, it, out, result);
+  VKL_CATCH()
+}
+OPENVKL_INTERFACE void vklIterateInterval8(const int *valid, VKLIntervalIterator8 it, VKLInterval8 *out, int *result)
+{
+  VKL_TRY(nullptr)
+  iterateIntervalW<8>(valid, it, out, result);
+  VKL_CATCH()
+}
+OPENVKL_INTERFACE void vklIterateInterval16(const int *valid, VKLIntervalIterator16 it, VKLInterval16 *out, int *result)
+{
+  VKL_TRY(nullptr)
+  iterateIntervalW<16>(valid, it, out, result);
+  VKL_CATCH()
+}
+
+// ---- hit iterators ---------------------------------------------------------------------------
+
+OPENVKL_INTERFACE size_t vklGetHitIteratorSize(const VKLHitIteratorContext *) { return sizeof(HitBuffer); }
+OPENVKL_INTERFACE size_t vklGetHitIteratorSize4(const VKLHitIteratorContext *) { return 4 * sizeof(HitBuffer); }
+OPENVKL_INTERFACE size_t vklGetHitIteratorSize8(const VKLHitIteratorContext *) { return 8 * sizeof(HitBuffer); }
+OPENVKL_INTERFACE size_t vklGetHitIteratorSize16(const VKLHitIteratorContext *) { return 16 * sizeof(HitBuffer); }
+
+OPENVKL_INTERFACE VKLHitIterator vklInitHitIterator(const VKLHitIteratorContext *c, const vkl_vec3f *o,
+                                                    const vkl_vec3f *d, const vkl_range1f *t, float time, void *buffer)
+{
+  VKL_TRY(contextOf(c, true)->device)
+  if (!buffer) throw VklError("null iterator buffer");
+  initHit(contextOf(c, true), o, d, t, time, static_cast<HitBuffer *>(buffer));
+  return reinterpret_cast<VKLHitIterator>(buffer);
+  VKL_CATCH(nullptr)
+}
+OPENVKL_INTERFACE VKLHitIterator4 vklInitHitIterator4(const int *valid, const VKLHitIteratorContext *c,
+                                                      const vkl_vvec3f4 *o, const vkl_vvec3f4 *d,
+                                                      const vkl_vrange1f4 *t, const float *times, void *buffer)
+{
+  VKL_TRY(contextOf(c, true)->device)
+  initHitW<4>(valid, c, o, d, t, times, buffer);
+  return reinterpret_cast<VKLHitIterator4>(buffer);
+  VKL_CATCH(nullptr)
+}
+OPENVKL_INTERFACE VKLHitIterator8 vklInitHitIterator8(const int *valid, const VKLHitIteratorContext *c,
+                                                      const vkl_vvec3f8 *o, const vkl_vvec3f8 *d,
+                                                      const vkl_vrange1f8 *t, const float *times, void *buffer)
+{
+  VKL_TRY(contextOf(c, true)->device)
+  initHitW<8>(valid, c, o, d, t, times, buffer);
+  return reinterpret_cast<VKLHitIterator8>(buffer);
+  VKL_CATCH(nullptr)
+}
+OPENVKL_INTERFACE VKLHitIterator16 vklInitHitIterator16(const int *valid, const VKLHitIteratorContext *c,
+                                                        const vkl_vvec3f16 *o, const vkl_vvec3f16 *d,
+                                                        const vkl_vrange1f16 *t, const float *times, void *buffer)
+{
+  VKL_TRY(contextOf(c, true)->device)
+  initHitW<16>(valid, c, o, d, t, times, buffer);
+  return reinterpret_cast<VKLHitIterator16>(buffer);
+  VKL_CATCH(nullptr)
+}
+
+OPENVKL_INTERFACE int vklIterateHit(VKLHitIterator it, VKLHit *out)
+{
+  HitBuffer *b = reinterpret_cast<HitBuffer *>(it);
+  VKL_TRY(b && b->magic == kHitMagic ? b->ctx->device : nullptr)
+  if (!dev_) throw VklError("invalid hit iterator");
+  return iterateHit(b, out);
+  VKL_CATCH(0)
+}
+OPENVKL_INTERFACE void vklIterateHit4(const int *valid, VKLHitIterator4 it, VKLHit4 *out, int *result)
+{
+  VKL_TRY(nullptr)
+  iterateHitW<4>(valid, it, out, result);
+  VKL_CATCH()
+}
+OPENVKL_INTERFACE void vklIterateHit8(const int *valid, VKLHitIterator8 it, VKLHit8 *out, int *result)
+{
+  VKL_TRY(nullptr)
+  iterateHitW<8>(valid, it, out, result);
+  VKL_CATCH()
+}
+OPENVKL_INTERFACE void vklIterateHit16(const int *valid, VKLHitIterator16 it, VKLHit16 *out, int *result)
+{
+  VKL_TRY(nullptr)
+  iterateHitW<16>(valid, it, out, result);
+  VKL_CATCH()
+}
+
+// ---- HIP-device extensions (openvkl/openvkl_hip.h) ----------------------------------------------
+
+OPENVKL_INTERFACE void vklComputeSampleNAsync(const VKLSampler *s, unsigned int N, const vkl_vec3f *oc, float *out,
+                                              unsigned int a, void *stream)
+{
+  VKL_TRY(samplerOf(s)->device)
+  Sampler *sp = samplerOf(s);
+  dev_->makeCurrent();
+  checkHip(launchSample(sp->view, sp->attr(a), a, N, reinterpret_cast<const float *>(oc), out,
+                        static_cast<hipStream_t>(stream)),
+           "sample kernel");
+  VKL_CATCH()
+}
+
+OPENVKL_INTERFACE void vklComputeGradientNAsync(const VKLSampler *s, unsigned int N, const vkl_vec3f *oc,
+                                                vkl_vec3f *out, unsigned int a, void *stream)
+{
+  VKL_TRY(samplerOf(s)->device)
+  Sampler *sp = samplerOf(s);
+  dev_->makeCurrent();
+  checkHip(launchGradient(sp->view, sp->attr(a), a, N, reinterpret_cast<const float *>(oc),
+                          reinterpret_cast<float *>(out), static_cast<hipStream_t>(stream)),
+           "gradient kernel");
+  VKL_CATCH()
+}
+
+OPENVKL_INTERFACE void vklIterateIntervalsBatch(const VKLIntervalIteratorContext *c, unsigned int numRays,
+                                                const vkl_vec3f *origins, const vkl_vec3f *directions,
+                                                const vkl_range1f *tRanges, const float *times, VKLInterval *intervals,
+                                                unsigned int maxIntervalsPerRay, unsigned int *numIntervals)
+{
+  VKL_TRY(contextOf(c, false)->device)
+  (void)times;
+  IteratorContext *ctx = contextOf(c, false);
+  dev_->makeCurrent();
+  if (numRays == 0) return;
+  hipStream_t st = dev_->stream;
+  Staged o(origins, numRays * 12ull, st, true), d(directions, numRays * 12ull, st, true);
+  Staged t(tRanges, tRanges ? numRays * 8ull : 0, st, true);
+  Staged out(intervals, (uint64_t)numRays * maxIntervalsPerRay * sizeof(VKLInterval), st, false);
+  Staged cnt(numIntervals, numRays * 4ull, st, false);
+  checkHip(launchIntervalBatch(ctx->view, numRays, o.as<float>(), d.as<float>(), t.as<float>(),
+                               out.as<IntervalOut>(), maxIntervalsPerRay, cnt.as<uint32_t>(), st),
+           "interval batch");
+  out.copyOut();
+  cnt.copyOut();
+  syncDevice(dev_);
+  VKL_CATCH()
+}
+
+OPENVKL_INTERFACE void vklIterateHitsBatch(const VKLHitIteratorContext *c, unsigned int numRays,
+                                           const vkl_vec3f *origins, const vkl_vec3f *directions,
+                                           const vkl_range1f *tRanges, const float *times, VKLHit *hits,
+                                           unsigned int maxHitsPerRay, unsigned int *numHits)
+{
+  VKL_TRY(contextOf(c, true)->device)
+  IteratorContext *ctx = contextOf(c, true);
+  dev_->makeCurrent();
+  if (numRays == 0) return;
+  hipStream_t st = dev_->stream;
+  Staged o(origins, numRays * 12ull, st, true), d(directions, numRays * 12ull, st, true);
+  Staged t(tRanges, tRanges ? numRays * 8ull : 0, st, true);
+  Staged tm(times, times ? numRays * 4ull : 0, st, true);
+  Staged out(hits, (uint64_t)numRays * maxHitsPerRay * sizeof(VKLHit), st, false);
+  Staged cnt(numHits, numRays * 4ull, st, false);
+  checkHip(launchHitBatch(ctx->view, numRays, o.as<float>(), d.as<float>(), t.as<float>(), tm.as<float>(),
+                          out.as<HitOut>(), maxHitsPerRay, cnt.as<uint32_t>(), st),
+           "hit batch");
+  out.copyOut();
+  cnt.copyOut();
+  syncDevice(dev_);
+  VKL_CATCH()
+}
+
+OPENVKL_INTERFACE int vklHipGetVolumeLevel(VKLVolume v, unsigned int level, uint64_t *numNodes,
+                                           const uint64_t **voxels, const vkl_range1f **valueRanges)
+{
+  VKL_TRY(deviceOf(v))
+  Volume *vol = as<Volume>(v, "volume");
+  if (level > 2) throw VklError("level must be 0, 1 or 2");
+  if (!vol->committed) throw VklError("volume is not committed");
+  if (numNodes) *numNodes = vol->grid.levels[level].numNodes;
+  if (voxels) *voxels = vol->hostVoxels[level].data();
+  if (valueRanges) *valueRanges = reinterpret_cast<const vkl_range1f *>(vol->hostRanges[level].data());
+  return 1;
+  VKL_CATCH(0)
+}
+
+}  // extern "C"

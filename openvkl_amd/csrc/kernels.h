@@ -1,0 +1,44 @@
+// Host-side launchers of the HIP kernels (implemented in *.hip).
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include "vkl_views.h"
+
+namespace vklhip {
+
+// vklComputeSampleN: out[i] = sample(xyz[i]) of attribute `attr`, whose view
+// (a host copy of s.grid.attrs[attr]) selects the kernel specialisation.
+hipError_t launchSample(const SamplerView &s, const AttrView &a, uint32_t attr, uint64_t n, const float *xyz,
+                        float *out, hipStream_t stream);
+// vklComputeGradientN: out[3i..3i+2] = object-space gradient.
+hipError_t launchGradient(const SamplerView &s, const AttrView &a, uint32_t attr, uint64_t n, const float *xyz,
+                          float *out, hipStream_t stream);
+// vklComputeSampleMN: out[i*m + a] = sample of attribute attrs[a].
+hipError_t launchSampleM(const SamplerView &s, uint32_t m, const uint32_t *attrsDev, uint64_t n, const float *xyz,
+                         float *out, hipStream_t stream);
+
+// Per-leaf value ranges over the [-1, +10) voxel window (majorant build).
+// leafOffsets: 3 u32 root-relative leaf origins per leaf.
+hipError_t launchLeafRanges(const GridView &g, uint32_t attr, uint64_t numLeaves, const uint32_t *leafOffsets,
+                            Range1f *out, hipStream_t stream);
+
+// Single-ray iterator steps for the scalar API (state in device memory).
+// ray: origin (3), direction (3), tRange (2), time (1) in device memory.
+hipError_t launchIntervalInit(const IteratorContextView &c, IntervalState *state, const float *ray, hipStream_t stream);
+hipError_t launchIntervalNext(const IteratorContextView &c, IntervalState *state, IntervalOut *out, int *result,
+                              hipStream_t stream);
+hipError_t launchHitInit(const IteratorContextView &c, HitState *state, const float *ray, hipStream_t stream);
+hipError_t launchHitNext(const IteratorContextView &c, HitState *state, HitOut *out, int *result, hipStream_t stream);
+
+// Batched iteration: one ray per thread; ray r writes up to maxOut results at
+// out[r*maxOut], count[r] = number written. org / dir: 3 floats per ray;
+// tRanges: 2 floats per ray (null = [0, inf)); times may be null.
+hipError_t launchIntervalBatch(const IteratorContextView &c, uint32_t numRays, const float *org, const float *dir,
+                               const float *tRanges, IntervalOut *out, uint32_t maxOut, uint32_t *count,
+                               hipStream_t stream);
+hipError_t launchHitBatch(const IteratorContextView &c, uint32_t numRays, const float *org, const float *dir,
+                          const float *tRanges, const float *times, HitOut *out, uint32_t maxOut, uint32_t *count,
+                          hipStream_t stream);
+
+}  // namespace vklhip

@@ -1,0 +1,221 @@
+// Majorant build and iterator kernels.
+//
+//  * kLeafRanges: per-leaf value range over the filter-support window
+//    (VdbSampler.ispc:460-557), the input of the majorant grid.
+//  * interval / hit iterators: single-ray steps for the per-ray C API and
+//    batched kernels (one ray per work-item) for vklIterate*Batch.
+
+#include "kernels.h"
+#include "vkl_device.hh"
+
+namespace vklhip {
+
+constexpr int kRangeBlock = 128;
+constexpr int kIterBlock  = 64;
+
+// ---- majorants ----------------------------------------------------------------
+
+// Value range of one neighbour node's voxel sub-window (VdbSampler.ispc:383-455).
+__device__ Range1f nodeRange(const GridView &g, const AttrView &a, uint32_t attr, uint32_t ox, uint32_t oy,
+                             uint32_t oz, const uint32_t *rx, const uint32_t *ry, const uint32_t *rz)
+{
+  Range1f r = {vinf(), -vinf()};
+  const bool inDom = ox < g.activeSize[0] && oy < g.activeSize[1] && oz < g.activeSize[2];
+  if (g.dense) {
+    if (!inDom) return {a.background, a.background};
+    for (uint32_t x = rx[0]; x < rx[1]; ++x)
+      for (uint32_t y = ry[0]; y < ry[1]; ++y)
+        for (uint32_t z = rz[0]; z < rz[1]; ++z) {
+          const uint32_t px = ox + x, py = oy + y, pz = oz + z;
+          if (!(px < g.activeSize[0] && py < g.activeSize[1] && pz < g.activeSize[2])) continue;
+          const uint64_t idx = ((uint64_t)pz * g.activeSize[1] + py) * (uint64_t)g.activeSize[0] + px;
+          const float v      = fetchAny(a.data, a.type, idx);
+          r.lower            = vmin(r.lower, v);
+          r.upper            = vmax(r.upper, v);
+        }
+    return r;
+  }
+  uint64_t voxel = 0;
+  uint32_t el = 0, eo = 0;
+  if (inDom) {
+    SparseSource<float> src(g, a, attr);
+    src.maxDepth = 3;
+    voxel        = src.traverse(ox & ~7u, oy & ~7u, oz & ~7u, el, eo);
+  }
+  if ((voxel & 3u) != 3u) {
+    if (!inDom) return {a.background, a.background};
+    return {0.f, 0.f};
+  }
+  const uint64_t leaf = g.leafSlot ? g.leafSlot[voxel >> 6] : (voxel >> 6);
+  if (((voxel >> 2) & 3u) == 0u) {
+    const float v = fetchAny(a.tile, a.type, leaf);
+    return {v, v};
+  }
+  for (uint32_t x = rx[0]; x < rx[1]; ++x)
+    for (uint32_t y = ry[0]; y < ry[1]; ++y)
+      for (uint32_t z = rz[0]; z < rz[1]; ++z) {
+        const float v = fetchAny(a.data, a.type, leaf * 512u + ((x << 6) | (y << 3) | z));
+        r.lower       = vmin(r.lower, v);
+        r.upper       = vmax(r.upper, v);
+      }
+  return r;
+}
+
+// 3x3x3 leaf neighbourhood, windows [7,8), [0,8), [0,2) per axis; NaN ranges
+// (e.g. an undefined background) are skipped.
+__global__ __launch_bounds__(kRangeBlock) void kLeafRanges(GridView g, AttrView a, uint32_t attr,
+                                                            uint64_t numLeaves, const uint32_t *__restrict__ offs,
+                                                            Range1f *__restrict__ out)
+{
+  const uint64_t i = (uint64_t)blockIdx.x * kRangeBlock + threadIdx.x;
+  if (i >= numLeaves) return;
+  const uint32_t cx = offs[3 * i], cy = offs[3 * i + 1], cz = offs[3 * i + 2];
+  const uint32_t win[3][2] = {{7, 8}, {0, 8}, {0, 2}};
+  Range1f range = {vinf(), -vinf()};
+  for (int ix = 0; ix < 3; ++ix)
+    for (int iy = 0; iy < 3; ++iy)
+      for (int iz = 0; iz < 3; ++iz) {
+        const uint32_t ox = cx + (uint32_t)((ix - 1) * 8), oy = cy + (uint32_t)((iy - 1) * 8),
+                       oz = cz + (uint32_t)((iz - 1) * 8);
+        const Range1f nr = nodeRange(g, a, attr, ox, oy, oz, win[ix], win[iy], win[iz]);
+        if (!isnan(nr.lower)) range.lower = vmin(range.lower, nr.lower);
+        if (!isnan(nr.upper)) range.upper = vmax(range.upper, nr.upper);
+      }
+  out[i] = range;
+}
+
+hipError_t launchLeafRanges(const GridView &g, uint32_t attr, uint64_t numLeaves, const uint32_t *leafOffsets,
+                            Range1f *out, hipStream_t stream)
+{
+  if (numLeaves == 0) return hipSuccess;
+  AttrView a;
+  hipError_t e = hipMemcpyAsync(&a, g.attrs + attr, sizeof(a), hipMemcpyDeviceToHost, stream);
+  if (e != hipSuccess) return e;
+  e = hipStreamSynchronize(stream);
+  if (e != hipSuccess) return e;
+  const unsigned blocks = (unsigned)((numLeaves + kRangeBlock - 1) / kRangeBlock);
+  hipLaunchKernelGGL(kLeafRanges, dim3(blocks), dim3(kRangeBlock), 0, stream, g, a, attr, numLeaves, leafOffsets,
+                     out);
+  return hipGetLastError();
+}
+
+// ---- single-ray steps ----------------------------------------------------------
+
+__global__ void kIntervalInit(IteratorContextView c, IntervalState *s, const float *ray)
+{
+  intervalInit(c, *s, F3{ray[0], ray[1], ray[2]}, F3{ray[3], ray[4], ray[5]}, Range1f{ray[6], ray[7]});
+}
+
+__global__ void kIntervalNext(IteratorContextView c, IntervalState *s, IntervalOut *out, int *result)
+{
+  IntervalState st = *s;
+  IntervalOut o;
+  const bool r = intervalNext(c, st, o);
+  *s           = st;
+  if (r) *out = o;
+  *result = r ? 1 : 0;
+}
+
+__global__ void kHitInit(IteratorContextView c, HitState *s, const float *ray)
+{
+  hitInit(c, *s, F3{ray[0], ray[1], ray[2]}, F3{ray[3], ray[4], ray[5]}, Range1f{ray[6], ray[7]}, ray[8]);
+}
+
+__global__ void kHitNext(IteratorContextView c, HitState *s, HitOut *out, int *result)
+{
+  HitState st = *s;
+  HitOut o;
+  const bool r = hitNext(c, st, o);
+  *s           = st;
+  if (r) *out = o;
+  *result = r ? 1 : 0;
+}
+
+hipError_t launchIntervalInit(const IteratorContextView &c, IntervalState *state, const float *ray, hipStream_t st)
+{
+  hipLaunchKernelGGL(kIntervalInit, dim3(1), dim3(1), 0, st, c, state, ray);
+  return hipGetLastError();
+}
+hipError_t launchIntervalNext(const IteratorContextView &c, IntervalState *state, IntervalOut *out, int *result,
+                              hipStream_t st)
+{
+  hipLaunchKernelGGL(kIntervalNext, dim3(1), dim3(1), 0, st, c, state, out, result);
+  return hipGetLastError();
+}
+hipError_t launchHitInit(const IteratorContextView &c, HitState *state, const float *ray, hipStream_t st)
+{
+  hipLaunchKernelGGL(kHitInit, dim3(1), dim3(1), 0, st, c, state, ray);
+  return hipGetLastError();
+}
+hipError_t launchHitNext(const IteratorContextView &c, HitState *state, HitOut *out, int *result, hipStream_t st)
+{
+  hipLaunchKernelGGL(kHitNext, dim3(1), dim3(1), 0, st, c, state, out, result);
+  return hipGetLastError();
+}
+
+// ---- batches ---------------------------------------------------------------------
+
+__device__ inline Range1f rayRange(const float *tRanges, uint32_t r)
+{
+  if (!tRanges) return {0.f, vinf()};
+  return {tRanges[2 * r], tRanges[2 * r + 1]};
+}
+
+__global__ __launch_bounds__(kIterBlock) void kIntervalBatch(IteratorContextView c, uint32_t numRays,
+                                                              const float *__restrict__ org,
+                                                              const float *__restrict__ dir,
+                                                              const float *__restrict__ tRanges,
+                                                              IntervalOut *__restrict__ out, uint32_t maxOut,
+                                                              uint32_t *__restrict__ count)
+{
+  const uint32_t r = blockIdx.x * kIterBlock + threadIdx.x;
+  if (r >= numRays) return;
+  IntervalState st;
+  intervalInit(c, st, F3{org[3 * r], org[3 * r + 1], org[3 * r + 2]}, F3{dir[3 * r], dir[3 * r + 1], dir[3 * r + 2]},
+               rayRange(tRanges, r));
+  uint32_t n = 0;
+  IntervalOut o;
+  IntervalOut *dst = out + (uint64_t)r * maxOut;
+  while (n < maxOut && intervalNext(c, st, o)) dst[n++] = o;
+  count[r] = n;
+}
+
+__global__ __launch_bounds__(kIterBlock) void kHitBatch(IteratorContextView c, uint32_t numRays,
+                                                         const float *__restrict__ org, const float *__restrict__ dir,
+                                                         const float *__restrict__ tRanges,
+                                                         const float *__restrict__ times, HitOut *__restrict__ out,
+                                                         uint32_t maxOut, uint32_t *__restrict__ count)
+{
+  const uint32_t r = blockIdx.x * kIterBlock + threadIdx.x;
+  if (r >= numRays) return;
+  HitState st;
+  hitInit(c, st, F3{org[3 * r], org[3 * r + 1], org[3 * r + 2]}, F3{dir[3 * r], dir[3 * r + 1], dir[3 * r + 2]},
+          rayRange(tRanges, r), times ? times[r] : 0.f);
+  uint32_t n = 0;
+  HitOut o;
+  HitOut *dst = out + (uint64_t)r * maxOut;
+  while (n < maxOut && hitNext(c, st, o)) dst[n++] = o;
+  count[r] = n;
+}
+
+hipError_t launchIntervalBatch(const IteratorContextView &c, uint32_t numRays, const float *org, const float *dir,
+                               const float *tRanges, IntervalOut *out, uint32_t maxOut, uint32_t *count,
+                               hipStream_t st)
+{
+  if (numRays == 0) return hipSuccess;
+  hipLaunchKernelGGL(kIntervalBatch, dim3((numRays + kIterBlock - 1) / kIterBlock), dim3(kIterBlock), 0, st, c,
+                     numRays, org, dir, tRanges, out, maxOut, count);
+  return hipGetLastError();
+}
+
+hipError_t launchHitBatch(const IteratorContextView &c, uint32_t numRays, const float *org, const float *dir,
+                          const float *tRanges, const float *times, HitOut *out, uint32_t maxOut, uint32_t *count,
+                          hipStream_t st)
+{
+  if (numRays == 0) return hipSuccess;
+  hipLaunchKernelGGL(kHitBatch, dim3((numRays + kIterBlock - 1) / kIterBlock), dim3(kIterBlock), 0, st, c, numRays,
+                     org, dir, tRanges, times, out, maxOut, count);
+  return hipGetLastError();
+}
+
+}  // namespace vklhip

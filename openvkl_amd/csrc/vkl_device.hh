@@ -1,0 +1,845 @@
+// Device functions of the HIP sampling / traversal path (gfx950).
+//
+// Numerics: compiled with -ffp-contract=off so every binary32 operation is
+// rounded on its own, in the operation order of the reference ISPC sources
+// cited below (files under openvkl/devices/cpu/). Nearest / linear samples,
+// gradients and interval iteration are therefore bit-identical to the
+// reference algorithm evaluated without FMA contraction; tricubic uses the
+// separable Catmull-Rom form (mathematically equal to the Lekien-Marsden
+// matrix on central-difference constraints; agrees to ~1e-6 relative).
+#pragma once
+
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+
+#include "vkl_views.h"
+
+namespace vklhip {
+
+#define VKL_DEV __device__ __forceinline__
+
+// ---- scalar helpers -----------------------------------------------------------
+
+VKL_DEV float vmin(float a, float b) { return a < b ? a : b; }  // ISPC min
+VKL_DEV float vmax(float a, float b) { return a > b ? a : b; }  // ISPC max
+VKL_DEV float vinf() { return __int_as_float(0x7f800000); }
+
+// rkcommon lerp: (1 - t) * a + t * b.
+VKL_DEV float lerp(float t, float a, float b) { return (1.f - t) * a + t * b; }
+
+// floor then float -> int32 with NaN / out-of-range -> INT32_MIN (the x86
+// cvttss2si result the reference CPU device produces).
+VKL_DEV int32_t floorToInt(float f)
+{
+  const float fl = floorf(f);
+  return (fl >= -2147483648.f && fl < 2147483648.f) ? (int32_t)fl : INT32_MIN;
+}
+
+// rkcommon divide_safe: 1 / x with |x| < FLT_MIN clamped to +-FLT_MIN.
+VKL_DEV float divideSafe(float x)
+{
+  const float fltMin = 1.17549435e-38f;
+  const float d      = (fabsf(x) < fltMin) ? (x >= 0.f ? fltMin : -fltMin) : x;
+  return 1.f / d;
+}
+
+template <typename T>
+VKL_DEV float toFloat(T v)
+{
+  return (float)v;
+}
+template <>
+VKL_DEV float toFloat<__half>(__half v)
+{
+  return __half2float(v);
+}
+
+VKL_DEV float fetchAny(const void *p, uint32_t type, uint64_t i)
+{
+  switch (type) {
+  case kTypeUchar: return (float)((const uint8_t *)p)[i];
+  case kTypeShort: return (float)((const int16_t *)p)[i];
+  case kTypeUshort: return (float)((const uint16_t *)p)[i];
+  case kTypeHalf: return __half2float(((const __half *)p)[i]);
+  case kTypeDouble: return (float)((const double *)p)[i];
+  default: return ((const float *)p)[i];
+  }
+}
+
+struct F3
+{
+  float x, y, z;
+};
+struct I3
+{
+  int32_t x, y, z;
+};
+
+// VdbGrid.h:56-89.
+VKL_DEV F3 xfmVector(const float *M, F3 p)
+{
+  return {M[0] * p.x + M[1] * p.y + M[2] * p.z, M[3] * p.x + M[4] * p.y + M[5] * p.z,
+          M[6] * p.x + M[7] * p.y + M[8] * p.z};
+}
+VKL_DEV F3 xfmPoint(const float *M, F3 p)
+{
+  const F3 r = xfmVector(M, p);
+  return {r.x + M[9], r.y + M[10], r.z + M[11]};
+}
+VKL_DEV F3 xfmNormal(const float *MI, F3 p)
+{
+  return {MI[0] * p.x + MI[3] * p.y + MI[6] * p.z, MI[1] * p.x + MI[4] * p.y + MI[7] * p.z,
+          MI[2] * p.x + MI[5] * p.y + MI[8] * p.z};
+}
+
+// vdb_topology.h.in:96-170: voxel index inside a level-l node, z fastest.
+VKL_DEV uint32_t offsetToLinear(uint32_t l, uint32_t x, uint32_t y, uint32_t z)
+{
+  const uint32_t m = kRes[l] - 1, s = kTotalLogRes[l + 1], L = kLogRes[l];
+  return (((x & m) >> s) << (2 * L)) + (((y & m) >> s) << L) + ((z & m) >> s);
+}
+
+// ---- voxel sources --------------------------------------------------------------
+
+// Dense x-fastest array (structuredRegular), VdbSampler_dense.ih:290-330 and
+// VdbSampler_traverseAndSample.ih:314-349: out-of-domain corners read the
+// background.
+template <typename T, bool kIdx64>
+struct DenseSource
+{
+  const T *data;
+  uint32_t nx, ny, nz;
+  int32_t rx, ry, rz;
+  float bg;
+
+  VKL_DEV DenseSource(const GridView &g, const AttrView &a, uint32_t = 0)
+      : data((const T *)a.data),
+        nx(g.activeSize[0]),
+        ny(g.activeSize[1]),
+        nz(g.activeSize[2]),
+        rx(g.rootOrigin[0]),
+        ry(g.rootOrigin[1]),
+        rz(g.rootOrigin[2]),
+        bg(a.background)
+  {
+  }
+
+  VKL_DEV float at(int32_t x, int32_t y, int32_t z) const
+  {
+    const uint32_t ox = (uint32_t)(x - rx), oy = (uint32_t)(y - ry), oz = (uint32_t)(z - rz);
+    if (!(ox < nx && oy < ny && oz < nz)) return bg;
+    if (kIdx64) return toFloat(data[((uint64_t)oz * ny + oy) * (uint64_t)nx + ox]);
+    return toFloat(data[(oz * ny + oy) * nx + ox]);
+  }
+
+  // Corners s[4x + 2y + z] of the cell at ic (VdbSampler_trilinear.ih:48-108):
+  // one bounds test when the whole stencil is inside, else per corner.
+  VKL_DEV void corners8(I3 ic, float *s) const
+  {
+    const uint32_t ox = (uint32_t)(ic.x - rx), oy = (uint32_t)(ic.y - ry), oz = (uint32_t)(ic.z - rz);
+    if (ox + 1 < nx && oy + 1 < ny && oz + 1 < nz && ox < nx && oy < ny && oz < nz) {
+      const uint64_t sy = nx, sz = (uint64_t)nx * ny;
+      const T *p = kIdx64 ? data + ((uint64_t)oz * ny + oy) * (uint64_t)nx + ox
+                          : data + ((oz * ny + oy) * nx + ox);
+      s[0] = toFloat(p[0]);
+      s[4] = toFloat(p[1]);
+      s[2] = toFloat(p[sy]);
+      s[6] = toFloat(p[sy + 1]);
+      s[1] = toFloat(p[sz]);
+      s[5] = toFloat(p[sz + 1]);
+      s[3] = toFloat(p[sz + sy]);
+      s[7] = toFloat(p[sz + sy + 1]);
+      return;
+    }
+#pragma unroll
+    for (int i = 0; i < 8; i++) s[i] = at(ic.x + ((i >> 2) & 1), ic.y + ((i >> 1) & 1), ic.z + (i & 1));
+  }
+};
+
+// Sparse VDB node table (VdbSampler_traverseAndSample.ih:85-202,
+// VdbSampleInner.ih.in:33-77). Leaf data is packed: dense leaves of 512
+// z-fastest voxels, tiles of one value.
+template <typename T>
+struct SparseSource
+{
+  LevelView lv[3];
+  const uint64_t *leafSlot;
+  const T *dense;
+  const T *tile;
+  int32_t root[3];
+  uint32_t size[3];
+  uint32_t maxDepth, numAttr, attr;
+  float bg;
+
+  VKL_DEV SparseSource(const GridView &g, const AttrView &a, uint32_t attributeIndex)
+      : leafSlot(g.leafSlot),
+        dense((const T *)a.data),
+        tile((const T *)a.tile),
+        maxDepth(g.maxSamplingDepth),
+        numAttr(g.numAttributes),
+        attr(attributeIndex),
+        bg(a.background)
+  {
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      lv[i]   = g.levels[i];
+      root[i] = g.rootOrigin[i];
+      size[i] = g.activeSize[i];
+    }
+  }
+
+  // Voxel code for an in-domain root-relative offset; error voxels report
+  // their level and voxel offset (VdbSampleInner.ih.in:46-54).
+  VKL_DEV uint64_t traverse(uint32_t ox, uint32_t oy, uint32_t oz, uint32_t &errLevel, uint32_t &errOffset) const
+  {
+    uint64_t node = 0;
+#pragma unroll
+    for (uint32_t l = 0; l < 3; ++l) {
+      const uint32_t vo  = (uint32_t)(node * kNumVoxels[l]) + offsetToLinear(l, ox, oy, oz);
+      const uint64_t v   = lv[l].voxels[vo];
+      const uint32_t tag = (uint32_t)v & 3u;
+      if (tag >= 2u && l + 1 > maxDepth) {
+        errLevel  = l;
+        errOffset = vo;
+        return 1u;
+      }
+      if (tag == 2u && l < 2) {
+        node = v >> 2;
+        continue;
+      }
+      return v;
+    }
+    return 0;
+  }
+
+  VKL_DEV uint64_t slotOf(uint64_t v) const { return leafSlot ? leafSlot[v >> 6] : (v >> 6); }
+
+  VKL_DEV float errorValue(uint32_t el, uint32_t eo) const
+  {
+    const Range1f r = lv[el].valueRange[(uint64_t)eo * numAttr + attr];
+    return 0.5f * (r.lower + r.upper);
+  }
+
+  VKL_DEV float leafValue(uint64_t v, uint32_t ox, uint32_t oy, uint32_t oz) const
+  {
+    const uint64_t leaf = slotOf(v);
+    if (((v >> 2) & 3u) == 1u) return toFloat(dense[leaf * 512u + (((ox & 7u) << 6) | ((oy & 7u) << 3) | (oz & 7u))]);
+    return toFloat(tile[leaf]);
+  }
+
+  VKL_DEV bool inDomain(uint32_t ox, uint32_t oy, uint32_t oz) const
+  {
+    return ox < size[0] && oy < size[1] && oz < size[2];
+  }
+
+  // VdbSampler_sample_inner, VdbSampler_traverseAndSample.ih:144-197.
+  VKL_DEV float at(int32_t x, int32_t y, int32_t z) const
+  {
+    const uint32_t ox = (uint32_t)(x - root[0]), oy = (uint32_t)(y - root[1]), oz = (uint32_t)(z - root[2]);
+    if (!inDomain(ox, oy, oz)) return bg;
+    uint32_t el = 0, eo = 0;
+    const uint64_t v   = traverse(ox, oy, oz, el, eo);
+    const uint32_t tag = (uint32_t)v & 3u;
+    if (tag == 3u) return leafValue(v, ox, oy, oz);
+    if (tag == 1u) return errorValue(el, eo);
+    return 0.f;
+  }
+
+  // When the 2x2x2 stencil lies in one leaf (7/8 per axis), one traversal
+  // serves all corners; the values equal eight separate traversals.
+  VKL_DEV void corners8(I3 ic, float *s) const
+  {
+    const uint32_t ox = (uint32_t)(ic.x - root[0]), oy = (uint32_t)(ic.y - root[1]), oz = (uint32_t)(ic.z - root[2]);
+    if (inDomain(ox, oy, oz) && inDomain(ox + 1, oy + 1, oz + 1) && (ox & 7u) != 7u && (oy & 7u) != 7u &&
+        (oz & 7u) != 7u) {
+      uint32_t el = 0, eo = 0;
+      const uint64_t v   = traverse(ox, oy, oz, el, eo);
+      const uint32_t tag = (uint32_t)v & 3u;
+      if (tag == 3u && ((v >> 2) & 3u) == 1u) {
+        const T *p = dense + slotOf(v) * 512u + (((ox & 7u) << 6) | ((oy & 7u) << 3) | (oz & 7u));
+        s[0] = toFloat(p[0]);
+        s[1] = toFloat(p[1]);
+        s[2] = toFloat(p[8]);
+        s[3] = toFloat(p[9]);
+        s[4] = toFloat(p[64]);
+        s[5] = toFloat(p[65]);
+        s[6] = toFloat(p[72]);
+        s[7] = toFloat(p[73]);
+        return;
+      }
+      const float c = (tag == 3u) ? toFloat(tile[slotOf(v)]) : (tag == 1u) ? errorValue(el, eo) : 0.f;
+#pragma unroll
+      for (int i = 0; i < 8; i++) s[i] = c;
+      return;
+    }
+#pragma unroll
+    for (int i = 0; i < 8; i++) s[i] = at(ic.x + ((i >> 2) & 1), ic.y + ((i >> 1) & 1), ic.z + (i & 1));
+  }
+};
+
+// ---- filters ------------------------------------------------------------------
+
+// VdbSampler_nearest.ih:25-57: +0.5 for vertex-centered data.
+template <typename Src>
+VKL_DEV float sampleNearest(const Src &src, F3 p, float off)
+{
+  return src.at(floorToInt(p.x + off), floorToInt(p.y + off), floorToInt(p.z + off));
+}
+
+// VdbSampler_trilinear.ih:378-399.
+template <typename Src>
+VKL_DEV float sampleLinear(const Src &src, F3 p)
+{
+  const I3 ic = {floorToInt(p.x), floorToInt(p.y), floorToInt(p.z)};
+  const F3 d  = {p.x - (float)ic.x, p.y - (float)ic.y, p.z - (float)ic.z};
+  float s[8];
+  src.corners8(ic, s);
+  return lerp(d.x, lerp(d.y, lerp(d.z, s[0], s[1]), lerp(d.z, s[2], s[3])),
+              lerp(d.y, lerp(d.z, s[4], s[5]), lerp(d.z, s[6], s[7])));
+}
+
+// VdbSampler_trilinear.ih:751-780 (index-space gradient).
+template <typename Src>
+VKL_DEV F3 gradientLinear(const Src &src, F3 p)
+{
+  const I3 ic = {floorToInt(p.x), floorToInt(p.y), floorToInt(p.z)};
+  const F3 d  = {p.x - (float)ic.x, p.y - (float)ic.y, p.z - (float)ic.z};
+  float s[8];
+  src.corners8(ic, s);
+  F3 r;
+  r.x = lerp(d.y, lerp(d.z, s[4] - s[0], s[5] - s[1]), lerp(d.z, s[6] - s[2], s[7] - s[3]));
+  r.y = lerp(d.x, lerp(d.z, s[2] - s[0], s[3] - s[1]), lerp(d.z, s[6] - s[4], s[7] - s[5]));
+  r.z = lerp(d.x, lerp(d.y, s[1] - s[0], s[3] - s[2]), lerp(d.y, s[5] - s[4], s[7] - s[6]));
+  return r;
+}
+
+// Catmull-Rom weights for taps -1, 0, 1, 2 at offset t in [0, 1), and their
+// derivatives. The tensor product of these equals the tricubic interpolant
+// of VdbSampler_tricubic.ih:34-578 (Hermite on central differences).
+VKL_DEV void crWeights(float t, float *w)
+{
+  const float t2 = t * t, t3 = t2 * t;
+  w[0] = 0.5f * (-t3 + 2.f * t2 - t);
+  w[1] = 0.5f * (3.f * t3 - 5.f * t2 + 2.f);
+  w[2] = 0.5f * (-3.f * t3 + 4.f * t2 + t);
+  w[3] = 0.5f * (t3 - t2);
+}
+VKL_DEV void crDerivWeights(float t, float *w)
+{
+  const float t2 = t * t;
+  w[0] = 0.5f * (-3.f * t2 + 4.f * t - 1.f);
+  w[1] = 0.5f * (9.f * t2 - 10.f * t);
+  w[2] = 0.5f * (-9.f * t2 + 8.f * t + 1.f);
+  w[3] = 0.5f * (3.f * t2 - 2.f * t);
+}
+
+template <typename Src>
+VKL_DEV void cubicTaps(const Src &src, I3 ic, float *S)
+{
+#pragma unroll
+  for (int i = 0; i < 64; i++) S[i] = src.at(ic.x + (i >> 4) - 1, ic.y + ((i >> 2) & 3) - 1, ic.z + (i & 3) - 1);
+}
+
+template <typename Src>
+VKL_DEV float sampleCubic(const Src &src, F3 p)
+{
+  const I3 ic = {floorToInt(p.x), floorToInt(p.y), floorToInt(p.z)};
+  float wx[4], wy[4], wz[4];
+  crWeights(p.x - (float)ic.x, wx);
+  crWeights(p.y - (float)ic.y, wy);
+  crWeights(p.z - (float)ic.z, wz);
+  float S[64];
+  cubicTaps(src, ic, S);
+  float r = 0.f;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    float ry = 0.f;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const float *q = S + 16 * i + 4 * j;
+      ry += wy[j] * (wz[0] * q[0] + wz[1] * q[1] + wz[2] * q[2] + wz[3] * q[3]);
+    }
+    r += wx[i] * ry;
+  }
+  return r;
+}
+
+template <typename Src>
+VKL_DEV F3 gradientCubic(const Src &src, F3 p)
+{
+  const I3 ic = {floorToInt(p.x), floorToInt(p.y), floorToInt(p.z)};
+  float wx[4], wy[4], wz[4], dx[4], dy[4], dz[4];
+  const F3 d = {p.x - (float)ic.x, p.y - (float)ic.y, p.z - (float)ic.z};
+  crWeights(d.x, wx);
+  crWeights(d.y, wy);
+  crWeights(d.z, wz);
+  crDerivWeights(d.x, dx);
+  crDerivWeights(d.y, dy);
+  crDerivWeights(d.z, dz);
+  float S[64];
+  cubicTaps(src, ic, S);
+  F3 g = {0.f, 0.f, 0.f};
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    float gy_v = 0.f, gy_dy = 0.f, gy_dz = 0.f;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const float *q = S + 16 * i + 4 * j;
+      const float vz = wz[0] * q[0] + wz[1] * q[1] + wz[2] * q[2] + wz[3] * q[3];
+      const float dzv = dz[0] * q[0] + dz[1] * q[1] + dz[2] * q[2] + dz[3] * q[3];
+      gy_v += wy[j] * vz;
+      gy_dy += dy[j] * vz;
+      gy_dz += wy[j] * dzv;
+    }
+    g.x += dx[i] * gy_v;
+    g.y += wx[i] * gy_dy;
+    g.z += wx[i] * gy_dz;
+  }
+  return g;
+}
+
+template <typename Src>
+VKL_DEV float sampleFiltered(const Src &src, int filter, F3 p, float nearestOffset)
+{
+  if (filter == kFilterNearest) return sampleNearest(src, p, nearestOffset);
+  if (filter == kFilterCubic) return sampleCubic(src, p);
+  return sampleLinear(src, p);
+}
+
+// Runtime-dispatched sample at an object-space point (the sampler entry of
+// VdbSampler_iterator_computeSample_varying, VdbSampler.ispc:341-374). Not
+// inlined: it covers every element type x filter, and iterator code calls it
+// from several places.
+static __device__ __noinline__ float sampleObject(const SamplerView &s, uint32_t attr, F3 oc)
+{
+  const GridView &g = s.grid;
+  const F3 p        = xfmPoint(g.o2i, oc);
+  const AttrView a  = g.attrs[attr];
+  const float noff  = g.constantCellData ? 0.f : 0.5f;
+  if (g.dense) {
+    const bool big = (uint64_t)g.activeSize[0] * g.activeSize[1] * g.activeSize[2] >= (1ull << 32);
+    switch (a.type) {
+#define VKL_DENSE_CASE(CODE, T)                                                       \
+  case CODE:                                                                          \
+    if (big) return sampleFiltered(DenseSource<T, true>(g, a, attr), s.filter, p, noff);   \
+    return sampleFiltered(DenseSource<T, false>(g, a, attr), s.filter, p, noff);
+      VKL_DENSE_CASE(kTypeUchar, uint8_t)
+      VKL_DENSE_CASE(kTypeShort, int16_t)
+      VKL_DENSE_CASE(kTypeUshort, uint16_t)
+      VKL_DENSE_CASE(kTypeHalf, __half)
+      VKL_DENSE_CASE(kTypeDouble, double)
+      VKL_DENSE_CASE(kTypeFloat, float)
+#undef VKL_DENSE_CASE
+    }
+    return 0.f;
+  }
+  if (a.type == kTypeHalf) return sampleFiltered(SparseSource<__half>(g, a, attr), s.filter, p, noff);
+  return sampleFiltered(SparseSource<float>(g, a, attr), s.filter, p, noff);
+}
+
+// ---- interval iteration (VdbIterator.ispc:36-180, Dda.ih, HDDA.ih.in,
+//      VdbQueryVoxel.ih.in, VdbQueryVoxelDense.ih) -----------------------------
+
+VKL_DEV int dirSign(float v) { return ((int)(0 < v)) - ((int)(v < 0)); }
+VKL_DEV float dirRcp(float v) { return (v == 0) ? vinf() : divideSafe(v); }
+
+// Dda.ih:66-84.
+VKL_DEV float planeT(float plane, float org, float inv)
+{
+  const float t = (plane - org) * inv;
+  return (plane == org || isnan(t)) ? vinf() : t;
+}
+
+// Dda.ih:127-140.
+VKL_DEV int32_t clampCell(float f, int32_t res) { return floorToInt(vmax(f, 0.f)) & ~(res - 1); }
+
+constexpr uint32_t kMaxDdaSteps = 1u << 22;
+
+#define DX(l) (l)
+#define DY(l) (4 + (l))
+#define DZ(l) (8 + (l))
+
+// Dda.ih:145-196.
+VKL_DEV void ddaInit(DdaState &d, F3 org, F3 dir, float tMin, uint32_t level)
+{
+  const F3 oo = {org.x + tMin * dir.x, org.y + tMin * dir.y, org.z + tMin * dir.z};
+  const int sx = dirSign(dir.x), sy = dirSign(dir.y), sz = dirSign(dir.z);
+  const F3 id = {dirRcp(dir.x), dirRcp(dir.y), dirRcp(dir.z)};
+  d.level     = level < 3u ? level : 3u;
+  d.t         = tMin;
+  d.rayOrg[0] = org.x;
+  d.rayOrg[1] = org.y;
+  d.rayOrg[2] = org.z;
+  d.rayDir[0] = dir.x;
+  d.rayDir[1] = dir.y;
+  d.rayDir[2] = dir.z;
+  for (uint32_t l = 0; l <= d.level; ++l) {
+    const int32_t res = (int32_t)kRes[l + 1];
+    const int32_t ix = clampCell(oo.x, res), iy = clampCell(oo.y, res), iz = clampCell(oo.z, res);
+    d.idx[DX(l)]      = ix;
+    d.idx[DY(l)]      = iy;
+    d.idx[DZ(l)]      = iz;
+    d.idxDelta[DX(l)] = sx * res;
+    d.idxDelta[DY(l)] = sy * res;
+    d.idxDelta[DZ(l)] = sz * res;
+    d.tDelta[DX(l)]   = (float)res * fabsf(id.x);
+    d.tDelta[DY(l)]   = (float)res * fabsf(id.y);
+    d.tDelta[DZ(l)]   = (float)res * fabsf(id.z);
+    d.tNext[DX(l)]    = planeT((float)(ix + res * (sx >= 0)), org.x, id.x);
+    d.tNext[DY(l)]    = planeT((float)(iy + res * (sy >= 0)), org.y, id.y);
+    d.tNext[DZ(l)]    = planeT((float)(iz + res * (sz >= 0)), org.z, id.z);
+  }
+}
+
+// Dda.ih:204-266.
+VKL_DEV void ddaReinitBelow(DdaState &d, uint32_t level, int32_t dx, int32_t dy)
+{
+  const F3 dir = {d.rayDir[0], d.rayDir[1], d.rayDir[2]};
+  const F3 id  = {dirRcp(dir.x), dirRcp(dir.y), dirRcp(dir.z)};
+  const int sx = dirSign(dir.x), sy = dirSign(dir.y), sz = dirSign(dir.z);
+  F3 oo = {d.rayOrg[0] + d.t * dir.x, d.rayOrg[1] + d.t * dir.y, d.rayOrg[2] + d.t * dir.z};
+  if (dx != 0)
+    oo.x = oo.x + ((float)sx) * 1e-2f;
+  else if (dy != 0)
+    oo.y = oo.y + ((float)sy) * 1e-2f;
+  else
+    oo.z = oo.z + ((float)sz) * 1e-2f;
+  for (uint32_t l = level + 1; l <= d.level; ++l) {
+    const int32_t res = (int32_t)kRes[l + 1];
+    d.idx[DX(l)]      = clampCell(oo.x, res);
+    d.idx[DY(l)]      = clampCell(oo.y, res);
+    d.idx[DZ(l)]      = clampCell(oo.z, res);
+    d.tNext[DX(l)]    = planeT((float)(d.idx[DX(l)] + res * (sx >= 0)), d.rayOrg[0], id.x);
+    d.tNext[DY(l)]    = planeT((float)(d.idx[DY(l)] + res * (sy >= 0)), d.rayOrg[1], id.y);
+    d.tNext[DZ(l)]    = planeT((float)(d.idx[DZ(l)] + res * (sz >= 0)), d.rayOrg[2], id.z);
+  }
+}
+
+// Dda.ih:271-309 (ties: x, then y, then z).
+VKL_DEV void ddaStep(DdaState &d, uint32_t l)
+{
+  const float tx = d.tNext[DX(l)], ty = d.tNext[DY(l)], tz = d.tNext[DZ(l)];
+  const float minT = vmin(vmin(tx, ty), tz);
+  int a = DZ(l);
+  if (minT == tx)
+    a = DX(l);
+  else if (minT == ty)
+    a = DY(l);
+  d.t = d.tNext[a];
+  d.tNext[a] += d.tDelta[a];
+  d.idx[a] += d.idxDelta[a];
+}
+
+// HDDA.ih.in:24-101: step on the level of the empty voxel, re-initialise the
+// levels below, and let parents follow a child that left their cell.
+VKL_DEV void hddaStep(DdaState &d, uint32_t descLevel)
+{
+  // Recursion unrolled: find the step level, then propagate upwards.
+  if (descLevel >= 3u) {
+    ddaStep(d, 3);
+  } else {
+    const uint32_t L  = descLevel;
+    const int32_t ox = d.idx[DX(L)], oy = d.idx[DY(L)];
+    ddaStep(d, L);
+    if (L < d.level) ddaReinitBelow(d, L, d.idx[DX(L)] - ox, d.idx[DY(L)] - oy);
+  }
+  // Parents of the stepped level, from the nearest one upwards (each one only
+  // steps if its child stepped and left its cell).
+  for (int P = (int)(descLevel >= 3u ? 3 : descLevel) - 1; P >= 0; --P) {
+    const int32_t res = (int32_t)kRes[P + 1];
+    const int C       = P + 1;
+    const bool left   = d.idx[DX(C)] < d.idx[DX(P)] || d.idx[DY(C)] < d.idx[DY(P)] || d.idx[DZ(C)] < d.idx[DZ(P)] ||
+                      d.idx[DX(C)] >= d.idx[DX(P)] + res || d.idx[DY(C)] >= d.idx[DY(P)] + res ||
+                      d.idx[DZ(C)] >= d.idx[DZ(P)] + res;
+    if (!left) break;
+    ddaStep(d, (uint32_t)P);
+  }
+}
+
+// ValueRanges.ih:22-52.
+VKL_DEV bool rangesOverlap(const IteratorContextView &c, Range1f r)
+{
+  if (c.numRanges == 0) return true;
+  if (!(c.rangesMinMax.upper >= r.lower && c.rangesMinMax.lower <= r.upper)) return false;
+  for (int i = 0; i < c.numRanges; i++)
+    if (c.ranges[i].upper >= r.lower && c.ranges[i].lower <= r.upper) return true;
+  return false;
+}
+
+struct VoxelDesc
+{
+  uint32_t level;
+  bool isEmpty;
+  Range1f valueRange;
+};
+
+// VdbQueryVoxel.ih.in:42-116.
+VKL_DEV void queryVoxel(const IteratorContextView &c, int32_t x, int32_t y, int32_t z, uint32_t depth,
+                        VoxelDesc &desc)
+{
+  const GridView &g = c.sampler.grid;
+  uint64_t node     = 0;
+  for (uint32_t L = 0; L < 3; ++L) {
+    const uint32_t vo = (uint32_t)(offsetToLinear(L, (uint32_t)x, (uint32_t)y, (uint32_t)z) + node * kNumVoxels[L]);
+    const uint64_t v  = g.levels[L].voxels[vo];
+    desc.valueRange   = g.levels[L].valueRange[(uint64_t)vo * g.numAttributes + c.attributeIndex];
+    desc.level        = L;
+    desc.isEmpty      = ((v & 3u) == 0u) || !rangesOverlap(c, desc.valueRange);
+    if (desc.isEmpty) return;
+    const bool atDepth = (L + 1 > depth);
+    const bool isLeaf  = (v & 3u) == 3u;
+    if (isLeaf && !atDepth) desc.level = L + 1;
+    if (L < 2 && !(atDepth || isLeaf)) {
+      node = v >> 2;
+      continue;
+    }
+    return;
+  }
+}
+
+// VdbQueryVoxelDense.ih:115-164 (nodes numbered x-fastest, one level-1 node).
+VKL_DEV void queryVoxelDense(const IteratorContextView &c, int32_t x, int32_t y, int32_t z, uint32_t depth,
+                             VoxelDesc &desc)
+{
+  const GridView &g = c.sampler.grid;
+  desc.level        = depth;
+  const uint32_t ox = (uint32_t)x, oy = (uint32_t)y, oz = (uint32_t)z;
+  if (!(ox < g.activeSize[0] && oy < g.activeSize[1] && oz < g.activeSize[2])) {
+    desc.isEmpty = true;
+    return;
+  }
+  const uint64_t n1x = (g.activeSize[0] + kRes[1] - 1) >> kTotalLogRes[1];
+  const uint64_t n1y = (g.activeSize[1] + kRes[1] - 1) >> kTotalLogRes[1];
+  const uint64_t l1  = (uint64_t)(oz >> kTotalLogRes[1]) * n1y * n1x + (uint64_t)(oy >> kTotalLogRes[1]) * n1x +
+                      (ox >> kTotalLogRes[1]);
+  uint64_t vi;
+  int lvl;
+  if (depth >= 2) {
+    const uint64_t n2x = (g.activeSize[0] + kRes[2] - 1) >> kTotalLogRes[2];
+    const uint64_t n2y = (g.activeSize[1] + kRes[2] - 1) >> kTotalLogRes[2];
+    const uint64_t l2  = (uint64_t)(oz >> kTotalLogRes[2]) * n2y * n2x + (uint64_t)(oy >> kTotalLogRes[2]) * n2x +
+                        (ox >> kTotalLogRes[2]);
+    vi  = l1 * kNumVoxels[1] + l2 * kNumVoxels[2] + offsetToLinear(2, ox, oy, oz);
+    lvl = 2;
+  } else if (depth == 1) {
+    vi  = l1 * kNumVoxels[1] + offsetToLinear(1, ox, oy, oz);
+    lvl = 1;
+  } else {
+    vi  = offsetToLinear(0, ox, oy, oz);
+    lvl = 0;
+  }
+  desc.valueRange = g.levels[lvl].valueRange[vi * g.numAttributes + c.attributeIndex];
+  desc.isEmpty    = !rangesOverlap(c, desc.valueRange);
+}
+
+// box_utility.ih:62-75.
+VKL_DEV Range1f intersectBox(F3 org, F3 dir, const float *box, Range1f lim)
+{
+  const F3 inv  = {divideSafe(dir.x), divideSafe(dir.y), divideSafe(dir.z)};
+  const F3 mins = {(box[0] - org.x) * inv.x, (box[1] - org.y) * inv.y, (box[2] - org.z) * inv.z};
+  const F3 maxs = {(box[3] - org.x) * inv.x, (box[4] - org.y) * inv.y, (box[5] - org.z) * inv.z};
+  Range1f r;
+  r.lower = vmax(vmax(vmin(mins.x, maxs.x), vmin(mins.y, maxs.y)), vmax(vmin(mins.z, maxs.z), lim.lower));
+  r.upper = vmin(vmin(vmax(mins.x, maxs.x), vmax(mins.y, maxs.y)), vmin(vmax(mins.z, maxs.z), lim.upper));
+  return r;
+}
+
+// VdbIterator_Initialize, VdbIterator.ispc:36-94.
+VKL_DEV void intervalInit(const IteratorContextView &c, IntervalState &it, F3 origin, F3 direction, Range1f tRange)
+{
+  const GridView &g = c.sampler.grid;
+  const F3 o        = xfmPoint(g.o2i, origin);
+  const F3 org      = {o.x - (float)g.rootOrigin[0], o.y - (float)g.rootOrigin[1], o.z - (float)g.rootOrigin[2]};
+  const F3 dir      = xfmVector(g.o2i, direction);
+  it.nominalDeltaT  = vmin(vmin(1.f * divideSafe(fabsf(dir.x)), 1.f * divideSafe(fabsf(dir.y))),
+                          1.f * divideSafe(fabsf(dir.z)));
+  const Range1f ti = intersectBox(org, dir, g.domainBox, tRange);
+  if (ti.upper <= ti.lower) {
+    it.tMax  = -vinf();
+    it.dda.t = 0.f;
+  } else {
+    it.tMax = ti.upper;
+    ddaInit(it.dda, org, dir, ti.lower, c.maxIteratorDepth);
+  }
+}
+
+// VdbIterator_iterateIntervalInternal, VdbIterator.ispc:96-159.
+VKL_DEV bool intervalNext(const IteratorContextView &c, IntervalState &it, IntervalOut &out)
+{
+  if (!(it.dda.t <= it.tMax)) return false;
+  // Dense node indices assume one level-1 node (VdbQueryVoxelDense.ih:48-57);
+  // larger volumes use the general query, which stays inside the node table.
+  const GridView &g = c.sampler.grid;
+  const bool dense  = g.dense != 0 && g.activeSize[0] <= kRes[1] && g.activeSize[1] <= kRes[1] &&
+                     g.activeSize[2] <= kRes[1];
+  VoxelDesc v{0, true, {0.f, 0.f}};
+  // Every step advances t by a positive tDelta; a ray whose step makes no
+  // progress (infinite direction components) is ended after kMaxDdaSteps.
+  uint32_t steps = 0;
+  while (it.dda.t <= it.tMax) {
+    if (++steps > kMaxDdaSteps) {
+      it.tMax = -vinf();
+      return false;
+    }
+    const uint32_t l = it.dda.level;
+    if (dense)
+      queryVoxelDense(c, it.dda.idx[DX(l)], it.dda.idx[DY(l)], it.dda.idx[DZ(l)], l, v);
+    else
+      queryVoxel(c, it.dda.idx[DX(l)], it.dda.idx[DY(l)], it.dda.idx[DZ(l)], l, v);
+    if (!v.isEmpty) break;
+    hddaStep(it.dda, v.level);
+  }
+  if (it.dda.t < it.tMax) {
+    out.vLower = v.valueRange.lower;
+    out.vUpper = v.valueRange.upper;
+    out.tLower = it.dda.t;
+    hddaStep(it.dda, v.level);
+    out.tUpper        = vmin(it.dda.t, it.tMax);
+    out.nominalDeltaT = it.nominalDeltaT;
+    return true;
+  }
+  return false;
+}
+
+// ---- hit iteration (DefaultIterator.ispc:141-258, Iterator.ih:17-200) --------
+
+VKL_DEV float sampleAlong(const IteratorContextView &c, F3 o, F3 d, float t)
+{
+  return sampleObject(c.sampler, c.attributeIndex, F3{o.x + t * d.x, o.y + t * d.y, o.z + t * d.z});
+}
+
+// bisect, Iterator.ih:23-85.
+VKL_DEV float bisect(const IteratorContextView &c, F3 o, F3 d, float t0, float s0, float t, float s, float iso,
+                     float tol)
+{
+  float tMid = 0.f;
+  for (int iter = 0; iter < 10; ++iter) {
+    tMid            = 0.5f * (t0 + t);
+    const float err = 0.5f * (t - t0);
+    if (err < tol) break;
+    const float sm = sampleAlong(c, o, d, tMid);
+    if (isnan(sm)) return (fabsf(iso - s0) < fabsf(iso - s)) ? t0 : t;
+    if (sm == iso || (iso - s0) * (iso - sm) < 0.f) {
+      t = tMid;
+      s = sm;
+    } else if (s == iso || (iso - s) * (iso - sm) < 0.f) {
+      t0 = tMid;
+      s0 = sm;
+    } else {
+      return vinf();
+    }
+  }
+  return tMid;
+}
+
+// intersectSurfacesBisection, Iterator.ih:87-197.
+VKL_DEV bool intersectBisection(const IteratorContextView &c, F3 o, F3 d, float tLo, float tHi, float step,
+                                HitOut &hit)
+{
+  float t0 = tLo;
+  float s0 = sampleAlong(c, o, d, t0);
+  for (int it = 0; it < 10 && isnan(s0) && t0 < tHi; ++it) {
+    t0 += vmax(1e-5f, 1e-5f * step);
+    s0 = sampleAlong(c, o, d, t0);
+  }
+  while (t0 < tHi) {
+    const float h = vmin(step, tHi - t0);
+    const float t = t0 + h;
+    if (!(t > t0)) break;  // no float progress: the march cannot continue
+    float s       = sampleAlong(c, o, d, t);
+    float ts      = t;
+    for (int it = 0; it < 10 && isnan(s) && ts > t0; ++it) {
+      ts -= vmax(1e-5f, 1e-5f * step);
+      s = sampleAlong(c, o, d, ts);
+    }
+    float tHit = vinf(), eps = vinf(), value = vinf();
+    for (int i = 0; i < c.numValues; i++) {
+      const float v = c.values[i];
+      if (fabsf(s0 - v) < 1e-6f) {
+        if (t0 < tHit && t0 <= tHi) {
+          tHit  = t0;
+          value = v;
+          eps   = 0.125f * step;
+        }
+      } else if (!isnan(s0 + s) && (v - s0) * (v - s) < 0.f) {
+        const float tIso = bisect(c, o, d, t0, s0, ts, s, v, vmin(1e-6f, 0.01f * h));
+        if (tIso < tHit && tIso <= tHi) {
+          tHit  = tIso;
+          value = v;
+          eps   = 0.125f * step;
+        }
+      }
+      if (t == tHi && fabsf(s - v) < 1e-6f) {
+        if (ts < tHit && ts <= tHi) {
+          tHit  = ts;
+          value = v;
+          eps   = 0.125f * step;
+        }
+      }
+    }
+    if (tHit < vinf()) {
+      hit.t       = tHit;
+      hit.sample  = value;
+      hit.epsilon = eps * sqrtf(d.x * d.x + d.y * d.y + d.z * d.z);
+      return true;
+    }
+    t0 = t;
+    s0 = s;
+  }
+  return false;
+}
+
+VKL_DEV void hitInit(const IteratorContextView &c, HitState &h, F3 origin, F3 direction, Range1f tRange, float time)
+{
+  intervalInit(c, h.iv, origin, direction, tRange);
+  h.origin[0] = origin.x;
+  h.origin[1] = origin.y;
+  h.origin[2] = origin.z;
+  h.direction[0] = direction.x;
+  h.direction[1] = direction.y;
+  h.direction[2] = direction.z;
+  h.time             = time;
+  h.curLower         = 1.f;  // resetInterval, Interval.ih:15-22
+  h.curUpper         = -1.f;
+  h.curNominalDeltaT = 0.f;
+  h.lastHitOffsetT   = -vinf();
+}
+
+// DefaultHitIterator_iterateHit, DefaultIterator.ispc:168-258.
+VKL_DEV bool hitNext(const IteratorContextView &c, HitState &h, HitOut &hit)
+{
+  if (c.numValues == 0) return false;
+  if (h.curLower == vinf()) return false;
+  const F3 o = {h.origin[0], h.origin[1], h.origin[2]};
+  const F3 d = {h.direction[0], h.direction[1], h.direction[2]};
+  for (;;) {
+    const bool need = h.curUpper <= h.curLower;
+    bool have       = !need;
+    if (need) {
+      IntervalOut iv;
+      have = intervalNext(c, h.iv, iv);
+      if (have) {
+        h.curLower         = iv.tLower;
+        h.curUpper         = iv.tUpper;
+        h.curNominalDeltaT = iv.nominalDeltaT;
+      }
+    }
+    if (need && have) {
+      h.curLower = vmax(h.curLower, h.lastHitOffsetT);
+      if (h.curUpper <= h.curLower) continue;
+    }
+    if (!have) {
+      h.curLower = vinf();
+      return false;
+    }
+    hit.t = vinf();
+    if (intersectBisection(c, o, d, h.curLower, h.curUpper, h.curNominalDeltaT, hit)) {
+      h.curLower       = hit.t + hit.epsilon;
+      h.lastHitOffsetT = hit.t + hit.epsilon;
+      return true;
+    }
+    h.curLower = vinf();
+  }
+}
+
+}  // namespace vklhip

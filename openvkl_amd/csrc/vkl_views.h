@@ -1,0 +1,131 @@
+// Device-resident views of committed objects. Plain structs shared by the
+// C++ object layer (which fills them at commit) and the HIP kernels (which
+// take them by value as kernel arguments).
+#pragma once
+
+#include <stdint.h>
+
+namespace vklhip {
+
+// VDB topology 6/5/4/3 (openvkl/CMakeLists.txt:37-49).
+constexpr int kNumLevels = 4;
+constexpr uint32_t kLogRes[5]      = {6, 5, 4, 3, 0};
+constexpr uint32_t kTotalLogRes[5] = {18, 12, 7, 3, 0};
+constexpr uint32_t kRes[5]         = {262144, 4096, 128, 8, 1};
+constexpr uint64_t kNumVoxels[4]   = {262144, 32768, 4096, 512};
+
+// VKLDataType codes used for voxel data.
+enum : uint32_t {
+  kTypeUchar  = 2500,
+  kTypeShort  = 3000,
+  kTypeUshort = 3500,
+  kTypeHalf   = 5800,
+  kTypeFloat  = 6000,
+  kTypeDouble = 7000,
+};
+
+enum : int { kFilterNearest = 0, kFilterLinear = 100, kFilterCubic = 200 };
+
+struct Range1f
+{
+  float lower, upper;
+};
+
+// One attribute of a volume.
+struct AttrView
+{
+  const void *data;   // dense: x-fastest voxels; vdb: packed dense leaves (512 each)
+  const void *tile;   // vdb: packed tile values (one per tile leaf), may be null
+  uint32_t type;      // VKLDataType of the elements
+  float background;   // value outside the domain
+};
+
+// One inner level of the node table (levels 0..2).
+struct LevelView
+{
+  const uint64_t *voxels;       // numNodes * res^3 voxel codes
+  const Range1f *valueRange;    // numNodes * res^3 * numAttributes
+  uint64_t numNodes;
+};
+
+struct GridView
+{
+  float o2i[12];          // object -> index, row-major 3x4
+  int32_t rootOrigin[3];  // index space
+  uint32_t activeSize[3]; // domain size in voxels
+  float domainBox[6];     // domain-space (root-relative) bounding box
+  int32_t dense;
+  int32_t constantCellData;
+  uint32_t numAttributes;
+  uint32_t maxSamplingDepth;
+  const AttrView *attrs;      // device array [numAttributes]
+  const uint64_t *leafSlot;   // vdb with per-leaf node.data: leaf index -> packed slot
+  LevelView levels[3];
+};
+
+// Sampler = grid + filters.
+struct SamplerView
+{
+  GridView grid;
+  int32_t filter;
+  int32_t gradientFilter;
+};
+
+// ---- iterator state (lives in caller-owned buffers; also used in batches) ----
+
+// Dda.ih:17-36 layout (4 levels x 3 axes).
+struct DdaState
+{
+  uint32_t level;
+  float t;
+  float rayOrg[3];
+  float rayDir[3];
+  int32_t idx[12];
+  float tNext[12];
+  float tDelta[12];
+  int32_t idxDelta[12];
+};
+
+struct IntervalState
+{
+  float tMax;
+  float nominalDeltaT;
+  DdaState dda;
+};
+
+struct HitState
+{
+  IntervalState iv;
+  float origin[3];
+  float direction[3];
+  float time;
+  float curLower, curUpper;  // current interval t range
+  float curNominalDeltaT;
+  float lastHitOffsetT;
+  uint32_t pad;
+};
+
+// Interval / hit iterator contexts.
+struct IteratorContextView
+{
+  SamplerView sampler;
+  uint32_t attributeIndex;
+  uint32_t maxIteratorDepth;
+  int32_t numRanges;       // value-range selector (0 = everything)
+  Range1f rangesMinMax;
+  const Range1f *ranges;   // device array [numRanges]
+  int32_t numValues;       // hit iterator iso values
+  const float *values;     // device array [numValues]
+};
+
+struct IntervalOut
+{
+  float tLower, tUpper, vLower, vUpper, nominalDeltaT;
+};
+
+struct HitOut
+{
+  float t, sample, epsilon;
+};
+
+}  // namespace vklhip

@@ -1,0 +1,103 @@
+/* CPU parity oracle for the Open VKL sampling / traversal hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY. This is a single-threaded (optionally
+ * multi-threaded for baseline timing) C++ restatement of the reference
+ * algorithms; it is the checker that tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py call. The shipped library never links or calls
+ * it.
+ *
+ * Floating-point convention: every arithmetic expression is evaluated in
+ * IEEE binary32 with no FMA contraction (built with -ffp-contract=off), in the
+ * operation order of the reference source.
+ */
+#pragma once
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct OracleGrid OracleGrid;
+
+/* structuredRegular (DenseVdbVolume). data[a] holds dims.x*dims.y*dims.z
+ * elements of dtypes[a] (VKLDataType code), x fastest. i2o: 3x4 row-major
+ * index->object affine (rotation-shear-scale | translation). background may be
+ * NULL (= VKL_BACKGROUND_UNDEFINED for every attribute). */
+OracleGrid *oracle_dense_new(const int32_t dims[3],
+                             const int32_t indexOrigin[3],
+                             const float i2o[12],
+                             int cellCentered,
+                             uint32_t numAttributes,
+                             const uint32_t *dtypes,
+                             const void *const *data,
+                             const float *background);
+
+/* vdb volume. Leaves: level, origin (index space, 3 ints each), format.
+ * packed != 0: packedDense[a] holds (#dense leaves)*512 elements and
+ * packedTile[a] (#tile leaves) elements (may be NULL when there are no tiles).
+ * packed == 0: leafData[leaf*numAttributes + a] points at 512 (dense) or 1
+ * (tile) elements. dtypes[a] is VKL_FLOAT or VKL_HALF. clip (6 ints) may be
+ * NULL; it is the "indexClippingBounds" parameter. */
+OracleGrid *oracle_vdb_new(uint64_t numLeaves,
+                           const uint32_t *level,
+                           const int32_t *origin,
+                           const uint32_t *format,
+                           uint32_t numAttributes,
+                           const uint32_t *dtypes,
+                           int packed,
+                           const void *const *packedDense,
+                           const void *const *packedTile,
+                           const void *const *leafData,
+                           const float i2o[12],
+                           const float *background,
+                           const int32_t *clip);
+
+void oracle_free(OracleGrid *g);
+const char *oracle_last_error(void);
+
+void oracle_set_sampler(OracleGrid *g, int filter, int gradientFilter, int maxSamplingDepth);
+
+void oracle_sample(const OracleGrid *g, uint64_t n, const float *xyz, float *out, uint32_t attr);
+void oracle_sample_mt(const OracleGrid *g, uint64_t n, const float *xyz, float *out, uint32_t attr, int threads);
+void oracle_sample_m(const OracleGrid *g, uint64_t n, const float *xyz, float *out, uint32_t m, const uint32_t *attrs);
+void oracle_gradient(const OracleGrid *g, uint64_t n, const float *xyz, float *out3, uint32_t attr);
+
+void oracle_bounding_box(const OracleGrid *g, float out[6]);
+void oracle_value_range(const OracleGrid *g, uint32_t attr, float out[2]);
+uint32_t oracle_num_attributes(const OracleGrid *g);
+void oracle_grid_info(const OracleGrid *g, int32_t rootOrigin[3], uint32_t activeSize[3], float domainBox[6], float o2i[12]);
+
+/* Node table of level l (0..2): numNodes, then origin (3 u32 per node),
+ * voxel codes (numNodes * res^3), value ranges (2 floats per voxel per attr). */
+uint64_t oracle_level_num_nodes(const OracleGrid *g, int l);
+void oracle_level_copy(const OracleGrid *g, int l, uint32_t *origin, uint64_t *voxels, float *ranges);
+
+/* intervalResolutionHint -> iterator depth (IteratorContext.cpp:49-174). */
+int oracle_map_depth(float hint);
+
+/* Run an interval iterator to completion (or maxOut) for one ray. out holds
+ * 5 floats per interval: tRange.lower, tRange.upper, valueRange.lower,
+ * valueRange.upper, nominalDeltaT. ranges: numRanges (lower, upper) pairs. */
+uint32_t oracle_iterate_intervals(const OracleGrid *g, uint32_t attr, int depth,
+                                  int numRanges, const float *ranges,
+                                  const float org[3], const float dir[3], const float tRange[2],
+                                  float *out, uint32_t maxOut);
+
+/* Run a hit iterator to completion (or maxOut) for one ray. out holds 3
+ * floats per hit: t, sample, epsilon. depth < 0 selects the reference default
+ * (3 for sparse vdb, 2 for structuredRegular). */
+uint32_t oracle_iterate_hits(const OracleGrid *g, uint32_t attr, int depth,
+                             int numValues, const float *values,
+                             const float org[3], const float dir[3], const float tRange[2], float time,
+                             float *out, uint32_t maxOut);
+
+/* The Lekien-Marsden 64x64 tricubic matrix (row = coefficient index
+ * 16i+4j+k, column = constraint index), derived at load time. */
+void oracle_tricubic_matrix(int32_t *out64x64);
+
+float oracle_half_to_float(uint16_t h);
+
+#ifdef __cplusplus
+}
+#endif

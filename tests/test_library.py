@@ -1,0 +1,41 @@
+"""The C-ABI library loads and exports every entry point include/ declares
+(no GPU calls). CPU only."""
+import ctypes
+import os
+import re
+
+import openvkl_amd as vkl
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    for d, _, files in os.walk(os.path.join(ROOT, "include")):
+        for f in files:
+            if f.endswith(".h"):
+                text = open(os.path.join(d, f)).read()
+                for m in re.finditer(r"OPENVKL_INTERFACE\s+[^;{]*?\b(vkl[A-Za-z0-9_]+)\s*\(", text):
+                    names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    names = declared_symbols()
+    assert len(names) >= 79
+    lib = ctypes.CDLL(vkl.LIB_PATH)
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_binding_signatures_resolve():
+    L = vkl.lib()
+    assert L.vklComputeSampleN is not None and L.vklIterateIntervalsBatch is not None
+
+
+def test_struct_layouts_match_reference():
+    # openvkl.h:175-180, 315-320; types.h:56-69
+    assert ctypes.sizeof(vkl.VKLInterval) == 20
+    assert ctypes.sizeof(vkl.VKLHit) == 12
+    assert ctypes.sizeof(vkl.VKLObject) == 16
+    assert ctypes.sizeof(vkl.WIDE[16][2]) == 16 * 5 * 4

@@ -1,0 +1,83 @@
+"""Seeded synthetic volumes and points shared by the tests and the bench.
+
+Generators follow the reference's procedural test volumes
+(testing/volume/procedural_functions.h:38-145: wavelet, xyz, x/y/z ramps;
+testing/volume/ProceduralStructuredVolume.h:126-143: voxel (x,y,z) sits at
+object position gridOrigin + (x,y,z) * gridSpacing, x fastest).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+SEED = 20261202
+
+
+def object_coords(n, origin, spacing):
+    """Per-axis object coordinates of voxel indices 0..n-1 (binary32 math)."""
+    idx = np.arange(n, dtype=np.float32)
+    return [(np.float32(origin[a]) + idx * np.float32(spacing[a])).astype(np.float64) for a in range(3)]
+
+
+def procedural(dims, fn, origin=(0, 0, 0), spacing=(1, 1, 1), dtype=np.float32):
+    """Array [nz, ny, nx] of fn(x, y, z) at the voxel object positions."""
+    nx, ny, nz = dims
+    cx = object_coords(nx, origin, spacing)[0]
+    cy = object_coords(ny, origin, spacing)[1]
+    cz = object_coords(nz, origin, spacing)[2]
+    z, y, x = np.meshgrid(cz, cy, cx, indexing="ij")
+    return fn(x, y, z).astype(dtype)
+
+
+def wavelet(x, y, z):
+    # procedural_functions.h:38-61 (time = 0)
+    return np.sin(3 * x) + np.sin(3 * y) + np.cos(3 * z)
+
+
+def xyz(x, y, z):
+    return x * y * z
+
+
+def zramp(x, y, z):
+    return z + 0 * x
+
+
+def config1_volume(n=128, seed=SEED):
+    """sin(x) cos(y) sin(z) + U(0, 0.1), origin 0, spacing 1, float32."""
+    rng = np.random.default_rng(seed)
+    v = procedural((n, n, n), lambda x, y, z: np.sin(x) * np.cos(y) * np.sin(z))
+    return (v + rng.uniform(0, 0.1, v.shape).astype(np.float32)).astype(np.float32)
+
+
+def uniform_points(count, lo, hi, seed=SEED + 1):
+    rng = np.random.default_rng(seed)
+    lo = np.asarray(lo, np.float32)
+    hi = np.asarray(hi, np.float32)
+    return (lo + rng.random((count, 3), dtype=np.float32) * (hi - lo)).astype(np.float32)
+
+
+def sparse_shell_leaves(res=256, radius=0.35, width=0.06, seed=SEED + 2, fbm_amp=0.5):
+    """Sphere-shell density with value noise in a res^3 index domain, stored as
+    8^3 leaves (z fastest inside a leaf); only leaves touching the shell are
+    kept. Returns (origins [n,3] int32, leaves [n,512] float32)."""
+    rng = np.random.default_rng(seed)
+    nl = res // 8
+    g = np.arange(nl) * 8 + 4.0
+    cz, cy, cx = np.meshgrid(g, g, g, indexing="ij")
+    c = res / 2.0
+    r = np.sqrt((cx - c) ** 2 + (cy - c) ** 2 + (cz - c) ** 2) / res
+    keep = np.abs(r - radius) < width + 8.0 / res
+    kz, ky, kx = np.nonzero(keep)
+    origins = np.stack([kx * 8, ky * 8, kz * 8], 1).astype(np.int32)
+    # voxel centres of every kept leaf, leaf-local order x*64 + y*8 + z
+    l = np.arange(8)
+    lx, ly, lz = np.meshgrid(l, l, l, indexing="ij")
+    px = origins[:, 0:1] + lx.reshape(1, -1) + 0.5
+    py = origins[:, 1:2] + ly.reshape(1, -1) + 0.5
+    pz = origins[:, 2:3] + lz.reshape(1, -1) + 0.5
+    rr = np.sqrt((px - c) ** 2 + (py - c) ** 2 + (pz - c) ** 2) / res
+    shell = np.clip(1.0 - np.abs(rr - radius) / width, 0.0, 1.0)
+    phase = rng.uniform(0, 2 * np.pi, 6)
+    noise = (np.sin(px * 0.11 + phase[0]) * np.sin(py * 0.13 + phase[1]) * np.sin(pz * 0.07 + phase[2])
+             + 0.5 * np.sin(px * 0.23 + phase[3]) * np.sin(py * 0.29 + phase[4]) * np.sin(pz * 0.31 + phase[5]))
+    vals = shell * (1.0 + fbm_amp * noise)
+    return origins, vals.astype(np.float32)
