@@ -49,6 +49,31 @@ OPENVKL_INTERFACE void vklIterateHitsBatch(const VKLHitIteratorContext *context,
 OPENVKL_INTERFACE int vklHipGetVolumeLevel(VKLVolume volume, unsigned int level, uint64_t *numNodes,
                                            const uint64_t **voxels, const vkl_range1f **valueRanges);
 
+/* ---- batched multi-volume sampling --------------------------------------- */
+
+typedef struct VKLSampleBatchPlan_ *VKLSampleBatchPlan;
+
+/* Plan one launch that samples numBatches (sampler, points) pairs:
+ * samples[i][k] = sample of attribute attributeIndices[i] (NULL = 0) of
+ * samplers[i] at objectCoordinates[i][k], k < counts[i]. Point and sample
+ * arrays must be device memory and stay valid while the plan is used; the
+ * samplers are retained. The launch is a persistent kernel over the
+ * flattened point space, captured in a hipGraph; with autotune != 0 its
+ * shape is chosen by timing candidates once per workload signature (cached,
+ * and persisted to the file OPENVKL_HIP_TUNE_CACHE names). Results equal one
+ * vklComputeSampleN per batch. */
+OPENVKL_INTERFACE VKLSampleBatchPlan vklNewSampleBatchPlan(unsigned int numBatches, const VKLSampler *samplers,
+                                                           const unsigned int *attributeIndices,
+                                                           const unsigned int *counts,
+                                                           const vkl_vec3f *const *objectCoordinates,
+                                                           float *const *samples, int autotune);
+/* Replay the plan's graph on `stream` (hipStream_t; NULL = null stream). */
+OPENVKL_INTERFACE void vklExecuteSampleBatchPlan(VKLSampleBatchPlan plan, void *stream);
+/* The launch shape the plan uses. */
+OPENVKL_INTERFACE void vklGetSampleBatchPlanLaunch(VKLSampleBatchPlan plan, unsigned int *blocks,
+                                                   unsigned int *threads, unsigned int *tile);
+OPENVKL_INTERFACE void vklReleaseSampleBatchPlan(VKLSampleBatchPlan plan);
+
 #ifdef __cplusplus
 }
 #endif

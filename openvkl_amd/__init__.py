@@ -142,6 +142,10 @@ def lib():
         "vklIterateHit": (i32, [vp, C.POINTER(VKLHit)]),
         "vklIterateIntervalsBatch": (None, [C.POINTER(O), C.c_uint, vp, vp, vp, vp, vp, C.c_uint, vp]),
         "vklIterateHitsBatch": (None, [C.POINTER(O), C.c_uint, vp, vp, vp, vp, vp, C.c_uint, vp]),
+        "vklNewSampleBatchPlan": (vp, [C.c_uint, vp, vp, vp, vp, vp, i32]),
+        "vklExecuteSampleBatchPlan": (None, [vp, vp]),
+        "vklGetSampleBatchPlanLaunch": (None, [vp, C.POINTER(C.c_uint), C.POINTER(C.c_uint), C.POINTER(C.c_uint)]),
+        "vklReleaseSampleBatchPlan": (None, [vp]),
         "vklHipGetVolumeLevel": (i32, [O, C.c_uint, C.POINTER(C.c_uint64), C.POINTER(C.POINTER(C.c_uint64)),
                                        C.POINTER(C.POINTER(range1f))]),
     }
@@ -544,3 +548,38 @@ class IteratorContext(Handle):
         fn(C.byref(self.h), r, _ptr(org), _ptr(dir), _ptr(t_ranges), _ptr(times), _ptr(out), max_per_ray, _ptr(cnt))
         self.device.check()
         return out, cnt
+
+
+class SampleBatchPlan:
+    """vklNewSampleBatchPlan: one graph-captured persistent launch sampling
+    many (sampler, device point array) pairs."""
+
+    def __init__(self, device, samplers, points, outs, attrs=None, autotune=True):
+        n = len(samplers)
+        self.device = device
+        self._samplers = (VKLObject * n)(*[s.h for s in samplers])
+        self._attrs = (C.c_uint * n)(*(attrs or [0] * n))
+        self._counts = (C.c_uint * n)(*[_n_points(p) for p in points])
+        self._pts = (C.c_void_p * n)(*[_ptr(p) for p in points])
+        self._outs = (C.c_void_p * n)(*[_ptr(o) for o in outs])
+        self._keep = (points, outs)
+        self.h = lib().vklNewSampleBatchPlan(n, C.addressof(self._samplers), C.addressof(self._attrs),
+                                             C.addressof(self._counts), C.addressof(self._pts),
+                                             C.addressof(self._outs), int(autotune))
+        device.check()
+        if not self.h:
+            raise RuntimeError("vklNewSampleBatchPlan failed")
+
+    def launch_shape(self):
+        b, t, k = C.c_uint(), C.c_uint(), C.c_uint()
+        lib().vklGetSampleBatchPlanLaunch(self.h, C.byref(b), C.byref(t), C.byref(k))
+        return b.value, t.value, k.value
+
+    def execute(self, stream=None):
+        lib().vklExecuteSampleBatchPlan(self.h, stream)
+        self.device.check()
+
+    def release(self):
+        if self.h:
+            lib().vklReleaseSampleBatchPlan(self.h)
+            self.h = None

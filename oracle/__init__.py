@@ -51,6 +51,7 @@ def lib():
         L.oracle_free.argtypes = [vp]
         L.oracle_last_error.restype = C.c_char_p
         L.oracle_set_sampler.argtypes = [vp, i32, i32, i32]
+        L.oracle_set_cubic_double.argtypes = [vp, i32]
         L.oracle_sample.argtypes = [vp, u64, vp, vp, u32]
         L.oracle_sample_mt.argtypes = [vp, u64, vp, vp, u32, i32]
         L.oracle_sample_m.argtypes = [vp, u64, vp, vp, u32, vp]
@@ -160,6 +161,9 @@ class Grid:
     def set_sampler(self, filter=LINEAR, gradient_filter=None, max_sampling_depth=3):
         gf = filter if gradient_filter is None else gradient_filter
         lib().oracle_set_sampler(self.h, filter, gf, max_sampling_depth)
+
+    def set_cubic_double(self, on=True):
+        lib().oracle_set_cubic_double(self.h, int(bool(on)))
 
     def sample(self, xyz, attr=0, threads=1):
         xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)

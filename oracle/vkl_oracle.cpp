@@ -261,6 +261,7 @@ struct OracleGrid
 
   int filter = kFilterLinear, gradientFilter = kFilterLinear;
   uint32_t maxSamplingDepth = 3;
+  bool cubicDouble = false;
 };
 
 namespace {
@@ -454,38 +455,36 @@ V3f gradientLinear(const OracleGrid &g, V3f p, uint32_t attr)
 // Tricubic (VdbSampler_tricubic.ih:391-578): stencil [-1,2]^3, index
 // (x+1)*16 + (y+1)*4 + (z+1); constraints by central differences; coefficients
 // via the Lekien-Marsden matrix, summed in increasing constraint order.
-void tricubicCoefficients(const OracleGrid &g, V3i ic, uint32_t attr, float *coef)
+template <typename R>
+void tricubicCoefficients(const OracleGrid &g, V3i ic, uint32_t attr, R *coef)
 {
-  float S[64];
+  R S[64];
   for (int i = 0; i < 64; i++)
-    S[i] = corner(g, attr, {ic.x + (i >> 4) - 1, ic.y + ((i >> 2) & 3) - 1, ic.z + (i & 3) - 1});
+    S[i] = (R)corner(g, attr, {ic.x + (i >> 4) - 1, ic.y + ((i >> 2) & 3) - 1, ic.z + (i & 3) - 1});
   auto spl = [&](int x, int y, int z) { return S[(x + 1) * 16 + (y + 1) * 4 + (z + 1)]; };
-  float C[64];
-  for (int q = 0; q < 8; q++) {
-    const int x = (q >> 2) & 1, y = (q >> 1) & 1, z = q & 1;
-    C[q]      = spl(x, y, z);
-    C[8 + q]  = 0.5f * (-spl(x - 1, y, z) + spl(x + 1, y, z));
-    C[16 + q] = 0.5f * (-spl(x, y - 1, z) + spl(x, y + 1, z));
-    C[24 + q] = 0.5f * (-spl(x, y, z - 1) + spl(x, y, z + 1));
-    C[32 + q] = 0.25f * (spl(x - 1, y - 1, z) - spl(x - 1, y + 1, z) - spl(x + 1, y - 1, z) +
-                         spl(x + 1, y + 1, z));
-    C[40 + q] = 0.25f * (spl(x - 1, y, z - 1) - spl(x - 1, y, z + 1) - spl(x + 1, y, z - 1) +
-                         spl(x + 1, y, z + 1));
-    C[48 + q] = 0.25f * (spl(x, y - 1, z - 1) - spl(x, y - 1, z + 1) - spl(x, y + 1, z - 1) +
-                         spl(x, y + 1, z + 1));
-    C[56 + q] = 0.125f * (-spl(x - 1, y - 1, z - 1) + spl(x - 1, y - 1, z + 1) +
-                          spl(x - 1, y + 1, z - 1) - spl(x - 1, y + 1, z + 1) +
-                          spl(x + 1, y - 1, z - 1) - spl(x + 1, y - 1, z + 1) -
-                          spl(x + 1, y + 1, z - 1) + spl(x + 1, y + 1, z + 1));
+  const R h = (R)0.5, q = (R)0.25, e = (R)0.125;
+  R C[64];
+  for (int c = 0; c < 8; c++) {
+    const int x = (c >> 2) & 1, y = (c >> 1) & 1, z = c & 1;
+    C[c]      = spl(x, y, z);
+    C[8 + c]  = h * (-spl(x - 1, y, z) + spl(x + 1, y, z));
+    C[16 + c] = h * (-spl(x, y - 1, z) + spl(x, y + 1, z));
+    C[24 + c] = h * (-spl(x, y, z - 1) + spl(x, y, z + 1));
+    C[32 + c] = q * (spl(x - 1, y - 1, z) - spl(x - 1, y + 1, z) - spl(x + 1, y - 1, z) + spl(x + 1, y + 1, z));
+    C[40 + c] = q * (spl(x - 1, y, z - 1) - spl(x - 1, y, z + 1) - spl(x + 1, y, z - 1) + spl(x + 1, y, z + 1));
+    C[48 + c] = q * (spl(x, y - 1, z - 1) - spl(x, y - 1, z + 1) - spl(x, y + 1, z - 1) + spl(x, y + 1, z + 1));
+    C[56 + c] = e * (-spl(x - 1, y - 1, z - 1) + spl(x - 1, y - 1, z + 1) + spl(x - 1, y + 1, z - 1) -
+                     spl(x - 1, y + 1, z + 1) + spl(x + 1, y - 1, z - 1) - spl(x + 1, y - 1, z + 1) -
+                     spl(x + 1, y + 1, z - 1) + spl(x + 1, y + 1, z + 1));
   }
   const Tricubic &T = tricubic();
   for (int r = 0; r < 64; r++) {
     bool first = true;
-    float acc  = 0.f;
+    R acc      = 0;
     for (int c = 0; c < 64; c++) {
       const int32_t a = T.A[r][c];
       if (a == 0) continue;
-      const float term = (float)a * C[c];
+      const R term = (R)a * C[c];
       acc   = first ? term : acc + term;
       first = false;
     }
@@ -493,41 +492,58 @@ void tricubicCoefficients(const OracleGrid &g, V3i ic, uint32_t attr, float *coe
   }
 }
 
-float sampleCubic(const OracleGrid &g, V3f p, uint32_t attr)
+template <typename R>
+float sampleCubicT(const OracleGrid &g, V3f p, uint32_t attr)
 {
   const V3i ic = floorToInt(p);
-  float coef[64];
-  tricubicCoefficients(g, ic, attr, coef);
+  R coef[64];
+  tricubicCoefficients<R>(g, ic, attr, coef);
   const V3f d = {p.x - (float)ic.x, p.y - (float)ic.y, p.z - (float)ic.z};
-  const float X[4] = {1.f, d.x, d.x * d.x, d.x * d.x * d.x};
-  const float Y[4] = {1.f, d.y, d.y * d.y, d.y * d.y * d.y};
-  const float Z[4] = {1.f, d.z, d.z * d.z, d.z * d.z * d.z};
-  float result = 0.f;
+  const R dx = d.x, dy = d.y, dz = d.z;
+  const R X[4] = {1, dx, dx * dx, dx * dx * dx};
+  const R Y[4] = {1, dy, dy * dy, dy * dy * dy};
+  const R Z[4] = {1, dz, dz * dz, dz * dz * dz};
+  R result = 0;
   for (int i = 0; i < 4; ++i)
     for (int j = 0; j < 4; ++j)
       for (int k = 0; k < 4; ++k) result += coef[16 * i + 4 * j + k] * X[i] * Y[j] * Z[k];
-  return result;
+  return (float)result;
+}
+
+template <typename R>
+V3f gradientCubicT(const OracleGrid &g, V3f p, uint32_t attr)
+{
+  const V3i ic = floorToInt(p);
+  R coef[64];
+  tricubicCoefficients<R>(g, ic, attr, coef);
+  const V3f d = {p.x - (float)ic.x, p.y - (float)ic.y, p.z - (float)ic.z};
+  const R dx = d.x, dy = d.y, dz = d.z;
+  const R X[4] = {1, dx, dx * dx, dx * dx * dx};
+  const R Y[4] = {1, dy, dy * dy, dy * dy * dy};
+  const R Z[4] = {1, dz, dz * dz, dz * dz * dz};
+  R rx = 0, ry = 0, rz = 0;
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j)
+      for (int k = 0; k < 4; ++k) {
+        const R c = coef[16 * i + 4 * j + k];
+        if (i > 0) rx += c * (R)i * X[i - 1] * Y[j] * Z[k];
+        if (j > 0) ry += c * (R)j * X[i] * Y[j - 1] * Z[k];
+        if (k > 0) rz += c * (R)k * X[i] * Y[j] * Z[k - 1];
+      }
+  return V3f{(float)rx, (float)ry, (float)rz};
+}
+
+// The literal reference form is binary32; the binary64 form evaluates the
+// same algorithm (same matrix, same order) with a 1e-16 rounding unit, for
+// bounding the binary32 form's own rounding error.
+float sampleCubic(const OracleGrid &g, V3f p, uint32_t attr)
+{
+  return g.cubicDouble ? sampleCubicT<double>(g, p, attr) : sampleCubicT<float>(g, p, attr);
 }
 
 V3f gradientCubic(const OracleGrid &g, V3f p, uint32_t attr)
 {
-  const V3i ic = floorToInt(p);
-  float coef[64];
-  tricubicCoefficients(g, ic, attr, coef);
-  const V3f d = {p.x - (float)ic.x, p.y - (float)ic.y, p.z - (float)ic.z};
-  const float X[4] = {1.f, d.x, d.x * d.x, d.x * d.x * d.x};
-  const float Y[4] = {1.f, d.y, d.y * d.y, d.y * d.y * d.y};
-  const float Z[4] = {1.f, d.z, d.z * d.z, d.z * d.z * d.z};
-  V3f r{0.f, 0.f, 0.f};
-  for (int i = 0; i < 4; ++i)
-    for (int j = 0; j < 4; ++j)
-      for (int k = 0; k < 4; ++k) {
-        const float c = coef[16 * i + 4 * j + k];
-        if (i > 0) r.x += c * (float)i * X[i - 1] * Y[j] * Z[k];
-        if (j > 0) r.y += c * (float)j * X[i] * Y[j - 1] * Z[k];
-        if (k > 0) r.z += c * (float)k * X[i] * Y[j] * Z[k - 1];
-      }
-  return r;
+  return g.cubicDouble ? gradientCubicT<double>(g, p, attr) : gradientCubicT<float>(g, p, attr);
 }
 
 float sampleIndex(const OracleGrid &g, int filter, V3f p, uint32_t attr)
@@ -1322,6 +1338,8 @@ OracleGrid *oracle_vdb_new(uint64_t numLeaves, const uint32_t *level, const int3
 }
 
 void oracle_free(OracleGrid *g) { delete g; }
+
+void oracle_set_cubic_double(OracleGrid *g, int on) { g->cubicDouble = on != 0; }
 
 void oracle_set_sampler(OracleGrid *g, int filter, int gradientFilter, int maxSamplingDepth)
 {

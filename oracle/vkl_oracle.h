@@ -57,6 +57,9 @@ void oracle_free(OracleGrid *g);
 const char *oracle_last_error(void);
 
 void oracle_set_sampler(OracleGrid *g, int filter, int gradientFilter, int maxSamplingDepth);
+/* Evaluate the tricubic filter (same matrix and order) in binary64 instead of
+ * the reference's binary32; used to bound the binary32 form's rounding. */
+void oracle_set_cubic_double(OracleGrid *g, int on);
 
 void oracle_sample(const OracleGrid *g, uint64_t n, const float *xyz, float *out, uint32_t attr);
 void oracle_sample_mt(const OracleGrid *g, uint64_t n, const float *xyz, float *out, uint32_t attr, int threads);
