@@ -75,6 +75,9 @@ def main():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--points-log2", type=int, default=26)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--brick", default="0,0,0", help="device voxel layout: brick shifts sx,sy,sz (0,0,0 = x-fastest)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one 1 GiB device-to-device copy (a known-size kernel for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported "
@@ -89,7 +92,8 @@ def main():
     dev = vkl.Device("hip")
     log(f"building {n}^3 volume ...")
     vol_t = make_volume(torch, n)
-    vol = dev.structured_regular(vol_t)
+    brick = tuple(int(v) for v in args.brick.split(","))
+    vol = dev.structured_regular(vol_t, brick_shift=brick)
     sampler = vol.sampler(filter=vkl.VKL_FILTER_LINEAR)
     g = torch.Generator(device="cuda").manual_seed(SEED + 1)
     pts = torch.rand((npts, 3), device="cuda", generator=g, dtype=torch.float32) * float(n - 1)
@@ -112,6 +116,15 @@ def main():
     ms_event = start.elapsed_time(stop) / args.steps
     value = npts * args.steps / wall / 1e6
 
+    if args.copy_calibration:
+        src = torch.empty(1 << 28, device="cuda", dtype=torch.float32).fill_(1.0)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy calibration: 1 GiB read + 1 GiB written")
+        del src, dst
+
     U = distinct_corner_voxels(torch, pts, n)
     bytes_per_launch = npts * (12 + 4) + 4 * U
     achieved = bytes_per_launch / (ms_event * 1e-3) / 1e9
@@ -133,7 +146,8 @@ def main():
         "data": "synthetic (seeded wavelet + U(0,0.1) noise volume; seeded uniform points)",
         "config": {"workload": f"structuredRegular {n}^3 float32 vertex-centered, VKL_FILTER_LINEAR, "
                                f"vklComputeSampleN, 2^{args.points_log2} uniform random points in bounds",
-                   "volume_dims": [n, n, n], "points": npts, "filter": "linear", "exact": True},
+                   "volume_dims": [n, n, n], "points": npts, "filter": "linear", "exact": True,
+                   "device_layout": "x-fastest" if brick == (0, 0, 0) else "bricks %dx%dx%d" % tuple(1 << b for b in brick)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "algorithmic_bytes_per_launch": bytes_per_launch, "kernel_ms": round(ms_event, 4)},

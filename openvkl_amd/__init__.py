@@ -253,7 +253,7 @@ class Device:
         return Volume(self, h)
 
     def structured_regular(self, voxels, origin=(0, 0, 0), spacing=(1, 1, 1), filter=None, cell_centered=None,
-                           background=None, index_origin=None, index_to_object=None):
+                           background=None, index_origin=None, index_to_object=None, brick_shift=None):
         """voxels: array [nz, ny, nx] (x fastest) or a list of them."""
         arrs = voxels if isinstance(voxels, (list, tuple)) else [voxels]
         v = self.new_volume("structuredRegular")
@@ -278,6 +278,8 @@ class Device:
             v.set_float("background", background)
         if index_origin is not None:
             v.set_vec3i("indexOrigin", *index_origin)
+        if brick_shift is not None:
+            v.set_vec3i("brickShift", *brick_shift)
         if index_to_object is not None:
             d = self.data(np.asarray(index_to_object, np.float32))
             v.set_data("indexToObject", d)

@@ -120,8 +120,8 @@ BatchLaunch tune(SampleBatchPlan &p, hipStream_t st)
 int32_t fastPathOf(const Sampler &s, const AttrView &a)
 {
   const GridView &g = s.view.grid;
-  const uint64_t nvox = (uint64_t)g.activeSize[0] * g.activeSize[1] * g.activeSize[2];
-  if (!g.dense || s.view.filter != kFilterLinear || nvox >= (1ull << 32)) return kBatchGeneric;
+  const uint64_t nbz = ((uint64_t)g.activeSize[2] + (1u << g.layout.sz) - 1) >> g.layout.sz;
+  if (!g.dense || s.view.filter != kFilterLinear || nbz * g.layout.strideZ >= (1ull << 32)) return kBatchGeneric;
   if (a.type == kTypeFloat) return kBatchDenseLinearF32;
   if (a.type == kTypeHalf) return kBatchDenseLinearF16;
   return kBatchGeneric;

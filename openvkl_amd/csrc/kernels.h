@@ -18,6 +18,11 @@ hipError_t launchGradient(const SamplerView &s, const AttrView &a, uint32_t attr
 hipError_t launchSampleM(const SamplerView &s, uint32_t m, const uint32_t *attrsDev, uint64_t n, const float *xyz,
                          float *out, hipStream_t stream);
 
+// Copy an x-fastest dims[0] x dims[1] x dims[2] array of elemSize-byte
+// elements into brick layout L (total = padded element count).
+hipError_t launchSwizzle(const void *src, void *dst, uint32_t elemSize, const uint32_t dims[3],
+                         const DenseLayout &L, uint64_t total, hipStream_t stream);
+
 // Per-leaf value ranges over the [-1, +10) voxel window (majorant build).
 // leafOffsets: 3 u32 root-relative leaf origins per leaf.
 hipError_t launchLeafRanges(const GridView &g, uint32_t attr, uint64_t numLeaves, const uint32_t *leafOffsets,

@@ -100,8 +100,9 @@ hipError_t dispatchDense(const SamplerView &s, const AttrView &a, uint32_t attr,
                          float *out, hipStream_t st, bool gradient)
 {
   const GridView &g = s.grid;
-  const uint64_t nvox = (uint64_t)g.activeSize[0] * g.activeSize[1] * g.activeSize[2];
-  if (nvox >= (1ull << 32)) return dispatchFilter<DenseSource<T, true>>(s, a, attr, n, xyz, out, st, gradient);
+  const uint64_t nbz = ((uint64_t)g.activeSize[2] + (1u << g.layout.sz) - 1) >> g.layout.sz;
+  if (nbz * g.layout.strideZ >= (1ull << 32))
+    return dispatchFilter<DenseSource<T, true>>(s, a, attr, n, xyz, out, st, gradient);
   return dispatchFilter<DenseSource<T, false>>(s, a, attr, n, xyz, out, st, gradient);
 }
 
@@ -145,6 +146,55 @@ hipError_t launchSampleM(const SamplerView &s, uint32_t m, const uint32_t *attrs
 {
   if (n == 0 || m == 0) return hipSuccess;
   hipLaunchKernelGGL(kSampleM, dim3(blocksFor(n)), dim3(kBlock), 0, stream, s, m, attrsDev, n, xyz, out);
+  return hipGetLastError();
+}
+
+// Re-lay an x-fastest array into the brick layout L (one work-item per
+// destination element; padding elements are zero and never read).
+template <typename E>
+__global__ __launch_bounds__(kBlock) void kSwizzle(const E *__restrict__ src, E *__restrict__ dst, uint32_t nx,
+                                                    uint32_t ny, uint32_t nz, DenseLayout L, uint64_t nbx,
+                                                    uint64_t nby, uint64_t total)
+{
+  const uint64_t d = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (d >= total) return;
+  const uint32_t S     = L.sx + L.sy + L.sz;
+  const uint64_t b     = d >> S;
+  const uint32_t inner = (uint32_t)(d & ((1ull << S) - 1));
+  const uint64_t bx = b % nbx, by = (b / nbx) % nby, bz = b / (nbx * nby);
+  const uint64_t x = (bx << L.sx) | (inner & ((1u << L.sx) - 1));
+  const uint64_t y = (by << L.sy) | ((inner >> L.sx) & ((1u << L.sy) - 1));
+  const uint64_t z = (bz << L.sz) | (inner >> (L.sx + L.sy));
+  E v{};
+  if (x < nx && y < ny && z < nz) v = src[(z * ny + y) * (uint64_t)nx + x];
+  dst[d] = v;
+}
+
+hipError_t launchSwizzle(const void *src, void *dst, uint32_t elemSize, const uint32_t dims[3],
+                         const DenseLayout &L, uint64_t total, hipStream_t stream)
+{
+  const uint64_t nbx = ((uint64_t)dims[0] + (1u << L.sx) - 1) >> L.sx;
+  const uint64_t nby = ((uint64_t)dims[1] + (1u << L.sy) - 1) >> L.sy;
+  const dim3 grid(blocksFor(total)), block(kBlock);
+  switch (elemSize) {
+  case 1:
+    hipLaunchKernelGGL(kSwizzle<uint8_t>, grid, block, 0, stream, (const uint8_t *)src, (uint8_t *)dst, dims[0],
+                       dims[1], dims[2], L, nbx, nby, total);
+    break;
+  case 2:
+    hipLaunchKernelGGL(kSwizzle<uint16_t>, grid, block, 0, stream, (const uint16_t *)src, (uint16_t *)dst, dims[0],
+                       dims[1], dims[2], L, nbx, nby, total);
+    break;
+  case 4:
+    hipLaunchKernelGGL(kSwizzle<uint32_t>, grid, block, 0, stream, (const uint32_t *)src, (uint32_t *)dst, dims[0],
+                       dims[1], dims[2], L, nbx, nby, total);
+    break;
+  case 8:
+    hipLaunchKernelGGL(kSwizzle<uint64_t>, grid, block, 0, stream, (const uint64_t *)src, (uint64_t *)dst, dims[0],
+                       dims[1], dims[2], L, nbx, nby, total);
+    break;
+  default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

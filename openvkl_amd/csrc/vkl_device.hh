@@ -12,6 +12,8 @@
 #include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "vkl_views.h"
 
 namespace vklhip {
@@ -101,15 +103,20 @@ VKL_DEV uint32_t offsetToLinear(uint32_t l, uint32_t x, uint32_t y, uint32_t z)
 
 // ---- voxel sources --------------------------------------------------------------
 
-// Dense x-fastest array (structuredRegular), VdbSampler_dense.ih:290-330 and
+// Dense voxels (structuredRegular), VdbSampler_dense.ih:290-330 and
 // VdbSampler_traverseAndSample.ih:314-349: out-of-domain corners read the
-// background.
+// background. Addressing per DenseLayout (x-fastest or brick-swizzled): each
+// axis contributes an independent term, so the 8 trilinear corners cost 6
+// axis terms and 8 adds.
 template <typename T, bool kIdx64>
 struct DenseSource
 {
+  using Off = typename std::conditional<kIdx64, uint64_t, uint32_t>::type;
   const T *data;
   uint32_t nx, ny, nz;
   int32_t rx, ry, rz;
+  uint32_t sx, sy, sz, S;
+  Off strideY, strideZ;
   float bg;
 
   VKL_DEV DenseSource(const GridView &g, const AttrView &a, uint32_t = 0)
@@ -120,16 +127,31 @@ struct DenseSource
         rx(g.rootOrigin[0]),
         ry(g.rootOrigin[1]),
         rz(g.rootOrigin[2]),
+        sx(g.layout.sx),
+        sy(g.layout.sy),
+        sz(g.layout.sz),
+        S(g.layout.sx + g.layout.sy + g.layout.sz),
+        strideY((Off)g.layout.strideY),
+        strideZ((Off)g.layout.strideZ),
         bg(a.background)
   {
+  }
+
+  VKL_DEV Off xTerm(uint32_t x) const { return ((Off)(x >> sx) << S) | (Off)(x & ((1u << sx) - 1)); }
+  VKL_DEV Off yTerm(uint32_t y) const
+  {
+    return (Off)(y >> sy) * strideY + (Off)((y & ((1u << sy) - 1)) << sx);
+  }
+  VKL_DEV Off zTerm(uint32_t z) const
+  {
+    return (Off)(z >> sz) * strideZ + (Off)((z & ((1u << sz) - 1)) << (sx + sy));
   }
 
   VKL_DEV float at(int32_t x, int32_t y, int32_t z) const
   {
     const uint32_t ox = (uint32_t)(x - rx), oy = (uint32_t)(y - ry), oz = (uint32_t)(z - rz);
     if (!(ox < nx && oy < ny && oz < nz)) return bg;
-    if (kIdx64) return toFloat(data[((uint64_t)oz * ny + oy) * (uint64_t)nx + ox]);
-    return toFloat(data[(oz * ny + oy) * nx + ox]);
+    return toFloat(data[xTerm(ox) + yTerm(oy) + zTerm(oz)]);
   }
 
   // Corners s[4x + 2y + z] of the cell at ic (VdbSampler_trilinear.ih:48-108):
@@ -138,17 +160,18 @@ struct DenseSource
   {
     const uint32_t ox = (uint32_t)(ic.x - rx), oy = (uint32_t)(ic.y - ry), oz = (uint32_t)(ic.z - rz);
     if (ox + 1 < nx && oy + 1 < ny && oz + 1 < nz && ox < nx && oy < ny && oz < nz) {
-      const uint64_t sy = nx, sz = (uint64_t)nx * ny;
-      const T *p = kIdx64 ? data + ((uint64_t)oz * ny + oy) * (uint64_t)nx + ox
-                          : data + ((oz * ny + oy) * nx + ox);
-      s[0] = toFloat(p[0]);
-      s[4] = toFloat(p[1]);
-      s[2] = toFloat(p[sy]);
-      s[6] = toFloat(p[sy + 1]);
-      s[1] = toFloat(p[sz]);
-      s[5] = toFloat(p[sz + 1]);
-      s[3] = toFloat(p[sz + sy]);
-      s[7] = toFloat(p[sz + sy + 1]);
+      const Off x0 = xTerm(ox), x1 = xTerm(ox + 1);
+      const Off y0 = yTerm(oy), y1 = yTerm(oy + 1);
+      const Off z0 = zTerm(oz), z1 = zTerm(oz + 1);
+      const Off a00 = y0 + z0, a01 = y0 + z1, a10 = y1 + z0, a11 = y1 + z1;
+      s[0] = toFloat(data[x0 + a00]);
+      s[4] = toFloat(data[x1 + a00]);
+      s[2] = toFloat(data[x0 + a10]);
+      s[6] = toFloat(data[x1 + a10]);
+      s[1] = toFloat(data[x0 + a01]);
+      s[5] = toFloat(data[x1 + a01]);
+      s[3] = toFloat(data[x0 + a11]);
+      s[7] = toFloat(data[x1 + a11]);
       return;
     }
 #pragma unroll
@@ -276,6 +299,13 @@ struct SparseSource
     for (int i = 0; i < 8; i++) s[i] = at(ic.x + ((i >> 2) & 1), ic.y + ((i >> 1) & 1), ic.z + (i & 1));
   }
 };
+
+// Whether dense offsets (including brick padding) can exceed 32 bits.
+VKL_DEV bool denseNeeds64(const GridView &g)
+{
+  const uint64_t nbz = ((uint64_t)g.activeSize[2] + (1u << g.layout.sz) - 1) >> g.layout.sz;
+  return nbz * g.layout.strideZ >= (1ull << 32);
+}
 
 // ---- filters ------------------------------------------------------------------
 
@@ -417,7 +447,7 @@ static __device__ __noinline__ float sampleObject(const SamplerView &s, uint32_t
   const AttrView a  = g.attrs[attr];
   const float noff  = g.constantCellData ? 0.f : 0.5f;
   if (g.dense) {
-    const bool big = (uint64_t)g.activeSize[0] * g.activeSize[1] * g.activeSize[2] >= (1ull << 32);
+    const bool big = denseNeeds64(g);
     switch (a.type) {
 #define VKL_DENSE_CASE(CODE, T)                                                       \
   case CODE:                                                                          \

@@ -48,6 +48,20 @@ struct LevelView
   uint64_t numNodes;
 };
 
+// Dense voxel addressing: element offset of root-relative voxel (x, y, z) is
+// X(x) + Y(y) + Z(z) with
+//   X(x) = ((x >> sx) << S) | (x & mx)
+//   Y(y) = (y >> sy) * strideY + ((y & my) << sx)
+//   Z(z) = (z >> sz) * strideZ + ((z & mz) << (sx + sy))
+// for bricks of 2^sx x 2^sy x 2^sz voxels (S = sx + sy + sz, bricks x-fastest).
+// sx = sy = sz = 0 is the reference's x-fastest array (strideY = nx,
+// strideZ = nx * ny).
+struct DenseLayout
+{
+  uint32_t sx, sy, sz;
+  uint64_t strideY, strideZ;
+};
+
 struct GridView
 {
   float o2i[12];          // object -> index, row-major 3x4
@@ -58,6 +72,7 @@ struct GridView
   int32_t constantCellData;
   uint32_t numAttributes;
   uint32_t maxSamplingDepth;
+  DenseLayout layout;         // dense volumes: voxel addressing
   const AttrView *attrs;      // device array [numAttributes]
   const uint64_t *leafSlot;   // vdb with per-leaf node.data: leaf index -> packed slot
   LevelView levels[3];

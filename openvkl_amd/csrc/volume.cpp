@@ -354,11 +354,48 @@ void DenseVolume::commit()
   }
   transformBounds(i2o, ibox, bbox);
 
+  // Voxel layout on the device: x-fastest (the caller's order) or bricks of
+  // 2^sx x 2^sy x 2^sz voxels ("brickShift", vec3i, each 0..4).
+  int32_t shift[3] = {0, 0, 0};
+  getVec3i("brickShift", shift);
+  for (int i = 0; i < 3; i++)
+    if (shift[i] < 0 || shift[i] > 4) throw VklError("brickShift components must be in [0, 4]");
+  DenseLayout &L = grid.layout;
+  L.sx = (uint32_t)shift[0];
+  L.sy = (uint32_t)shift[1];
+  L.sz = (uint32_t)shift[2];
+  const uint64_t nbx = ((uint64_t)dims[0] + (1u << L.sx) - 1) >> L.sx;
+  const uint64_t nby = ((uint64_t)dims[1] + (1u << L.sy) - 1) >> L.sy;
+  const uint64_t nbz = ((uint64_t)dims[2] + (1u << L.sz) - 1) >> L.sz;
+  const uint32_t S   = L.sx + L.sy + L.sz;
+  L.strideY          = nbx << S;
+  L.strideZ          = (nbx * nby) << S;
+  const bool bricked = S != 0;
+  const uint32_t udims[3] = {(uint32_t)dims[0], (uint32_t)dims[1], (uint32_t)dims[2]};
+
   const std::vector<float> bg = parseBackground(*this, grid.numAttributes);
   attrViews.clear();
   for (size_t a = 0; a < attrs.size(); a++) {
     AttrView av;
-    av.data       = deviceData(*this, *attrs[a], st);
+    if (bricked) {
+      // Swizzle from the caller's device buffer, or from a temporary upload.
+      const Data &d = *attrs[a];
+      std::unique_ptr<DeviceBuffer> tmp;
+      const void *src = d.base;
+      if (!(d.onDevice && d.compact())) {
+        std::vector<uint8_t> h = d.hostCopy();
+        tmp.reset(new DeviceBuffer(h.size()));
+        tmp->upload(h.data(), h.size(), st);
+        src = tmp->ptr;
+      }
+      const size_t es   = sizeOfType(d.dataType);
+      DeviceBuffer *dst = newBuffer((nbx * nby * nbz << S) * es);
+      checkHip(launchSwizzle(src, dst->ptr, (uint32_t)es, udims, L, (nbx * nby * nbz) << S, st), "swizzle");
+      checkHip(hipStreamSynchronize(st), "swizzle");
+      av.data = dst->ptr;
+    } else {
+      av.data = deviceData(*this, *attrs[a], st);
+    }
     av.tile       = nullptr;
     av.type       = (uint32_t)attrs[a]->dataType;
     av.background = bg[a];
