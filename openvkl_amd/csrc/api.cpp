@@ -130,6 +130,8 @@ void IteratorContext::commit()
     valuesBuf.upload(values.data(), values.size() * sizeof(float), device->stream);
     view.values = valuesBuf.as<float>();
   }
+  deviceView.allocate(sizeof(IteratorContextView));
+  deviceView.upload(&view, sizeof(IteratorContextView), device->stream);
   committed = true;
 }
 
@@ -271,9 +273,13 @@ void sampleMN(Sampler *s, uint64_t n, const vkl_vec3f *oc, float *out, uint32_t 
   if (n == 0 || m == 0) return;
   for (uint32_t k = 0; k < m; k++) (void)s->attr(attrs[k]);
   Staged in(oc, n * 12, d->stream, true);
-  Staged at(attrs, m * 4, d->stream, true);
   Staged o(out, n * m * 4, d->stream, false);
-  checkHip(launchSampleM(s->view, m, at.as<uint32_t>(), n, in.as<float>(), o.as<float>(), d->stream), "sampleM kernel");
+  // Sampler.h:226-244 layout samples[i * M + k]: one specialised launch per
+  // attribute (attributes may differ in element type), strided by M.
+  for (uint32_t k = 0; k < m; k++)
+    checkHip(launchSampleStrided(s->view, s->attr(attrs[k]), attrs[k], n, in.as<float>(), o.as<float>() + k, m,
+                                 d->stream),
+             "sampleM kernel");
   o.copyOut();
   syncDevice(d);
 }
@@ -433,7 +439,7 @@ void initHit(IteratorContext *ctx, const vkl_vec3f *o, const vkl_vec3f *dir, con
   float ray[9] = {o->x, o->y, o->z, dir->x, dir->y, dir->z, t ? t->lower : 0.f,
                   t ? t->upper : std::numeric_limits<float>::infinity(), time};
   checkHip(hipMemcpyAsync(s, ray, sizeof(ray), hipMemcpyHostToDevice, d->stream), "hipMemcpyAsync");
-  checkHip(launchHitInit(ctx->view, reinterpret_cast<HitState *>(s + 64), reinterpret_cast<float *>(s), d->stream),
+  checkHip(launchHitInit(ctx->deviceView.as<IteratorContextView>(), reinterpret_cast<HitState *>(s + 64), reinterpret_cast<float *>(s), d->stream),
            "hit init");
   checkHip(hipMemcpyAsync(&buf->state, s + 64, sizeof(HitState), hipMemcpyDeviceToHost, d->stream), "copy");
   syncDevice(d);
@@ -449,7 +455,7 @@ int iterateHit(HitBuffer *buf, VKLHit *out)
   d->makeCurrent();
   uint8_t *s = stepScratch(d);
   checkHip(hipMemcpyAsync(s + 64, &buf->state, sizeof(HitState), hipMemcpyHostToDevice, d->stream), "copy");
-  checkHip(launchHitNext(ctx->view, reinterpret_cast<HitState *>(s + 64), reinterpret_cast<HitOut *>(s + 1024),
+  checkHip(launchHitNext(ctx->deviceView.as<IteratorContextView>(), reinterpret_cast<HitState *>(s + 64), reinterpret_cast<HitOut *>(s + 1024),
                          reinterpret_cast<int *>(s), d->stream),
            "hit step");
   int result = 0;
@@ -1102,7 +1108,7 @@ OPENVKL_INTERFACE void vklIterateHitsBatch(const VKLHitIteratorContext *c, unsig
   Staged tm(times, times ? numRays * 4ull : 0, st, true);
   Staged out(hits, (uint64_t)numRays * maxHitsPerRay * sizeof(VKLHit), st, false);
   Staged cnt(numHits, numRays * 4ull, st, false);
-  checkHip(launchHitBatch(ctx->view, numRays, o.as<float>(), d.as<float>(), t.as<float>(), tm.as<float>(),
+  checkHip(launchHitBatch(ctx->deviceView.as<IteratorContextView>(), numRays, o.as<float>(), d.as<float>(), t.as<float>(), tm.as<float>(),
                           out.as<HitOut>(), maxHitsPerRay, cnt.as<uint32_t>(), st),
            "hit batch");
   out.copyOut();

@@ -14,9 +14,10 @@ hipError_t launchSample(const SamplerView &s, const AttrView &a, uint32_t attr, 
 // vklComputeGradientN: out[3i..3i+2] = object-space gradient.
 hipError_t launchGradient(const SamplerView &s, const AttrView &a, uint32_t attr, uint64_t n, const float *xyz,
                           float *out, hipStream_t stream);
-// vklComputeSampleMN: out[i*m + a] = sample of attribute attrs[a].
-hipError_t launchSampleM(const SamplerView &s, uint32_t m, const uint32_t *attrsDev, uint64_t n, const float *xyz,
-                         float *out, hipStream_t stream);
+// out[i * outStride] = sample(xyz[i]) of attribute `attr` (vklComputeSampleMN
+// issues one launch per requested attribute with outStride = M).
+hipError_t launchSampleStrided(const SamplerView &s, const AttrView &a, uint32_t attr, uint64_t n, const float *xyz,
+                               float *out, uint32_t outStride, hipStream_t stream);
 
 // Copy an x-fastest dims[0] x dims[1] x dims[2] array of elemSize-byte
 // elements into brick layout L (total = padded element count).
@@ -33,8 +34,9 @@ hipError_t launchLeafRanges(const GridView &g, uint32_t attr, uint64_t numLeaves
 hipError_t launchIntervalInit(const IteratorContextView &c, IntervalState *state, const float *ray, hipStream_t stream);
 hipError_t launchIntervalNext(const IteratorContextView &c, IntervalState *state, IntervalOut *out, int *result,
                               hipStream_t stream);
-hipError_t launchHitInit(const IteratorContextView &c, HitState *state, const float *ray, hipStream_t stream);
-hipError_t launchHitNext(const IteratorContextView &c, HitState *state, HitOut *out, int *result, hipStream_t stream);
+// Hit steps take the context in device memory (IteratorContext::deviceView).
+hipError_t launchHitInit(const IteratorContextView *c, HitState *state, const float *ray, hipStream_t stream);
+hipError_t launchHitNext(const IteratorContextView *c, HitState *state, HitOut *out, int *result, hipStream_t stream);
 
 // Batched iteration: one ray per thread; ray r writes up to maxOut results at
 // out[r*maxOut], count[r] = number written. org / dir: 3 floats per ray;
@@ -42,7 +44,7 @@ hipError_t launchHitNext(const IteratorContextView &c, HitState *state, HitOut *
 hipError_t launchIntervalBatch(const IteratorContextView &c, uint32_t numRays, const float *org, const float *dir,
                                const float *tRanges, IntervalOut *out, uint32_t maxOut, uint32_t *count,
                                hipStream_t stream);
-hipError_t launchHitBatch(const IteratorContextView &c, uint32_t numRays, const float *org, const float *dir,
+hipError_t launchHitBatch(const IteratorContextView *c, uint32_t numRays, const float *org, const float *dir,
                           const float *tRanges, const float *times, HitOut *out, uint32_t maxOut, uint32_t *count,
                           hipStream_t stream);
 

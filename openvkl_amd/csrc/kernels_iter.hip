@@ -119,13 +119,18 @@ __global__ void kIntervalNext(IteratorContextView c, IntervalState *s, IntervalO
   *result = r ? 1 : 0;
 }
 
-__global__ void kHitInit(IteratorContextView c, HitState *s, const float *ray)
+// The hit kernels read their context from device memory: the exact sampler
+// they call takes it by reference, and a kernel-argument copy would have to
+// be spilled to the private segment first.
+__global__ void kHitInit(const IteratorContextView *__restrict__ cp, HitState *s, const float *ray)
 {
+  const IteratorContextView &c = *cp;
   hitInit(c, *s, F3{ray[0], ray[1], ray[2]}, F3{ray[3], ray[4], ray[5]}, Range1f{ray[6], ray[7]}, ray[8]);
 }
 
-__global__ void kHitNext(IteratorContextView c, HitState *s, HitOut *out, int *result)
+__global__ void kHitNext(const IteratorContextView *__restrict__ cp, HitState *s, HitOut *out, int *result)
 {
+  const IteratorContextView &c = *cp;
   HitState st = *s;
   HitOut o;
   const bool r = hitNext(c, st, o);
@@ -145,12 +150,12 @@ hipError_t launchIntervalNext(const IteratorContextView &c, IntervalState *state
   hipLaunchKernelGGL(kIntervalNext, dim3(1), dim3(1), 0, st, c, state, out, result);
   return hipGetLastError();
 }
-hipError_t launchHitInit(const IteratorContextView &c, HitState *state, const float *ray, hipStream_t st)
+hipError_t launchHitInit(const IteratorContextView *c, HitState *state, const float *ray, hipStream_t st)
 {
   hipLaunchKernelGGL(kHitInit, dim3(1), dim3(1), 0, st, c, state, ray);
   return hipGetLastError();
 }
-hipError_t launchHitNext(const IteratorContextView &c, HitState *state, HitOut *out, int *result, hipStream_t st)
+hipError_t launchHitNext(const IteratorContextView *c, HitState *state, HitOut *out, int *result, hipStream_t st)
 {
   hipLaunchKernelGGL(kHitNext, dim3(1), dim3(1), 0, st, c, state, out, result);
   return hipGetLastError();
@@ -183,7 +188,7 @@ __global__ __launch_bounds__(kIterBlock) void kIntervalBatch(IteratorContextView
   count[r] = n;
 }
 
-__global__ __launch_bounds__(kIterBlock) void kHitBatch(IteratorContextView c, uint32_t numRays,
+__global__ __launch_bounds__(kIterBlock) void kHitBatch(const IteratorContextView *__restrict__ cp, uint32_t numRays,
                                                          const float *__restrict__ org, const float *__restrict__ dir,
                                                          const float *__restrict__ tRanges,
                                                          const float *__restrict__ times, HitOut *__restrict__ out,
@@ -191,6 +196,7 @@ __global__ __launch_bounds__(kIterBlock) void kHitBatch(IteratorContextView c, u
 {
   const uint32_t r = blockIdx.x * kIterBlock + threadIdx.x;
   if (r >= numRays) return;
+  const IteratorContextView &c = *cp;
   HitState st;
   hitInit(c, st, F3{org[3 * r], org[3 * r + 1], org[3 * r + 2]}, F3{dir[3 * r], dir[3 * r + 1], dir[3 * r + 2]},
           rayRange(tRanges, r), times ? times[r] : 0.f);
@@ -211,7 +217,7 @@ hipError_t launchIntervalBatch(const IteratorContextView &c, uint32_t numRays, c
   return hipGetLastError();
 }
 
-hipError_t launchHitBatch(const IteratorContextView &c, uint32_t numRays, const float *org, const float *dir,
+hipError_t launchHitBatch(const IteratorContextView *c, uint32_t numRays, const float *org, const float *dir,
                           const float *tRanges, const float *times, HitOut *out, uint32_t maxOut, uint32_t *count,
                           hipStream_t st)
 {

@@ -15,7 +15,8 @@ constexpr int kBlock = 256;
 
 template <typename Src, int kFilter>
 __global__ __launch_bounds__(kBlock) void kSample(GridView g, AttrView a, uint32_t attr, uint64_t n,
-                                                   const float *__restrict__ xyz, float *__restrict__ out, float noff)
+                                                   const float *__restrict__ xyz, float *__restrict__ out, float noff,
+                                                   uint32_t outStride)
 {
   const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
@@ -29,7 +30,7 @@ __global__ __launch_bounds__(kBlock) void kSample(GridView g, AttrView a, uint32
     r = sampleCubic(src, p);
   else
     r = sampleLinear(src, p);
-  out[i] = r;
+  out[i * outStride] = r;
 }
 
 // VdbSampler.ispc:258-335: index-space gradient, then xfmNormal with the
@@ -53,25 +54,13 @@ __global__ __launch_bounds__(kBlock) void kGradient(GridView g, AttrView a, uint
   out[3 * i + 2] = r.z;
 }
 
-// Sampler.h:226-244 layout samples[i * M + a]; attributes may differ in type,
-// so this path dispatches per attribute at run time.
-__global__ __launch_bounds__(kBlock) void kSampleM(SamplerView s, uint32_t m, const uint32_t *__restrict__ attrs,
-                                                    uint64_t n, const float *__restrict__ xyz,
-                                                    float *__restrict__ out)
-{
-  const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
-  const F3 oc = {xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]};
-  for (uint32_t k = 0; k < m; k++) out[i * m + k] = sampleObject(s, attrs[k], oc);
-}
-
 namespace {
 
 inline unsigned blocksFor(uint64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
 
 template <typename Src>
 hipError_t dispatchFilter(const SamplerView &s, const AttrView &a, uint32_t attr, uint64_t n, const float *xyz,
-                          float *out, hipStream_t st, bool gradient)
+                          float *out, uint32_t stride, hipStream_t st, bool gradient)
 {
   const GridView &g = s.grid;
   const float noff  = g.constantCellData ? 0.f : 0.5f;
@@ -86,42 +75,45 @@ hipError_t dispatchFilter(const SamplerView &s, const AttrView &a, uint32_t attr
       hipLaunchKernelGGL((kGradient<Src, kFilterLinear>), grid, block, 0, st, g, a, attr, n, xyz, out);
   } else {
     if (f == kFilterNearest)
-      hipLaunchKernelGGL((kSample<Src, kFilterNearest>), grid, block, 0, st, g, a, attr, n, xyz, out, noff);
+      hipLaunchKernelGGL((kSample<Src, kFilterNearest>), grid, block, 0, st, g, a, attr, n, xyz, out, noff,
+                         stride);
     else if (f == kFilterCubic)
-      hipLaunchKernelGGL((kSample<Src, kFilterCubic>), grid, block, 0, st, g, a, attr, n, xyz, out, noff);
+      hipLaunchKernelGGL((kSample<Src, kFilterCubic>), grid, block, 0, st, g, a, attr, n, xyz, out, noff,
+                         stride);
     else
-      hipLaunchKernelGGL((kSample<Src, kFilterLinear>), grid, block, 0, st, g, a, attr, n, xyz, out, noff);
+      hipLaunchKernelGGL((kSample<Src, kFilterLinear>), grid, block, 0, st, g, a, attr, n, xyz, out, noff,
+                         stride);
   }
   return hipGetLastError();
 }
 
 template <typename T>
 hipError_t dispatchDense(const SamplerView &s, const AttrView &a, uint32_t attr, uint64_t n, const float *xyz,
-                         float *out, hipStream_t st, bool gradient)
+                         float *out, uint32_t stride, hipStream_t st, bool gradient)
 {
   const GridView &g = s.grid;
   const uint64_t nbz = ((uint64_t)g.activeSize[2] + (1u << g.layout.sz) - 1) >> g.layout.sz;
   if (nbz * g.layout.strideZ >= (1ull << 32))
-    return dispatchFilter<DenseSource<T, true>>(s, a, attr, n, xyz, out, st, gradient);
-  return dispatchFilter<DenseSource<T, false>>(s, a, attr, n, xyz, out, st, gradient);
+    return dispatchFilter<DenseSource<T, true>>(s, a, attr, n, xyz, out, stride, st, gradient);
+  return dispatchFilter<DenseSource<T, false>>(s, a, attr, n, xyz, out, stride, st, gradient);
 }
 
 hipError_t dispatch(const SamplerView &s, uint32_t attr, const AttrView &a, uint64_t n, const float *xyz,
-                    float *out, hipStream_t st, bool gradient)
+                    float *out, uint32_t stride, hipStream_t st, bool gradient)
 {
   if (s.grid.dense) {
     switch (a.type) {
-    case kTypeUchar: return dispatchDense<uint8_t>(s, a, attr, n, xyz, out, st, gradient);
-    case kTypeShort: return dispatchDense<int16_t>(s, a, attr, n, xyz, out, st, gradient);
-    case kTypeUshort: return dispatchDense<uint16_t>(s, a, attr, n, xyz, out, st, gradient);
-    case kTypeHalf: return dispatchDense<__half>(s, a, attr, n, xyz, out, st, gradient);
-    case kTypeFloat: return dispatchDense<float>(s, a, attr, n, xyz, out, st, gradient);
-    case kTypeDouble: return dispatchDense<double>(s, a, attr, n, xyz, out, st, gradient);
+    case kTypeUchar: return dispatchDense<uint8_t>(s, a, attr, n, xyz, out, stride, st, gradient);
+    case kTypeShort: return dispatchDense<int16_t>(s, a, attr, n, xyz, out, stride, st, gradient);
+    case kTypeUshort: return dispatchDense<uint16_t>(s, a, attr, n, xyz, out, stride, st, gradient);
+    case kTypeHalf: return dispatchDense<__half>(s, a, attr, n, xyz, out, stride, st, gradient);
+    case kTypeFloat: return dispatchDense<float>(s, a, attr, n, xyz, out, stride, st, gradient);
+    case kTypeDouble: return dispatchDense<double>(s, a, attr, n, xyz, out, stride, st, gradient);
     default: return hipErrorInvalidValue;
     }
   }
-  if (a.type == kTypeHalf) return dispatchFilter<SparseSource<__half>>(s, a, attr, n, xyz, out, st, gradient);
-  if (a.type == kTypeFloat) return dispatchFilter<SparseSource<float>>(s, a, attr, n, xyz, out, st, gradient);
+  if (a.type == kTypeHalf) return dispatchFilter<SparseSource<__half>>(s, a, attr, n, xyz, out, stride, st, gradient);
+  if (a.type == kTypeFloat) return dispatchFilter<SparseSource<float>>(s, a, attr, n, xyz, out, stride, st, gradient);
   return hipErrorInvalidValue;
 }
 
@@ -131,22 +123,21 @@ hipError_t launchSample(const SamplerView &s, const AttrView &a, uint32_t attr, 
                         float *out, hipStream_t stream)
 {
   if (n == 0) return hipSuccess;
-  return dispatch(s, attr, a, n, xyz, out, stream, false);
+  return dispatch(s, attr, a, n, xyz, out, 1, stream, false);
 }
 
 hipError_t launchGradient(const SamplerView &s, const AttrView &a, uint32_t attr, uint64_t n, const float *xyz,
                           float *out, hipStream_t stream)
 {
   if (n == 0) return hipSuccess;
-  return dispatch(s, attr, a, n, xyz, out, stream, true);
+  return dispatch(s, attr, a, n, xyz, out, 1, stream, true);
 }
 
-hipError_t launchSampleM(const SamplerView &s, uint32_t m, const uint32_t *attrsDev, uint64_t n, const float *xyz,
-                         float *out, hipStream_t stream)
+hipError_t launchSampleStrided(const SamplerView &s, const AttrView &a, uint32_t attr, uint64_t n, const float *xyz,
+                               float *out, uint32_t outStride, hipStream_t stream)
 {
-  if (n == 0 || m == 0) return hipSuccess;
-  hipLaunchKernelGGL(kSampleM, dim3(blocksFor(n)), dim3(kBlock), 0, stream, s, m, attrsDev, n, xyz, out);
-  return hipGetLastError();
+  if (n == 0) return hipSuccess;
+  return dispatch(s, attr, a, n, xyz, out, outStride, stream, false);
 }
 
 // Re-lay an x-fastest array into the brick layout L (one work-item per

@@ -127,6 +127,8 @@ struct Volume : ManagedObject
   using ManagedObject::ManagedObject;
 
   GridView grid{};
+  float i2o[12]{};                  // index -> object, row-major 3x4
+  std::vector<int32_t> denseLeafOrigins;  // vdb: index-space origins of dense leaves, packed-slot order
   std::vector<AttrView> attrViews;  // host copies of grid.attrs
   float bbox[6]{};
   std::vector<Range1f> valueRanges;
@@ -180,6 +182,7 @@ struct IteratorContext : ManagedObject
   Sampler *sampler;
   IteratorContextView view{};
   DeviceBuffer rangesBuf, valuesBuf;
+  DeviceBuffer deviceView;  // copy of `view` in device memory (hit kernels)
   bool isHit;
   bool committed = false;
 
