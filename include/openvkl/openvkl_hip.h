@@ -74,6 +74,49 @@ OPENVKL_INTERFACE void vklGetSampleBatchPlanLaunch(VKLSampleBatchPlan plan, unsi
                                                    unsigned int *threads, unsigned int *tile);
 OPENVKL_INTERFACE void vklReleaseSampleBatchPlan(VKLSampleBatchPlan plan);
 
+/* ---- neural-field sampler ---------------------------------------------------
+ * A multiresolution hash-grid (16 levels x 2 binary16 features, 2^19 entries
+ * per level) + 32-64-64-1 ReLU MLP that stands in for a vdb volume's exact
+ * sampler. Trained on the GPU (binary16 MFMA forward / backward, binary32
+ * accumulation, Adam) on samples of the exact sampler; per-leaf error bounds
+ * decide where it may answer, everything else falls back to the exact
+ * kernel. */
+
+typedef struct VKLNeuralField_ *VKLNeuralField;
+
+/* Create an untrained field for `exactSampler` (a vdb sampler; its filter is
+ * what the field learns). maxResolution: finest grid resolution over the
+ * volume's bounding cube (0 = 2048). */
+OPENVKL_INTERFACE VKLNeuralField vklNewNeuralField(const VKLSampler *exactSampler, unsigned int attributeIndex,
+                                                   unsigned int maxResolution, unsigned int seed);
+OPENVKL_INTERFACE void vklReleaseNeuralField(VKLNeuralField field);
+/* `steps` Adam steps of `batchSize` fresh exact samples each (a fraction
+ * leafFraction drawn inside leaves, the rest in the bounding box). Writes the
+ * per-step MSE to lossHistory (host, may be NULL); returns the last one. */
+OPENVKL_INTERFACE float vklNeuralFieldTrain(VKLNeuralField field, unsigned int steps, unsigned int batchSize,
+                                            float learningRate, float leafFraction, unsigned int seed,
+                                            float *lossHistory);
+/* Per-leaf error bounds: a leaf is approved when the maximum |field - exact|
+ * over pointsPerLeaf random points in it, times `safety`, is <= tolerance.
+ * Returns the number of approved leaves. */
+OPENVKL_INTERFACE unsigned int vklNeuralFieldFinalize(VKLNeuralField field, float tolerance,
+                                                      unsigned int pointsPerLeaf, float safety, unsigned int seed);
+/* Sample N object-space points (device pointers). useFallback != 0: points
+ * outside approved leaves are sampled by the exact kernel (count written to
+ * *numFallback). workspace: NULL or >= 4 * N + 4 bytes of device memory. */
+OPENVKL_INTERFACE void vklNeuralFieldSampleN(VKLNeuralField field, unsigned int N, const vkl_vec3f *objectCoordinates,
+                                             float *samples, int useFallback, unsigned int *numFallback,
+                                             void *workspace);
+/* One forward + backward pass (host pointers) without an optimizer step:
+ * summed gradients of sum((y - t)^2); returns the loss sum. */
+OPENVKL_INTERFACE float vklNeuralFieldGradients(VKLNeuralField field, unsigned int N,
+                                                const vkl_vec3f *objectCoordinates, const float *targets,
+                                                float *gTable, float *gW1, float *gW2, float *gW3);
+/* Host copies of the binary16 parameters and the encoding constants. */
+OPENVKL_INTERFACE void vklNeuralFieldGetParams(VKLNeuralField field, uint16_t *table, uint16_t *W1, uint16_t *W2,
+                                               float *W3, unsigned int *resolutions, float *boxLo, float *invExtent);
+OPENVKL_INTERFACE unsigned int vklNeuralFieldNumApprovedLeaves(VKLNeuralField field, unsigned int *numLeaves);
+
 #ifdef __cplusplus
 }
 #endif

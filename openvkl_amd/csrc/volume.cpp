@@ -331,7 +331,6 @@ void DenseVolume::commit()
 
   float col[12] = {spacing[0], 0.f, 0.f, 0.f, spacing[1], 0.f, 0.f, 0.f, spacing[2], origin[0], origin[1], origin[2]};
   getAffine3f("indexToObject", col);
-  float i2o[12];
   buildTransforms(col, i2o, grid.o2i);
 
   const bool cellCentered = getBool("cellCentered", false);
@@ -561,7 +560,6 @@ void VdbVolume::commit()
 
   float col[12] = {1.f, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f, 0.f};
   getAffine3f("indexToObject", col);
-  float i2o[12];
   buildTransforms(col, i2o, grid.o2i);
 
   // computeBbox / computeRootOrigin (VdbVolume.cpp:71-80, 567-581).
@@ -593,6 +591,9 @@ void VdbVolume::commit()
   std::vector<uint32_t> offs(3 * n);
   for (uint64_t i = 0; i < n; i++)
     for (int c = 0; c < 3; c++) offs[3 * i + c] = (uint32_t)(origin[3 * i + c] - grid.rootOrigin[c]);
+  denseLeafOrigins.clear();
+  for (uint64_t i = 0; i < n; i++)
+    if (format[i] == VKL_FORMAT_DENSE_ZYX) denseLeafOrigins.insert(denseLeafOrigins.end(), &origin[3 * i], &origin[3 * i] + 3);
   // Leaf codes carry the packed index for packed input, else the input index.
   std::vector<uint64_t> leafId(n);
   for (uint64_t i = 0; i < n; i++) leafId[i] = packedD ? slot[i] : i;
