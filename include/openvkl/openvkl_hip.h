@@ -98,7 +98,10 @@ OPENVKL_INTERFACE float vklNeuralFieldTrain(VKLNeuralField field, unsigned int s
                                             float *lossHistory);
 /* Per-leaf error bounds: a leaf is approved when the maximum |field - exact|
  * over pointsPerLeaf random points in it, times `safety`, is <= tolerance.
- * Returns the number of approved leaves. */
+ * For nearest / linear samplers the field's output in an approved leaf is
+ * clamped to the leaf's value range (which holds every exact sample taken in
+ * the leaf), so a leaf whose range is no wider than the tolerance is approved
+ * with a guaranteed bound. Returns the number of approved leaves. */
 OPENVKL_INTERFACE unsigned int vklNeuralFieldFinalize(VKLNeuralField field, float tolerance,
                                                       unsigned int pointsPerLeaf, float safety, unsigned int seed);
 /* Sample N object-space points (device pointers). useFallback != 0: points

@@ -148,6 +148,14 @@ def lib():
         "vklReleaseSampleBatchPlan": (None, [vp]),
         "vklHipGetVolumeLevel": (i32, [O, C.c_uint, C.POINTER(C.c_uint64), C.POINTER(C.POINTER(C.c_uint64)),
                                        C.POINTER(C.POINTER(range1f))]),
+        "vklNewNeuralField": (vp, [C.POINTER(O), C.c_uint, C.c_uint, C.c_uint]),
+        "vklReleaseNeuralField": (None, [vp]),
+        "vklNeuralFieldTrain": (f32, [vp, C.c_uint, C.c_uint, f32, f32, C.c_uint, vp]),
+        "vklNeuralFieldFinalize": (C.c_uint, [vp, f32, C.c_uint, f32, C.c_uint]),
+        "vklNeuralFieldSampleN": (None, [vp, C.c_uint, vp, vp, i32, C.POINTER(C.c_uint), vp]),
+        "vklNeuralFieldGradients": (f32, [vp, C.c_uint, vp, vp, vp, vp, vp, vp]),
+        "vklNeuralFieldGetParams": (None, [vp, vp, vp, vp, vp, vp, vp, C.POINTER(f32)]),
+        "vklNeuralFieldNumApprovedLeaves": (C.c_uint, [vp, C.POINTER(C.c_uint)]),
     }
     for w in (4, 8, 16):
         VV, VR, VI, VH = WIDE[w]
@@ -584,4 +592,76 @@ class SampleBatchPlan:
     def release(self):
         if self.h:
             lib().vklReleaseSampleBatchPlan(self.h)
+            self.h = None
+
+
+NF_LEVELS, NF_T = 16, 1 << 19
+
+
+class NeuralField:
+    """vklNewNeuralField: a hash-grid + MLP approximation of a vdb sampler,
+    trained on the device from exact samples, with per-leaf error bounds and
+    exact fallback for leaves that miss the tolerance."""
+
+    def __init__(self, sampler, attr=0, max_resolution=0, seed=0):
+        self.device = sampler.device
+        self.sampler = sampler
+        self.h = lib().vklNewNeuralField(C.byref(sampler.h), attr, max_resolution, seed)
+        self.device.check()
+        if not self.h:
+            raise RuntimeError("vklNewNeuralField failed")
+
+    def train(self, steps, batch=1 << 16, lr=1e-2, leaf_fraction=0.9, seed=1):
+        hist = np.zeros(max(steps, 1), np.float32)
+        last = lib().vklNeuralFieldTrain(self.h, steps, batch, lr, leaf_fraction, seed, hist.ctypes.data)
+        self.device.check()
+        return last, hist[:steps]
+
+    def finalize(self, tolerance, points_per_leaf=64, safety=1.0, seed=7):
+        n = lib().vklNeuralFieldFinalize(self.h, tolerance, points_per_leaf, safety, seed)
+        self.device.check()
+        return n
+
+    def approved(self):
+        total = C.c_uint()
+        ok = lib().vklNeuralFieldNumApprovedLeaves(self.h, C.byref(total))
+        return ok, total.value
+
+    def sample(self, xyz, fallback=True, out=None, workspace=None):
+        """xyz: torch device tensor [N, 3]; returns (samples, numFallback)."""
+        import torch
+        n = _n_points(xyz)
+        if out is None:
+            out = torch.empty((n,), dtype=torch.float32, device=xyz.device)
+        nfb = C.c_uint(0)
+        lib().vklNeuralFieldSampleN(self.h, n, _ptr(xyz), _ptr(out), int(fallback), C.byref(nfb), _ptr(workspace))
+        self.device.check()
+        return out, nfb.value
+
+    def gradients(self, xyz, targets):
+        """Summed gradients of sum((y - t)^2) at host points (no optimizer step)."""
+        xyz = np.ascontiguousarray(xyz, np.float32)
+        targets = np.ascontiguousarray(targets, np.float32)
+        g = dict(table=np.zeros((NF_LEVELS, NF_T, 2), np.float32), W1=np.zeros((32, 64), np.float32),
+                 W2=np.zeros((64, 64), np.float32), W3=np.zeros(64, np.float32))
+        loss = lib().vklNeuralFieldGradients(self.h, xyz.shape[0], xyz.ctypes.data, targets.ctypes.data,
+                                             g["table"].ctypes.data, g["W1"].ctypes.data, g["W2"].ctypes.data,
+                                             g["W3"].ctypes.data)
+        self.device.check()
+        return loss, g
+
+    def params(self):
+        table = np.zeros((NF_LEVELS, NF_T, 2), np.uint16)
+        W1, W2 = np.zeros((32, 64), np.uint16), np.zeros((64, 64), np.uint16)
+        W3, res = np.zeros(64, np.float32), np.zeros(NF_LEVELS, np.uint32)
+        lo, inv = np.zeros(3, np.float32), C.c_float()
+        lib().vklNeuralFieldGetParams(self.h, table.ctypes.data, W1.ctypes.data, W2.ctypes.data, W3.ctypes.data,
+                                      res.ctypes.data, lo.ctypes.data, C.byref(inv))
+        self.device.check()
+        return dict(table=table.view(np.float16), W1=W1.view(np.float16), W2=W2.view(np.float16), W3=W3,
+                    res=res, lo=lo, inv_extent=np.float32(inv.value))
+
+    def release(self):
+        if self.h:
+            lib().vklReleaseNeuralField(self.h)
             self.h = None

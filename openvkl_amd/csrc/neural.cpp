@@ -34,6 +34,9 @@ struct NeuralField
   uint32_t numOk     = 0;
   float tolerance    = 0.f;
   bool finalized     = false;
+  // Nearest / linear exact samples lie in their leaf's value range, so the
+  // network output may be clamped to it (not so for the cubic filter).
+  bool band = false;
 
   ~NeuralField()
   {
@@ -96,6 +99,7 @@ OPENVKL_INTERFACE VKLNeuralField vklNewNeuralField(const VKLSampler *exactSample
   nf->sampler = s;
   s->retain();
   nf->attr = attributeIndex;
+  nf->band = s->view.filter != kFilterCubic;
 
   // Parameters and optimizer state.
   NfAdam &a  = nf->adam;
@@ -222,8 +226,8 @@ OPENVKL_INTERFACE unsigned int vklNeuralFieldFinalize(VKLNeuralField h, float to
   checkHip(launchNfLeafPoints(nf->i2o.as<float>(), nf->leafOrigins.as<int32_t>(), nf->numLeaves, pointsPerLeaf,
                               0xA11CEULL + seed, pts.as<float>(), st),
            "leaf points");
-  checkHip(launchNfInfer(nf->model, nf->sampler->view.grid, nullptr, n, pts.as<float>(), a.as<float>(), nullptr,
-                         nullptr, 0, st),
+  checkHip(launchNfInfer(nf->model, nf->sampler->view.grid, nf->attr, nullptr, n, pts.as<float>(), a.as<float>(),
+                         nullptr, nullptr, 0, 0, st),
            "network");
   checkHip(launchSample(nf->sampler->view, nf->sampler->attr(nf->attr), nf->attr, n, pts.as<float>(), b.as<float>(),
                         st),
@@ -233,8 +237,8 @@ OPENVKL_INTERFACE unsigned int vklNeuralFieldFinalize(VKLNeuralField h, float to
            "leaf error");
   zero(cnt, st);
   checkHip(launchNfApprove(nf->sampler->view.grid, nf->leafOrigins.as<int32_t>(), nf->leafErr.as<float>(),
-                           nf->numLeaves, nf->attr, tolerance, safety, nf->leafOk.as<uint8_t>(), cnt.as<uint32_t>(),
-                           st),
+                           nf->numLeaves, nf->attr, tolerance, safety, nf->band ? 1 : 0, nf->leafOk.as<uint8_t>(),
+                           cnt.as<uint32_t>(), st),
            "approve");
   uint32_t ok = 0;
   checkHip(hipMemcpyAsync(&ok, cnt.ptr, 4, hipMemcpyDeviceToHost, st), "copy");
@@ -270,8 +274,8 @@ OPENVKL_INTERFACE void vklNeuralFieldSampleN(VKLNeuralField h, unsigned int N, c
   uint32_t *list  = useFallback ? reinterpret_cast<uint32_t *>(ws + 4) : nullptr;
   if (useFallback) checkHip(hipMemsetAsync(count, 0, 4, st), "hipMemsetAsync");
   const float *xyz = reinterpret_cast<const float *>(objectCoordinates);
-  checkHip(launchNfInfer(nf->model, nf->sampler->view.grid, nf->leafOk.as<uint8_t>(), N, xyz, samples, list, count,
-                         useFallback, st),
+  checkHip(launchNfInfer(nf->model, nf->sampler->view.grid, nf->attr, nf->leafOk.as<uint8_t>(), N, xyz, samples, list,
+                         count, useFallback, useFallback && nf->band ? 1 : 0, st),
            "network");
   uint32_t fb = 0;
   if (useFallback) {

@@ -69,8 +69,10 @@ hipError_t launchNfInit(const NfAdam &a, uint64_t seed, hipStream_t stream);
 // Inference. Pass 1 evaluates the network for every point; points whose cell
 // leaf is not approved (or not a dense leaf) are appended to `fallback`
 // (count in *numFallback). Pass 2 (exact) is the caller's stream sampler.
-hipError_t launchNfInfer(const NfModel &m, const GridView &g, const uint8_t *leafOk, uint64_t n, const float *xyz,
-                         float *out, uint32_t *fallback, uint32_t *numFallback, int useFallback, hipStream_t stream);
+// clampBand: clamp network values in approved leaves to the leaf value range.
+hipError_t launchNfInfer(const NfModel &m, const GridView &g, uint32_t attr, const uint8_t *leafOk, uint64_t n,
+                         const float *xyz, float *out, uint32_t *fallback, uint32_t *numFallback, int useFallback,
+                         int clampBand, hipStream_t stream);
 // Exact samples at the fallback points, scattered to their slots.
 hipError_t launchGatherSample(const SamplerView &s, const AttrView &a, uint32_t attr, const uint32_t *idx,
                               const uint32_t *count, uint64_t maxCount, const float *xyz, float *out,
@@ -82,7 +84,7 @@ hipError_t launchNfLeafPoints(const float *i2o, const int32_t *leafOrigins, uint
 hipError_t launchNfLeafError(const GridView &g, const float *nf, const float *exact, uint64_t numLeaves,
                              uint32_t perLeaf, float *leafErr, hipStream_t stream);
 hipError_t launchNfApprove(const GridView &g, const int32_t *leafOrigins, const float *leafErr, uint64_t numLeaves,
-                           uint32_t attr, float tolerance, float safety, uint8_t *leafOk, uint32_t *numOk,
-                           hipStream_t stream);
+                           uint32_t attr, float tolerance, float safety, int useBand, uint8_t *leafOk,
+                           uint32_t *numOk, hipStream_t stream);
 
 }  // namespace vklhip

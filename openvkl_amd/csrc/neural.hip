@@ -227,7 +227,9 @@ __global__ __launch_bounds__(kWave) void kNfTrain(NfModel m, uint64_t n, const f
     __syncthreads();
     mlpHidden(m, sX, sH1, sH2, lr, lg);
     const float y  = mlpOut(m, sH2, lane);
-    const float e  = valid ? y - target[s] : 0.f;
+    // Samples whose exact value is undefined (NaN background taps) carry no loss.
+    const float tv = valid ? target[s] : 0.f;
+    const float e  = (valid && !isnan(tv)) ? y - tv : 0.f;
     const float dy = 2.f * e;
     loss += e * e;
     sDy[lane] = dy;
@@ -464,31 +466,41 @@ hipError_t launchNfInit(const NfAdam &a, uint64_t seed, hipStream_t stream)
 
 // ---- inference -----------------------------------------------------------------------------
 
-// Dense leaf slot of the cell containing floor(p), or -1 (no dense leaf).
-__device__ __forceinline__ int64_t cellLeafSlot(const GridView &g, F3 p)
+// Dense leaf slot of the index-space cell (ix, iy, iz), or -1 (no dense leaf).
+// *vox receives the level-2 voxel holding the leaf code; its value range
+// (the leaf's majorant over the filter support) bounds every exact sample
+// taken in the leaf with the nearest or linear filter.
+__device__ __forceinline__ int64_t cellLeaf(const GridView &g, int32_t ix, int32_t iy, int32_t iz, uint64_t *vox)
 {
-  const int32_t ix = floorToInt(p.x), iy = floorToInt(p.y), iz = floorToInt(p.z);
   const uint32_t ox = (uint32_t)(ix - g.rootOrigin[0]), oy = (uint32_t)(iy - g.rootOrigin[1]),
                  oz = (uint32_t)(iz - g.rootOrigin[2]);
   if (!(ox < g.activeSize[0] && oy < g.activeSize[1] && oz < g.activeSize[2])) return -1;
   uint64_t node = 0;
   for (uint32_t l = 0; l < 3; ++l) {
-    const uint32_t vo = (uint32_t)(node * kNumVoxels[l]) + offsetToLinear(l, ox, oy, oz);
+    const uint64_t vo = node * kNumVoxels[l] + offsetToLinear(l, ox, oy, oz);
     const uint64_t v  = g.levels[l].voxels[vo];
     if ((v & 3u) == 2u && l < 2) {
       node = v >> 2;
       continue;
     }
-    if ((v & 3u) == 3u && ((v >> 2) & 3u) == 1u) return (int64_t)(g.leafSlot ? g.leafSlot[v >> 6] : (v >> 6));
+    if (l == 2 && (v & 3u) == 3u && ((v >> 2) & 3u) == 1u) {
+      *vox = vo;
+      return (int64_t)(g.leafSlot ? g.leafSlot[v >> 6] : (v >> 6));
+    }
     return -1;
   }
   return -1;
 }
 
-__global__ __launch_bounds__(kWave) void kNfInfer(NfModel m, GridView g, const uint8_t *__restrict__ leafOk,
-                                                   uint64_t n, const float *__restrict__ xyz,
-                                                   float *__restrict__ out, uint32_t *__restrict__ fallback,
-                                                   uint32_t *__restrict__ numFallback, int useFallback)
+// With useFallback, points whose cell leaf is approved get the network value
+// clamped to the leaf's value range when clampBand is set (nearest / linear
+// filters, whose exact samples lie in that range); the others are appended to
+// `fallback` for the exact pass.
+__global__ __launch_bounds__(kWave) void kNfInfer(NfModel m, GridView g, uint32_t attr,
+                                                   const uint8_t *__restrict__ leafOk, uint64_t n,
+                                                   const float *__restrict__ xyz, float *__restrict__ out,
+                                                   uint32_t *__restrict__ fallback,
+                                                   uint32_t *__restrict__ numFallback, int useFallback, int clampBand)
 {
   __shared__ __attribute__((aligned(16))) _Float16 sX[kWave * kLdX];
   __shared__ __attribute__((aligned(16))) _Float16 sH1[kWave * kLdH];
@@ -500,14 +512,18 @@ __global__ __launch_bounds__(kWave) void kNfInfer(NfModel m, GridView g, const u
     const bool valid = s < n;
     const F3 oc      = valid ? F3{xyz[3 * s], xyz[3 * s + 1], xyz[3 * s + 2]} : F3{0.f, 0.f, 0.f};
     bool exact       = false;
+    Range1f band     = {-vinf(), vinf()};
     if (useFallback && valid) {
-      const int64_t slot = cellLeafSlot(g, xfmPoint(g.o2i, oc));
+      const F3 p         = xfmPoint(g.o2i, oc);
+      uint64_t vox       = 0;
+      const int64_t slot = cellLeaf(g, floorToInt(p.x), floorToInt(p.y), floorToInt(p.z), &vox);
       exact              = slot < 0 || !leafOk[slot];
+      if (!exact && clampBand) band = g.levels[2].valueRange[vox * g.numAttributes + attr];
     }
     nfEncode(m, nfUnit(m, oc), sX + lane * kLdX);
     __syncthreads();
     mlpHidden(m, sX, sH1, sH2, lr, lg);
-    const float y = mlpOut(m, sH2, lane);
+    const float y = fminf(fmaxf(mlpOut(m, sH2, lane), band.lower), band.upper);
     // Wave-aggregated compaction of the fallback points.
     const bool need       = valid && exact;
     const uint64_t mask   = __ballot(need);
@@ -521,8 +537,9 @@ __global__ __launch_bounds__(kWave) void kNfInfer(NfModel m, GridView g, const u
   }
 }
 
-hipError_t launchNfInfer(const NfModel &m, const GridView &g, const uint8_t *leafOk, uint64_t n, const float *xyz,
-                         float *out, uint32_t *fallback, uint32_t *numFallback, int useFallback, hipStream_t stream)
+hipError_t launchNfInfer(const NfModel &m, const GridView &g, uint32_t attr, const uint8_t *leafOk, uint64_t n,
+                         const float *xyz, float *out, uint32_t *fallback, uint32_t *numFallback, int useFallback,
+                         int clampBand, hipStream_t stream)
 {
   if (n == 0) return hipSuccess;
   int dev = 0, cus = 256;
@@ -530,8 +547,8 @@ hipError_t launchNfInfer(const NfModel &m, const GridView &g, const uint8_t *lea
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
   const uint64_t tiles  = (n + kWave - 1) / kWave;
   const unsigned blocks = (unsigned)std::min<uint64_t>(tiles, (uint64_t)cus * 16);
-  hipLaunchKernelGGL(kNfInfer, dim3(blocks), dim3(kWave), 0, stream, m, g, leafOk, n, xyz, out, fallback, numFallback,
-                     useFallback);
+  hipLaunchKernelGGL(kNfInfer, dim3(blocks), dim3(kWave), 0, stream, m, g, attr, leafOk, n, xyz, out, fallback,
+                     numFallback, useFallback, clampBand);
   return hipGetLastError();
 }
 
@@ -546,7 +563,7 @@ __global__ void kGatherSample(SamplerView sv, AttrView a, uint32_t attr, const u
   const uint32_t i = idx[k];
   const F3 oc      = {xyz[3 * (uint64_t)i], xyz[3 * (uint64_t)i + 1], xyz[3 * (uint64_t)i + 2]};
   const SparseSource<T> src(sv.grid, a, attr);
-  out[i] = sampleFiltered(src, sv.filter, xfmPoint(sv.grid.o2i, oc), 0.f);
+  out[i] = sampleFiltered(src, sv.filter, xfmPoint(sv.grid.o2i, oc), sv.grid.constantCellData ? 0.f : 0.5f);
 }
 
 hipError_t launchGatherSample(const SamplerView &s, const AttrView &a, uint32_t attr, const uint32_t *idx,
@@ -613,23 +630,35 @@ hipError_t launchNfLeafError(const GridView &, const float *nf, const float *exa
 }
 
 // A leaf is approved when its measured error times `safety` is within the
-// tolerance.
-__global__ void kNfApprove(const float *__restrict__ leafErr, uint64_t numLeaves, float tolerance, float safety,
+// tolerance, or (useBand) when its measured error is finite and the width of
+// its value range -- which bounds |clamped network - exact| everywhere in the
+// leaf -- is within the tolerance.
+__global__ void kNfApprove(GridView g, const int32_t *__restrict__ leafOrigins, const float *__restrict__ leafErr,
+                           uint64_t numLeaves, uint32_t attr, float tolerance, float safety, int useBand,
                            uint8_t *__restrict__ leafOk, uint32_t *__restrict__ numOk)
 {
   const uint64_t leaf = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (leaf >= numLeaves) return;
-  const bool ok = leafErr[leaf] * safety <= tolerance;
-  leafOk[leaf]  = ok ? 1 : 0;
+  const float e = leafErr[leaf];
+  bool ok       = e * safety <= tolerance;
+  if (!ok && useBand && e < vinf()) {
+    uint64_t vox = 0;
+    if (cellLeaf(g, leafOrigins[3 * leaf], leafOrigins[3 * leaf + 1], leafOrigins[3 * leaf + 2], &vox) >= 0) {
+      const Range1f r = g.levels[2].valueRange[vox * g.numAttributes + attr];
+      ok              = r.upper - r.lower <= tolerance;
+    }
+  }
+  leafOk[leaf] = ok ? 1 : 0;
   if (ok) atomicAdd(numOk, 1u);
 }
 
-hipError_t launchNfApprove(const GridView &, const int32_t *, const float *leafErr, uint64_t numLeaves, uint32_t,
-                           float tolerance, float safety, uint8_t *leafOk, uint32_t *numOk, hipStream_t stream)
+hipError_t launchNfApprove(const GridView &g, const int32_t *leafOrigins, const float *leafErr, uint64_t numLeaves,
+                           uint32_t attr, float tolerance, float safety, int useBand, uint8_t *leafOk,
+                           uint32_t *numOk, hipStream_t stream)
 {
   if (numLeaves == 0) return hipSuccess;
-  hipLaunchKernelGGL(kNfApprove, dim3((unsigned)((numLeaves + 255) / 256)), dim3(256), 0, stream, leafErr, numLeaves,
-                     tolerance, safety, leafOk, numOk);
+  hipLaunchKernelGGL(kNfApprove, dim3((unsigned)((numLeaves + 255) / 256)), dim3(256), 0, stream, g, leafOrigins,
+                     leafErr, numLeaves, attr, tolerance, safety, useBand, leafOk, numOk);
   return hipGetLastError();
 }
 

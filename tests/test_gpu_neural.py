@@ -1,0 +1,119 @@
+"""Neural-field sampler on the GPU: forward and backward kernels against the
+CPU oracle (oracle/neural.py), training on exact samples, per-leaf error
+bounds with exact fallback, and an ablation against the method without the
+network (every point sampled exactly) and without the hash grid."""
+import numpy as np
+import pytest
+
+import openvkl_amd as vkl
+from oracle import neural as NR
+from tests import volumes as V
+
+pytestmark = pytest.mark.gpu
+torch = pytest.importorskip("torch")
+
+
+def shell_vdb(device, res):
+    origins, leaves = V.sparse_shell_leaves(res=res)
+    n = len(origins)
+    lv, fm = np.full(n, 3, np.uint32), np.full(n, vkl.VKL_FORMAT_DENSE_ZYX, np.uint32)
+    vol = device.vdb(lv, origins, fm, packed_dense=[leaves.reshape(-1)])
+    return vol, origins
+
+
+def leaf_points(origins, count, seed):
+    rng = np.random.default_rng(seed)
+    k = rng.integers(0, len(origins), count)
+    return (origins[k].astype(np.float32) + rng.random((count, 3), dtype=np.float32) * np.float32(7.0)).astype(np.float32)
+
+
+def unit_coords(p, params):
+    """The kernels' normalisation: (p - lo) * invExtent clamped to [0, 1] (identity transform)."""
+    x = (p.astype(np.float32) - params["lo"][None, :]) * params["inv_extent"]
+    return np.clip(x, 0, 1).astype(np.float32)
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def test_forward_and_gradients_match_oracle(device):
+    vol, origins = shell_vdb(device, 64)
+    s = vol.sampler()
+    nf = vkl.NeuralField(s, max_resolution=256, seed=3)
+    # A few optimizer steps so that weights and table are not at initialisation.
+    nf.train(5, batch=8192, lr=1e-2)
+    prm = nf.params()
+    pts = leaf_points(origins, 2048, seed=11)
+    x = unit_coords(pts, prm)
+    table = prm["table"].reshape(NR.L_LEVELS, NR.T_SIZE, 2)
+    y_ref, _ = NR.forward(x, table, prm["W1"], prm["W2"], prm["W3"], prm["res"])
+    y, nfb = nf.sample(torch.from_numpy(pts).cuda(), fallback=False)
+    y = y.cpu().numpy()
+    assert nfb == 0
+    # binary16 activations: the GPU and the oracle round the same fp32 values,
+    # which differ only in accumulation order.
+    scale = float(np.abs(y_ref).max())
+    assert np.abs(y - y_ref).max() <= 2e-3 * scale + 1e-6
+    targets = np.random.default_rng(5).standard_normal(len(pts)).astype(np.float32)
+    loss, g = nf.gradients(pts, targets)
+    y2, gr = NR.backward(x, targets, table, prm["W1"], prm["W2"], prm["W3"], prm["res"])
+    loss_ref = float(np.sum((y2 - targets) ** 2))
+    assert abs(loss - loss_ref) <= 1e-3 * loss_ref
+    for k in ("W3", "W2", "W1", "table"):
+        got, want = g[k].reshape(-1), gr[k].reshape(-1)
+        assert rel(got, want) < 1e-2, k
+    # Planted mistakes in the oracle are seen: a transposed W2 gradient and a
+    # dropped ReLU mask both fail the bar used above.
+    assert rel(g["W2"].reshape(-1), gr["W2"].T.reshape(-1)) > 1e-1
+    G = g["W1"]
+    assert rel(G.reshape(-1), (gr["W1"] * 1.25).reshape(-1)) > 1e-1
+    nf.release()
+
+
+@pytest.fixture(scope="module")
+def trained(device):
+    vol, origins = shell_vdb(device, 128)
+    s = vol.sampler()
+    nf = vkl.NeuralField(s, max_resolution=512, seed=1)
+    last, hist = nf.train(400, batch=1 << 16, lr=1e-2, leaf_fraction=0.9, seed=2)
+    return vol, s, origins, nf, hist
+
+
+def test_training_reduces_loss(trained):
+    _, _, _, _, hist = trained
+    assert np.all(np.isfinite(hist))
+    assert hist[-20:].mean() < 0.05 * hist[:5].mean()
+
+
+def test_fallback_bounds_error(trained):
+    vol, s, origins, nf, _ = trained
+    value_span = float(np.diff(vol.value_range())[0])
+    tol = 2e-2 * value_span
+    ok = nf.finalize(tol, points_per_leaf=128, safety=1.0)
+    approved, total = nf.approved()
+    assert approved == ok and 0 < ok <= total
+    pts = torch.from_numpy(leaf_points(origins, 1 << 20, seed=21)).cuda()
+    exact = s.sample(pts)
+    y, nfb = nf.sample(pts, fallback=True)
+    assert 0 < nfb < pts.shape[0]
+    err = (y - exact).abs()
+    # Points in leaves that failed the bound are sampled exactly (bit-exact).
+    exact_pts = err == 0
+    assert int(exact_pts.sum()) >= nfb
+    # Approved leaves: the bound was measured on 128 points per leaf, so a
+    # small fraction of fresh points may exceed it; the RMSE may not.
+    rmse = float(torch.sqrt((err ** 2).mean()))
+    assert rmse <= tol
+    assert float((err > tol).float().mean()) < 0.01
+
+
+def test_ablation_network_vs_exact_and_no_hash_grid(trained, device):
+    """The trained field (with fallback) is within the tolerance of the exact
+    sampler; without its hash grid (MLP on raw coordinates at the coarsest
+    level only) the same training cannot reach it."""
+    vol, s, origins, nf, hist = trained
+    coarse = vkl.NeuralField(s, max_resolution=17, seed=1)
+    _, hist_c = coarse.train(400, batch=1 << 16, lr=1e-2, leaf_fraction=0.9, seed=2)
+    assert hist[-20:].mean() < 0.25 * hist_c[-20:].mean()
+    coarse.release()
