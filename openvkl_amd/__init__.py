@@ -408,11 +408,15 @@ class Volume(Handle):
              if count else np.zeros((0, na, 2), np.float32))
         return n.value, v, r
 
-    def sampler(self, filter=None, gradient_filter=None, max_sampling_depth=None):
+    def sampler(self, filter=None, gradient_filter=None, max_sampling_depth=None, binned=None):
+        """binned: None = automatic, False / True = \"binnedSampling\" 0 / 1 (spatially
+        binned stream sampling of dense volumes, linear filter)."""
         h = lib().vklNewSampler(self.h)
         self.device.check()
         s = Sampler(self.device, h)
         s.volume = self
+        if binned is not None:
+            s.set_int("binnedSampling", int(bool(binned)))
         if filter is not None:
             s.set_int("filter", filter)
         if gradient_filter is not None:

@@ -15,6 +15,7 @@
 #include <cstring>
 #include <limits>
 
+#include "binned.h"
 #include "kernels.h"
 #include "objects.h"
 #include "openvkl/openvkl_hip.h"
@@ -46,6 +47,7 @@ void Sampler::commit()
   view.grid.maxSamplingDepth = (uint32_t)std::max(md, 0);
   view.filter                = f;
   view.gradientFilter        = gf;
+  binnedMode                 = getInt("binnedSampling", -1);
   device->makeCurrent();
   if (!deviceView.ptr) deviceView.allocate(sizeof(SamplerView));
   deviceView.upload(&view, sizeof(view), device->stream);
@@ -240,15 +242,85 @@ Sampler *samplerOf(const VKLSampler *s)
   return sp;
 }
 
+// Binned sampling pays off once the volume is much larger than the L2 caches
+// and the batch is large enough to amortise the binning pass.
+constexpr uint64_t kBinnedMinPoints      = 1ull << 20;
+constexpr uint64_t kBinnedMinVolumeBytes = 64ull << 20;
+constexpr uint32_t kBinnedMaxBins        = 16384;  // LDS histogram of the binning kernel
+
+int maxLdsPerBlock(int device)
+{
+  static int cached[64] = {0};
+  int &c = cached[device & 63];
+  if (!c) {
+    int v = 65536;
+    (void)hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device);
+    c = v;
+  }
+  return c;
+}
+
+// Geometry of the binned path for this call, or false for the direct kernel.
+bool binnedGeometry(Sampler *s, const AttrView &a, uint64_t n, BinGeometry &geo)
+{
+  const GridView &g = s->view.grid;
+  if (s->binnedMode == 0 || !g.dense || s->view.filter != kFilterLinear) return false;
+  const uint32_t es = (uint32_t)sizeOfType((VKLDataType)a.type);
+  if (s->binnedMode < 0) {
+    const uint64_t bytes = (uint64_t)g.activeSize[0] * g.activeSize[1] * g.activeSize[2] * es;
+    if (n < kBinnedMinPoints || bytes < kBinnedMinVolumeBytes) return false;
+  }
+  // Half the LDS of a CU per brick (two bins resident per CU), at most the
+  // per-workgroup limit.
+  const size_t lds = std::min<size_t>((size_t)maxLdsPerBlock(s->device->hipDevice), 80u << 10);
+  return chooseBinGeometry(g.activeSize, es, lds, kBinnedMaxBins, geo);
+}
+
+// Scratch of a binned call (sampler-owned, grown on demand).
+BinScratch binnedScratch(Sampler *s, const BinGeometry &geo, uint64_t n)
+{
+  const uint64_t segs = (uint64_t)geo.numBins * 8;
+  const uint64_t cap  = (n + segs - 1) / segs * 5 / 4 + 64;
+  auto align          = [](uint64_t x) { return (x + 255) & ~255ull; };
+  const uint64_t recBytes = align(segs * cap * 16), cntBytes = align(segs * 4), ovBytes = align(n * 4);
+  const uint64_t total    = recBytes + cntBytes + ovBytes + 256;
+  if (s->binBuf.bytes < total) s->binBuf.allocate(total);
+  uint8_t *base = s->binBuf.as<uint8_t>();
+  BinScratch sc;
+  sc.records       = reinterpret_cast<float4 *>(base);
+  sc.segCount      = reinterpret_cast<uint32_t *>(base + recBytes);
+  sc.overflow      = reinterpret_cast<uint32_t *>(base + recBytes + cntBytes);
+  sc.overflowCount = reinterpret_cast<uint32_t *>(base + recBytes + cntBytes + ovBytes);
+  sc.capacity      = (uint32_t)cap;
+  return sc;
+}
+
+// One attribute into out[i * stride]: binned or direct.
+void sampleInto(Sampler *s, uint32_t attr, uint64_t n, const float *xyz, float *out, uint32_t stride)
+{
+  Device *d         = s->device;
+  const AttrView &a = s->attr(attr);
+  BinGeometry geo;
+  if (binnedGeometry(s, a, n, geo)) {
+    std::lock_guard<std::mutex> lock(s->binMutex);
+    const BinScratch sc = binnedScratch(s, geo, n);
+    checkHip(launchSampleBinned(s->view, a, geo, n, xyz, out, stride, sc, d->stream), "binned sample kernels");
+    // The scratch is reused by the next call: finish before releasing it.
+    syncDevice(d);
+    return;
+  }
+  checkHip(launchSampleStrided(s->view, a, attr, n, xyz, out, stride, d->stream), "sample kernel");
+}
+
 void sampleN(Sampler *s, uint64_t n, const vkl_vec3f *oc, float *out, uint32_t attr)
 {
   Device *d = s->device;
   d->makeCurrent();
-  const AttrView &a = s->attr(attr);
+  (void)s->attr(attr);
   if (n == 0) return;
   Staged in(oc, n * 12, d->stream, true);
   Staged o(out, n * 4, d->stream, false);
-  checkHip(launchSample(s->view, a, attr, n, in.as<float>(), o.as<float>(), d->stream), "sample kernel");
+  sampleInto(s, attr, n, in.as<float>(), o.as<float>(), 1);
   o.copyOut();
   syncDevice(d);
 }
@@ -276,10 +348,7 @@ void sampleMN(Sampler *s, uint64_t n, const vkl_vec3f *oc, float *out, uint32_t 
   Staged o(out, n * m * 4, d->stream, false);
   // Sampler.h:226-244 layout samples[i * M + k]: one specialised launch per
   // attribute (attributes may differ in element type), strided by M.
-  for (uint32_t k = 0; k < m; k++)
-    checkHip(launchSampleStrided(s->view, s->attr(attrs[k]), attrs[k], n, in.as<float>(), o.as<float>() + k, m,
-                                 d->stream),
-             "sampleM kernel");
+  for (uint32_t k = 0; k < m; k++) sampleInto(s, attrs[k], n, in.as<float>(), o.as<float>() + k, m);
   o.copyOut();
   syncDevice(d);
 }

@@ -169,6 +169,12 @@ struct Sampler : ManagedObject
   SamplerView view{};
   DeviceBuffer deviceView;  // SamplerView in device memory (the VKLSampler.device handle)
   bool committed = false;
+  // Spatially binned stream sampling ("binnedSampling": -1 automatic, 0 off,
+  // 1 whenever the volume and filter allow it). One scratch per sampler,
+  // serialised by binMutex (calls are synchronous).
+  int binnedMode = -1;
+  std::mutex binMutex;
+  DeviceBuffer binBuf;
 
   Sampler(Device *d, Volume *v);
   ~Sampler() override;
