@@ -51,6 +51,147 @@ def distinct_corner_voxels(torch, pts, n):
     return int(mask.sum().item())
 
 
+def timed(torch, fn, steps, warmup):
+    """Average ms per call of fn over `steps` calls (HIP events on the null
+    stream, which the library uses by default)."""
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    for _ in range(steps):
+        fn()
+    stop.record()
+    torch.cuda.synchronize()
+    return start.elapsed_time(stop) / steps
+
+
+def variant_direct(torch, vkl, vol, pts, out, steps, warmup, alg_bytes):
+    """The same workload through the direct gather kernel (no binning)."""
+    s = vol.sampler(filter=vkl.VKL_FILTER_LINEAR, binned=False)
+    ms = timed(torch, lambda: s.sample(pts, out=out), steps, warmup)
+    gbs = alg_bytes / (ms * 1e-3) / 1e9
+    return {"msamples_s": round(pts.shape[0] / ms / 1e3, 1), "ms": round(ms, 4), "achieved_gbs": round(gbs, 1),
+            "frac": round(gbs / HBM_PEAK_GBS, 4)}
+
+
+def variant_batched(torch, vkl, dev, steps, warmup):
+    """64 structuredRegular volumes, cube sides seeded in [32, 384], float32 and
+    half alternating, point counts log-uniform in [1e4, 1e7]: one persistent,
+    graph-captured launch with an autotuned shape (vklNewSampleBatchPlan)."""
+    rng = np.random.default_rng(SEED + 7)
+    g = torch.Generator(device="cuda").manual_seed(SEED + 8)
+    keep, samplers, pts, outs = [], [], [], []
+    alg = 0.0
+    for i in range(64):
+        side = int(rng.integers(32, 385))
+        dt = torch.float32 if i % 2 == 0 else torch.float16
+        data = torch.rand((side, side, side), device="cuda", generator=g).to(dt)
+        v = dev.structured_regular(data)
+        s = v.sampler(filter=vkl.VKL_FILTER_LINEAR)
+        count = int(np.exp(rng.uniform(np.log(1e4), np.log(1e7))))
+        p = torch.rand((count, 3), device="cuda", generator=g) * float(side - 1)
+        o = torch.empty(count, device="cuda")
+        vox = float(side) ** 3
+        alg += count * 16 + (4 if dt == torch.float32 else 2) * vox * (1 - np.exp(-8.0 * count / vox))
+        keep.append((data, v))
+        samplers.append(s)
+        pts.append(p)
+        outs.append(o)
+    plan = vkl.SampleBatchPlan(dev, samplers, pts, outs, autotune=True)
+    total = sum(int(p.shape[0]) for p in pts)
+    ms = timed(torch, plan.execute, steps, warmup)
+    blocks, threads, tile = plan.launch_shape()
+    plan.release()
+    gbs = alg / (ms * 1e-3) / 1e9
+    return {"msamples_s": round(total / ms / 1e3, 1), "ms": round(ms, 4), "points": total, "volumes": 64,
+            "achieved_gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
+            "launch": {"blocks": blocks, "threads": threads, "tile": tile}}
+
+
+def shell_vdb(torch, res, radius=0.35, width=0.0095, seed=SEED + 9):
+    """Sphere shell |r - R| < w modulated by 4-octave sine fBm, as 8^3 float32
+    leaves on the GPU. Returns (origins int32 [n,3] numpy, leaves [n*512] cuda)."""
+    nl, c = res // 8, res / 2.0
+    centres = torch.arange(nl, device="cuda", dtype=torch.float32) * 8 + 4.0
+    yy, xx = torch.meshgrid(centres, centres, indexing="ij")
+    margin = width + 4.0 * 3 ** 0.5 / res
+    orgs = []
+    for z in range(nl):
+        r = torch.sqrt((xx - c) ** 2 + (yy - c) ** 2 + (centres[z] - c) ** 2) / res
+        ky, kx = torch.nonzero((r - radius).abs() < margin, as_tuple=True)
+        orgs.append(torch.stack([kx * 8, ky * 8, torch.full_like(kx, z * 8)], 1))
+    origins = torch.cat(orgs).to(torch.int32)
+    rng = np.random.default_rng(seed)
+    freq = [0.02 * 2 ** k for k in range(4)]
+    phase = rng.uniform(0, 2 * np.pi, (4, 3))
+    l8 = torch.arange(8, device="cuda", dtype=torch.float32)
+    lx, ly, lz = torch.meshgrid(l8, l8, l8, indexing="ij")  # leaf order x*64 + y*8 + z
+    lx, ly, lz = lx.reshape(1, -1), ly.reshape(1, -1), lz.reshape(1, -1)
+    leaves = torch.empty((origins.shape[0], 512), device="cuda", dtype=torch.float32)
+    for a in range(0, origins.shape[0], 1 << 17):
+        o = origins[a:a + (1 << 17)].float()
+        px, py, pz = o[:, 0:1] + lx + 0.5, o[:, 1:2] + ly + 0.5, o[:, 2:3] + lz + 0.5
+        rr = torch.sqrt((px - c) ** 2 + (py - c) ** 2 + (pz - c) ** 2) / res
+        shell = torch.clamp(1.0 - (rr - radius).abs() / width, 0.0, 1.0)
+        noise = torch.zeros_like(px)
+        for k in range(4):
+            noise += 0.5 ** k * torch.sin(px * freq[k] + phase[k, 0]) * torch.sin(py * freq[k] + phase[k, 1]) \
+                * torch.sin(pz * freq[k] + phase[k, 2])
+        leaves[a:a + (1 << 17)] = shell * (1.0 + 0.5 * noise)
+    return origins.cpu().numpy(), leaves.reshape(-1)
+
+
+def leaf_points(torch, origins_np, count, seed):
+    """Uniform points inside the active leaves (the data-carrying region)."""
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    o = torch.from_numpy(origins_np).cuda().float()
+    k = torch.randint(0, o.shape[0], (count,), device="cuda", generator=g)
+    return (o[k] + torch.rand((count, 3), device="cuda", generator=g) * 7.999).contiguous()
+
+
+def variant_neural(torch, vkl, dev, res, steps, warmup, train_log2=28, infer_log2=26, batch_log2=16):
+    """Neural-field stand-in for vdb sampling: train on 2^28 exact samples,
+    per-leaf bounds with exact fallback, infer 2^26 leaf points; compared with
+    the exact vdb sampler on the same points (normalised RMSE = RMSE / value
+    range of the volume)."""
+    origins, leaves = shell_vdb(torch, res)
+    n = origins.shape[0]
+    vol = dev.vdb(np.full(n, 3, np.uint32), origins, np.full(n, vkl.VKL_FORMAT_DENSE_ZYX, np.uint32),
+                  packed_dense=[leaves])
+    s = vol.sampler(filter=vkl.VKL_FILTER_LINEAR)
+    span = float(np.diff(vol.value_range())[0])
+    nf = vkl.NeuralField(s, seed=1)
+    train_steps = 1 << (train_log2 - batch_log2)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, hist = nf.train(train_steps, batch=1 << batch_log2, lr=1e-2, leaf_fraction=0.9, seed=2)
+    train_s = time.perf_counter() - t0
+    tol = 1e-2 * span
+    approved = nf.finalize(tol, points_per_leaf=64, safety=1.0)
+    pts = leaf_points(torch, origins, 1 << infer_log2, SEED + 10)
+    exact = torch.empty(pts.shape[0], device="cuda")
+    approx = torch.empty_like(exact)
+    ws = torch.empty(pts.shape[0] + 1, device="cuda", dtype=torch.int32)
+    ms_exact = timed(torch, lambda: s.sample(pts, out=exact), steps, warmup)
+    fb = [0]
+
+    def run_nf():
+        fb[0] = nf.sample(pts, fallback=True, out=approx, workspace=ws)[1]
+
+    ms_nf = timed(torch, run_nf, steps, warmup)
+    err = (approx - exact).double()
+    nrmse = float(torch.sqrt((err ** 2).mean())) / span
+    nf.release()
+    return {"volume": f"vdb shell+fBm, {res}^3 index domain, {n} float32 8^3 leaves",
+            "exact_msamples_s": round(pts.shape[0] / ms_exact / 1e3, 1),
+            "neural_msamples_s": round(pts.shape[0] / ms_nf / 1e3, 1),
+            "speedup_vs_exact": round(ms_exact / ms_nf, 3), "nrmse": nrmse, "leaf_tolerance": tol,
+            "approved_leaves": int(approved), "fallback_fraction": round(fb[0] / pts.shape[0], 4),
+            "train_samples": train_steps << batch_log2, "train_s": round(train_s, 2),
+            "final_train_mse": float(hist[-16:].mean()), "points": int(pts.shape[0])}
+
+
 def cpu_baseline(data_np, pts_np, count):
     import oracle as O
 
@@ -78,6 +219,9 @@ def main():
     ap.add_argument("--brick", default="0,0,0", help="device voxel layout: brick shifts sx,sy,sz (0,0,0 = x-fastest)")
     ap.add_argument("--binned", choices=("auto", "on", "off"), default="auto",
                     help="spatially binned sampling (auto: on for large volumes and batches)")
+    ap.add_argument("--variants", default="direct,batched,neural",
+                    help="comma list of extra measurements: direct, batched, neural ('' = none)")
+    ap.add_argument("--nf-res", type=int, default=1024, help="index-domain edge of the neural-field vdb")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 1 GiB device-to-device copy (a known-size kernel for HBM counters)")
     args = ap.parse_args()
@@ -156,6 +300,20 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "algorithmic_bytes_per_launch": bytes_per_launch, "kernel_ms": round(ms_event, 4)},
     }
+    variants = [v for v in args.variants.split(",") if v]
+    extra = {}
+    for v in variants:
+        log(f"variant {v} ...")
+        if v == "direct":
+            extra["direct_gather"] = variant_direct(torch, vkl, vol, pts, out, args.steps, args.warmup,
+                                                    bytes_per_launch)
+        elif v == "batched":
+            extra["batched"] = variant_batched(torch, vkl, dev, args.steps, args.warmup)
+        elif v == "neural":
+            extra["neural_field"] = variant_neural(torch, vkl, dev, args.nf_res, args.steps, args.warmup)
+        log(f"  {json.dumps(extra[list(extra)[-1]])}")
+    if extra:
+        result["variants"] = extra
     if not args.no_cpu_baseline:
         log("timing the CPU oracle ...")
         data_np = vol_t.cpu().numpy()

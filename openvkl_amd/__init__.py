@@ -183,8 +183,12 @@ def _ptr(a):
     if a is None:
         return None
     if isinstance(a, np.ndarray):
+        if not a.flags["C_CONTIGUOUS"]:
+            raise ValueError("arrays passed to the library must be C-contiguous")
         return a.ctypes.data
-    return a.data_ptr()  # torch.Tensor (host or device)
+    if not a.is_contiguous():  # torch.Tensor (host or device)
+        raise ValueError("tensors passed to the library must be contiguous")
+    return a.data_ptr()
 
 
 def _n_points(a):

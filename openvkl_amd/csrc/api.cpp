@@ -261,10 +261,10 @@ int maxLdsPerBlock(int device)
 }
 
 // Geometry of the binned path for this call, or false for the direct kernel.
-bool binnedGeometry(Sampler *s, const AttrView &a, uint64_t n, BinGeometry &geo)
+bool binnedGeometry(Sampler *s, const AttrView &a, uint64_t n, BinOp op, BinGeometry &geo)
 {
   const GridView &g = s->view.grid;
-  if (s->binnedMode == 0 || !g.dense || s->view.filter != kFilterLinear) return false;
+  if (s->binnedMode == 0 || !g.dense) return false;
   const uint32_t es = (uint32_t)sizeOfType((VKLDataType)a.type);
   if (s->binnedMode < 0) {
     const uint64_t bytes = (uint64_t)g.activeSize[0] * g.activeSize[1] * g.activeSize[2] * es;
@@ -273,7 +273,12 @@ bool binnedGeometry(Sampler *s, const AttrView &a, uint64_t n, BinGeometry &geo)
   // Half the LDS of a CU per brick (two bins resident per CU), at most the
   // per-workgroup limit.
   const size_t lds = std::min<size_t>((size_t)maxLdsPerBlock(s->device->hipDevice), 80u << 10);
-  return chooseBinGeometry(g.activeSize, es, lds, kBinnedMaxBins, geo);
+  return chooseBinGeometry(g.activeSize, es, op, lds, kBinnedMaxBins, geo);
+}
+
+BinOp sampleOp(int filter)
+{
+  return filter == kFilterNearest ? kBinSampleNearest : filter == kFilterCubic ? kBinSampleCubic : kBinSampleLinear;
 }
 
 // Scratch of a binned call (sampler-owned, grown on demand).
@@ -301,10 +306,11 @@ void sampleInto(Sampler *s, uint32_t attr, uint64_t n, const float *xyz, float *
   Device *d         = s->device;
   const AttrView &a = s->attr(attr);
   BinGeometry geo;
-  if (binnedGeometry(s, a, n, geo)) {
+  const BinOp op = sampleOp(s->view.filter);
+  if (binnedGeometry(s, a, n, op, geo)) {
     std::lock_guard<std::mutex> lock(s->binMutex);
     const BinScratch sc = binnedScratch(s, geo, n);
-    checkHip(launchSampleBinned(s->view, a, geo, n, xyz, out, stride, sc, d->stream), "binned sample kernels");
+    checkHip(launchBinned(s->view, a, geo, op, n, xyz, out, stride, sc, d->stream), "binned sample kernels");
     // The scratch is reused by the next call: finish before releasing it.
     syncDevice(d);
     return;
@@ -333,7 +339,17 @@ void gradientN(Sampler *s, uint64_t n, const vkl_vec3f *oc, vkl_vec3f *out, uint
   if (n == 0) return;
   Staged in(oc, n * 12, d->stream, true);
   Staged o(out, n * 12, d->stream, false);
-  checkHip(launchGradient(s->view, a, attr, n, in.as<float>(), o.as<float>(), d->stream), "gradient kernel");
+  BinGeometry geo;
+  const int gf   = s->view.gradientFilter;
+  const BinOp op = gf == kFilterCubic ? kBinGradientCubic : kBinGradientLinear;
+  if (gf != kFilterNearest && binnedGeometry(s, a, n, op, geo)) {
+    std::lock_guard<std::mutex> lock(s->binMutex);
+    const BinScratch sc = binnedScratch(s, geo, n);
+    checkHip(launchBinned(s->view, a, geo, op, n, in.as<float>(), o.as<float>(), 1, sc, d->stream),
+             "binned gradient kernels");
+  } else {
+    checkHip(launchGradient(s->view, a, attr, n, in.as<float>(), o.as<float>(), d->stream), "gradient kernel");
+  }
   o.copyOut();
   syncDevice(d);
 }

@@ -29,7 +29,7 @@ namespace vklhip {
 
 constexpr int kScatterBlock = 1024;
 constexpr int kScatterPts   = 16;
-constexpr int kSampleBlock  = 256;
+constexpr int kSampleBlock  = 512;
 constexpr int kXcds         = 8;
 
 // Root-relative cell clamped into [0, n - 1] (NaN positions land in cell 0).
@@ -95,24 +95,31 @@ __global__ __launch_bounds__(kScatterBlock) void kBinScatter(GridView g, BinGeom
   }
 }
 
-// Dense source whose corners come from an LDS brick when the whole 2x2x2
-// stencil is inside the domain and the brick; otherwise from the global
-// source (same values, same domain test).
+// Dense source whose taps come from an LDS brick when they lie inside the
+// domain and the brick; otherwise from the global source (same values, same
+// domain test, so results equal the direct kernels').
 template <typename T, bool kIdx64>
 struct LdsBrickSource
 {
   DenseSource<T, kIdx64> global;
   const T *brick;
-  uint32_t bx, by, bz;  // brick origin (root-relative voxels)
+  int32_t bx, by, bz;   // brick origin (root-relative voxels; may be -1)
   uint32_t ex, ey, ez;  // brick extents (elements)
 
-  VKL_DEV float at(int32_t x, int32_t y, int32_t z) const { return global.at(x, y, z); }
+  VKL_DEV float at(int32_t x, int32_t y, int32_t z) const
+  {
+    const uint32_t ox = (uint32_t)(x - global.rx), oy = (uint32_t)(y - global.ry), oz = (uint32_t)(z - global.rz);
+    const uint32_t lx = ox - (uint32_t)bx, ly = oy - (uint32_t)by, lz = oz - (uint32_t)bz;
+    if (ox < global.nx && oy < global.ny && oz < global.nz && lx < ex && ly < ey && lz < ez)
+      return toFloat(brick[(lz * ey + ly) * ex + lx]);
+    return global.at(x, y, z);
+  }
 
   VKL_DEV void corners8(I3 ic, float *s) const
   {
     const uint32_t ox = (uint32_t)(ic.x - global.rx), oy = (uint32_t)(ic.y - global.ry),
                    oz = (uint32_t)(ic.z - global.rz);
-    const uint32_t lx = ox - bx, ly = oy - by, lz = oz - bz;
+    const uint32_t lx = ox - (uint32_t)bx, ly = oy - (uint32_t)by, lz = oz - (uint32_t)bz;
     // In-domain test identical to DenseSource::corners8, plus brick cover.
     if (ox + 1 < global.nx && oy + 1 < global.ny && oz + 1 < global.nz && ox < global.nx && oy < global.ny &&
         oz < global.nz && lx + 1 < ex && ly + 1 < ey && lz + 1 < ez) {
@@ -132,37 +139,67 @@ struct LdsBrickSource
   }
 };
 
-template <typename T, bool kIdx64>
-__global__ __launch_bounds__(kSampleBlock) void kBinnedSample(GridView g, AttrView a, BinGeometry b, uint64_t n,
+// One point of a binned launch (the direct kernels' arithmetic).
+template <int kOp, typename Src>
+VKL_DEV void binnedPoint(const GridView &g, const Src &src, F3 p, float noff, uint64_t i, float *out,
+                         uint32_t outStride)
+{
+  if (kOp == kBinGradientLinear || kOp == kBinGradientCubic) {
+    const F3 gi = (kOp == kBinGradientCubic) ? gradientCubic(src, p) : gradientLinear(src, p);
+    const F3 r  = xfmNormal(g.o2i, gi);
+    out[3 * i]     = r.x;
+    out[3 * i + 1] = r.y;
+    out[3 * i + 2] = r.z;
+    return;
+  }
+  float r;
+  if (kOp == kBinSampleNearest)
+    r = sampleNearest(src, p, noff);
+  else if (kOp == kBinSampleCubic)
+    r = sampleCubic(src, p);
+  else
+    r = sampleLinear(src, p);
+  out[i * outStride] = r;
+}
+
+template <typename T, bool kIdx64, int kOp>
+__global__ __launch_bounds__(kSampleBlock) void kBinnedSample(GridView g, AttrView a, BinGeometry b,
                                                                const float *__restrict__ xyz,
                                                                float *__restrict__ out, uint32_t outStride,
-                                                               BinScratch s)
+                                                               float noff, BinScratch s)
 {
   extern __shared__ __attribute__((aligned(16))) unsigned char ldsRaw[];
   T *brick = reinterpret_cast<T *>(ldsRaw);
   const uint32_t bin = blockIdx.x;
   const uint32_t cx = bin % b.count[0], cy = (bin / b.count[0]) % b.count[1], cz = bin / (b.count[0] * b.count[1]);
-  LdsBrickSource<T, kIdx64> src{DenseSource<T, kIdx64>(g, a), brick, cx << b.shift[0], cy << b.shift[1],
-                                cz << b.shift[2], (1u << b.shift[0]) + 1, (1u << b.shift[1]) + 1,
-                                (1u << b.shift[2]) + 1};
-  const uint32_t total = src.ex * src.ey * src.ez;
-  // Stage the brick; voxels outside the domain are never read from LDS.
-  for (uint32_t e = threadIdx.x; e < total; e += kSampleBlock) {
-    const uint32_t lx = e % src.ex, ly = (e / src.ex) % src.ey, lz = e / (src.ex * src.ey);
-    const uint32_t x = src.bx + lx, y = src.by + ly, z = src.bz + lz;
-    T v{};
-    if (x < src.global.nx && y < src.global.ny && z < src.global.nz)
-      v = src.global.data[src.global.xTerm(x) + src.global.yTerm(y) + src.global.zTerm(z)];
-    brick[e] = v;
-  }
+  const int32_t lo = (int32_t)b.haloLo;
+  const uint32_t w = b.haloLo + b.haloHi;
+  LdsBrickSource<T, kIdx64> src{DenseSource<T, kIdx64>(g, a),
+                                brick,
+                                (int32_t)(cx << b.shift[0]) - lo,
+                                (int32_t)(cy << b.shift[1]) - lo,
+                                (int32_t)(cz << b.shift[2]) - lo,
+                                (1u << b.shift[0]) + w,
+                                (1u << b.shift[1]) + w,
+                                (1u << b.shift[2]) + w};
+  // Stage the brick row by row; voxels outside the domain are never read
+  // from LDS (their slots stay unwritten).
+  const uint32_t rows = src.ey * src.ez, perPass = kSampleBlock / src.ex, lx = threadIdx.x % src.ex;
+  if (threadIdx.x < perPass * src.ex)
+    for (uint32_t r = threadIdx.x / src.ex; r < rows; r += perPass) {
+      const uint32_t ly = r % src.ey, lz = r / src.ey;
+      const uint32_t x = (uint32_t)(src.bx + (int32_t)lx), y = (uint32_t)(src.by + (int32_t)ly),
+                     z = (uint32_t)(src.bz + (int32_t)lz);
+      if (x < src.global.nx && y < src.global.ny && z < src.global.nz)
+        brick[r * src.ex + lx] = src.global.data[src.global.xTerm(x) + src.global.yTerm(y) + src.global.zTerm(z)];
+    }
   __syncthreads();
   for (uint32_t seg = 0; seg < kXcds; seg++) {
     const uint32_t cnt = min(s.segCount[bin * kXcds + seg], s.capacity);
     const float4 *rec  = s.records + ((uint64_t)bin * kXcds + seg) * s.capacity;
     for (uint32_t k = threadIdx.x; k < cnt; k += kSampleBlock) {
       const float4 r = rec[k];
-      const uint32_t i = __float_as_uint(r.w);
-      out[(uint64_t)i * outStride] = sampleLinear(src, F3{r.x, r.y, r.z});
+      binnedPoint<kOp>(g, src, F3{r.x, r.y, r.z}, noff, __float_as_uint(r.w), out, outStride);
     }
   }
   // Overflow records (segments that were full): direct path.
@@ -170,51 +207,74 @@ __global__ __launch_bounds__(kSampleBlock) void kBinnedSample(GridView g, AttrVi
   for (uint64_t k = (uint64_t)bin * kSampleBlock + threadIdx.x; k < over; k += (uint64_t)b.numBins * kSampleBlock) {
     const uint32_t i = s.overflow[k];
     const F3 oc      = {xyz[3 * (uint64_t)i], xyz[3 * (uint64_t)i + 1], xyz[3 * (uint64_t)i + 2]};
-    out[(uint64_t)i * outStride] = sampleLinear(src.global, xfmPoint(g.o2i, oc));
+    binnedPoint<kOp>(g, src.global, xfmPoint(g.o2i, oc), noff, i, out, outStride);
   }
 }
 
 namespace {
 
-template <typename T>
-hipError_t launchBinnedSampleT(const GridView &g, const AttrView &a, const BinGeometry &b, uint64_t n,
-                               const float *xyz, float *out, uint32_t outStride, const BinScratch &s,
-                               hipStream_t st)
+template <typename T, int kOp>
+hipError_t launchBinnedT(const GridView &g, const AttrView &a, const BinGeometry &b, const float *xyz, float *out,
+                         uint32_t outStride, float noff, const BinScratch &s, hipStream_t st)
 {
   const size_t lds   = binnedBrickBytes(b, sizeof(T));
   const uint64_t nbz = ((uint64_t)g.activeSize[2] + (1u << g.layout.sz) - 1) >> g.layout.sz;
   if (nbz * g.layout.strideZ >= (1ull << 32))
-    hipLaunchKernelGGL((kBinnedSample<T, true>), dim3(b.numBins), dim3(kSampleBlock), lds, st, g, a, b, n, xyz, out,
-                       outStride, s);
+    hipLaunchKernelGGL((kBinnedSample<T, true, kOp>), dim3(b.numBins), dim3(kSampleBlock), lds, st, g, a, b, xyz,
+                       out, outStride, noff, s);
   else
-    hipLaunchKernelGGL((kBinnedSample<T, false>), dim3(b.numBins), dim3(kSampleBlock), lds, st, g, a, b, n, xyz, out,
-                       outStride, s);
+    hipLaunchKernelGGL((kBinnedSample<T, false, kOp>), dim3(b.numBins), dim3(kSampleBlock), lds, st, g, a, b, xyz,
+                       out, outStride, noff, s);
   return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launchBinnedOp(const GridView &g, const AttrView &a, const BinGeometry &b, BinOp op, const float *xyz,
+                          float *out, uint32_t outStride, float noff, const BinScratch &s, hipStream_t st)
+{
+  switch (op) {
+  case kBinSampleNearest: return launchBinnedT<T, kBinSampleNearest>(g, a, b, xyz, out, outStride, noff, s, st);
+  case kBinSampleLinear: return launchBinnedT<T, kBinSampleLinear>(g, a, b, xyz, out, outStride, noff, s, st);
+  case kBinSampleCubic: return launchBinnedT<T, kBinSampleCubic>(g, a, b, xyz, out, outStride, noff, s, st);
+  case kBinGradientLinear: return launchBinnedT<T, kBinGradientLinear>(g, a, b, xyz, out, outStride, noff, s, st);
+  case kBinGradientCubic: return launchBinnedT<T, kBinGradientCubic>(g, a, b, xyz, out, outStride, noff, s, st);
+  default: return hipErrorInvalidValue;
+  }
+}
+
+void haloOf(BinOp op, uint32_t &lo, uint32_t &hi)
+{
+  const bool cubic = op == kBinSampleCubic || op == kBinGradientCubic;
+  lo               = cubic ? 1 : 0;
+  hi               = cubic ? 2 : 1;
 }
 
 }  // namespace
 
 size_t binnedBrickBytes(const BinGeometry &b, uint32_t elemSize)
 {
-  return (size_t)((1u << b.shift[0]) + 1) * ((1u << b.shift[1]) + 1) * ((1u << b.shift[2]) + 1) * elemSize;
+  const uint32_t w = b.haloLo + b.haloHi;
+  return (size_t)((1u << b.shift[0]) + w) * ((1u << b.shift[1]) + w) * ((1u << b.shift[2]) + w) * elemSize;
 }
 
-bool chooseBinGeometry(const uint32_t dims[3], uint32_t elemSize, size_t maxBrickBytes, uint32_t maxBins,
+bool chooseBinGeometry(const uint32_t dims[3], uint32_t elemSize, BinOp op, size_t maxBrickBytes, uint32_t maxBins,
                        BinGeometry &out)
 {
-  // Candidate bricks, largest first; x gets the longest edge (rows are the
+  // Candidate regions, largest first; x gets the longest edge (rows are the
   // contiguous direction of the staging loads).
   static const uint32_t kShapes[][3] = {{6, 6, 5}, {6, 5, 5}, {5, 5, 5}, {5, 5, 4}, {5, 4, 4}, {4, 4, 4}, {4, 4, 3}};
   for (const auto &sh : kShapes) {
     BinGeometry b{};
+    haloOf(op, b.haloLo, b.haloHi);
     uint64_t bins = 1;
     for (int i = 0; i < 3; i++) {
       b.shift[i] = sh[i];
       b.count[i] = (dims[i] + (1u << sh[i]) - 1) >> sh[i];
       bins *= b.count[i];
     }
-    if (binnedBrickBytes(b, elemSize) > maxBrickBytes) continue;
-    if (bins > maxBins) return false;  // smaller bricks only add bins
+    // Rows of the staging loop must fit one workgroup.
+    if (binnedBrickBytes(b, elemSize) > maxBrickBytes || (1u << sh[0]) + b.haloLo + b.haloHi > kSampleBlock) continue;
+    if (bins > maxBins) return false;  // smaller regions only add bins
     b.numBins = (uint32_t)bins;
     out       = b;
     return true;
@@ -222,9 +282,8 @@ bool chooseBinGeometry(const uint32_t dims[3], uint32_t elemSize, size_t maxBric
   return false;
 }
 
-hipError_t launchSampleBinned(const SamplerView &sv, const AttrView &a, const BinGeometry &b, uint64_t n,
-                              const float *xyz, float *out, uint32_t outStride, const BinScratch &s,
-                              hipStream_t st)
+hipError_t launchBinned(const SamplerView &sv, const AttrView &a, const BinGeometry &b, BinOp op, uint64_t n,
+                        const float *xyz, float *out, uint32_t outStride, const BinScratch &s, hipStream_t st)
 {
   if (n == 0) return hipSuccess;
   hipError_t e = hipMemsetAsync(s.segCount, 0, (size_t)b.numBins * kXcds * sizeof(uint32_t), st);
@@ -237,13 +296,14 @@ hipError_t launchSampleBinned(const SamplerView &sv, const AttrView &a, const Bi
                      n, xyz, s);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
+  const float noff = sv.grid.constantCellData ? 0.f : 0.5f;
   switch (a.type) {
-  case kTypeUchar: return launchBinnedSampleT<uint8_t>(sv.grid, a, b, n, xyz, out, outStride, s, st);
-  case kTypeShort: return launchBinnedSampleT<int16_t>(sv.grid, a, b, n, xyz, out, outStride, s, st);
-  case kTypeUshort: return launchBinnedSampleT<uint16_t>(sv.grid, a, b, n, xyz, out, outStride, s, st);
-  case kTypeHalf: return launchBinnedSampleT<__half>(sv.grid, a, b, n, xyz, out, outStride, s, st);
-  case kTypeFloat: return launchBinnedSampleT<float>(sv.grid, a, b, n, xyz, out, outStride, s, st);
-  case kTypeDouble: return launchBinnedSampleT<double>(sv.grid, a, b, n, xyz, out, outStride, s, st);
+  case kTypeUchar: return launchBinnedOp<uint8_t>(sv.grid, a, b, op, xyz, out, outStride, noff, s, st);
+  case kTypeShort: return launchBinnedOp<int16_t>(sv.grid, a, b, op, xyz, out, outStride, noff, s, st);
+  case kTypeUshort: return launchBinnedOp<uint16_t>(sv.grid, a, b, op, xyz, out, outStride, noff, s, st);
+  case kTypeHalf: return launchBinnedOp<__half>(sv.grid, a, b, op, xyz, out, outStride, noff, s, st);
+  case kTypeFloat: return launchBinnedOp<float>(sv.grid, a, b, op, xyz, out, outStride, noff, s, st);
+  case kTypeDouble: return launchBinnedOp<double>(sv.grid, a, b, op, xyz, out, outStride, noff, s, st);
   default: return hipErrorInvalidValue;
   }
 }
