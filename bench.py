@@ -192,6 +192,55 @@ def variant_neural(torch, vkl, dev, res, steps, warmup, train_log2=28, infer_log
             "final_train_mse": float(hist[-16:].mean()), "points": int(pts.shape[0])}
 
 
+def boundary_rays(torch, lo, hi, count, seed):
+    """Origins uniform on the faces of the box [lo, hi], directions towards a
+    uniform point inside it (unit length)."""
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    lo_t = torch.tensor(lo, device="cuda", dtype=torch.float32)
+    ext = torch.tensor(hi, device="cuda", dtype=torch.float32) - lo_t
+    org = lo_t + torch.rand((count, 3), device="cuda", generator=g) * ext
+    axis = torch.randint(0, 3, (count,), device="cuda", generator=g)
+    side = torch.randint(0, 2, (count,), device="cuda", generator=g).float()
+    rows = torch.arange(count, device="cuda")
+    org[rows, axis] = lo_t[axis] + side * ext[axis]
+    tgt = lo_t + torch.rand((count, 3), device="cuda", generator=g) * ext
+    d = tgt - org
+    d = d / torch.linalg.norm(d, dim=1, keepdim=True)
+    return org.contiguous(), d.contiguous()
+
+
+def variant_iterators(torch, vkl, dev, res, steps, warmup, rays_log2=21):
+    """Sparse vdb (sphere shell + fBm, `res`^3 index domain, 8^3 float32
+    leaves): vklIterateIntervalsBatch (default hint) and vklIterateHitsBatch
+    (iso 0.5) over 2^21 rays entering the bounding box."""
+    origins, leaves = shell_vdb(torch, res)
+    n = origins.shape[0]
+    vol = dev.vdb(np.full(n, 3, np.uint32), origins, np.full(n, vkl.VKL_FORMAT_DENSE_ZYX, np.uint32),
+                  packed_dense=[leaves])
+    s = vol.sampler(filter=vkl.VKL_FILTER_LINEAR)
+    bb = vol.bounding_box()
+    org, d = boundary_rays(torch, bb[:3], bb[3:], 1 << rays_log2, SEED + 11)
+    ictx = s.interval_context()
+    hctx = s.hit_context([0.5])
+    res_i, res_h = {}, {}
+
+    def run_i():
+        res_i["out"], res_i["cnt"] = ictx.batch(org, d, max_per_ray=256)
+
+    def run_h():
+        res_h["out"], res_h["cnt"] = hctx.batch(org, d, max_per_ray=16)
+
+    ms_i = timed(torch, run_i, max(1, steps // 2), 1)
+    ms_h = timed(torch, run_h, max(1, steps // 2), 1)
+    ci, ch = res_i["cnt"].double(), res_h["cnt"].double()
+    r = org.shape[0]
+    return {"volume": f"vdb shell+fBm, {res}^3 index domain, {n} float32 8^3 leaves ({n * 2048 / 2 ** 30:.2f} GiB)",
+            "rays": r, "interval_mrays_s": round(r / ms_i / 1e3, 2), "interval_ms": round(ms_i, 3),
+            "intervals_per_ray": round(float(ci.mean()), 2), "max_intervals_per_ray": int(ci.max()),
+            "hit_mrays_s": round(r / ms_h / 1e3, 2), "hit_ms": round(ms_h, 3),
+            "hits_per_ray": round(float(ch.mean()), 3)}
+
+
 def cpu_baseline(data_np, pts_np, count):
     import oracle as O
 
@@ -219,9 +268,10 @@ def main():
     ap.add_argument("--brick", default="0,0,0", help="device voxel layout: brick shifts sx,sy,sz (0,0,0 = x-fastest)")
     ap.add_argument("--binned", choices=("auto", "on", "off"), default="auto",
                     help="spatially binned sampling (auto: on for large volumes and batches)")
-    ap.add_argument("--variants", default="direct,batched,neural",
-                    help="comma list of extra measurements: direct, batched, neural ('' = none)")
+    ap.add_argument("--variants", default="direct,batched,neural,iterators",
+                    help="comma list of extra measurements: direct, batched, neural, iterators ('' = none)")
     ap.add_argument("--nf-res", type=int, default=1024, help="index-domain edge of the neural-field vdb")
+    ap.add_argument("--vdb-res", type=int, default=4096, help="index-domain edge of the iterator vdb")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 1 GiB device-to-device copy (a known-size kernel for HBM counters)")
     args = ap.parse_args()
@@ -311,6 +361,8 @@ def main():
             extra["batched"] = variant_batched(torch, vkl, dev, args.steps, args.warmup)
         elif v == "neural":
             extra["neural_field"] = variant_neural(torch, vkl, dev, args.nf_res, args.steps, args.warmup)
+        elif v == "iterators":
+            extra["vdb_iterators"] = variant_iterators(torch, vkl, dev, args.vdb_res, args.steps, args.warmup)
         log(f"  {json.dumps(extra[list(extra)[-1]])}")
     if extra:
         result["variants"] = extra

@@ -65,32 +65,65 @@ def encode(x, table, res):
     return enc, cache
 
 
-def forward(x, table, W1, W2, W3, res):
-    enc, cache = encode(x, table, res)
-    X = enc.astype(np.float16).astype(np.float32)
-    h1 = np.maximum(X @ W1.astype(np.float32), 0).astype(np.float32)
-    H1 = h1.astype(np.float16).astype(np.float32)
-    h2 = np.maximum(H1 @ W2.astype(np.float32), 0).astype(np.float32)
-    H2 = h2.astype(np.float16).astype(np.float32)
-    y = (H2 @ W3.astype(np.float32)).reshape(-1)
+def _round(a, half):
+    """binary16 rounding of activations (the kernels' cast points), or none."""
+    return a.astype(np.float16).astype(np.float32) if half else a
+
+
+def forward(x, table, W1, W2, W3, res, half=True):
+    """half=False evaluates every step in binary64 without binary16 rounding
+    (the exact function the kernels approximate; used to pin the backward
+    against finite differences)."""
+    ft = np.float32 if half else np.float64
+    enc, cache = encode(x, table, res) if half else encode64(x, table, res)
+    X = _round(enc, half)
+    h1 = np.maximum(X @ W1.astype(ft), 0).astype(ft)
+    H1 = _round(h1, half)
+    h2 = np.maximum(H1 @ W2.astype(ft), 0).astype(ft)
+    H2 = _round(h2, half)
+    y = (H2 @ W3.astype(ft)).reshape(-1)
     return y, dict(X=X, H1=H1, H2=H2, cache=cache)
 
 
-def backward(x, target, table, W1, W2, W3, res):
+def encode64(x, table, res):
+    """encode() in binary64 (no rounding anywhere)."""
+    S = x.shape[0]
+    enc = np.zeros((S, L_LEVELS * F_FEAT), np.float64)
+    cache = []
+    for l in range(L_LEVELS):
+        pos = x.astype(np.float64) * float(res[l])
+        c = np.clip(np.floor(pos).astype(np.int64), 0, int(res[l]) - 1)
+        w = pos - c
+        c = c.astype(np.uint32)
+        idxs, wts = [], []
+        for k in range(8):
+            dx, dy, dz = (k >> 2) & 1, (k >> 1) & 1, k & 1
+            wk = (w[:, 0] if dx else 1 - w[:, 0]) * (w[:, 1] if dy else 1 - w[:, 1]) * (w[:, 2] if dz else 1 - w[:, 2])
+            idx = corner_index(res[l], c[:, 0] + dx, c[:, 1] + dy, c[:, 2] + dz)
+            enc[:, 2 * l:2 * l + 2] += wk[:, None] * table[l, idx].astype(np.float64)
+            idxs.append(idx)
+            wts.append(wk)
+        cache.append((np.stack(idxs, 1), np.stack(wts, 1)))
+    return enc, cache
+
+
+def backward(x, target, table, W1, W2, W3, res, half=True):
     """Gradients of sum((y - target)^2) (per-sample dy = 2 (y - t), not divided
-    by the batch size), the GPU's unscaled convention."""
-    y, st = forward(x, table, W1, W2, W3, res)
-    dy = (np.float32(2) * (y - target)).astype(np.float32)
+    by the batch size), the GPU's unscaled convention. half=False: binary64
+    without binary16 rounding."""
+    ft = np.float32 if half else np.float64
+    y, st = forward(x, table, W1, W2, W3, res, half)
+    dy = (ft(2) * (y - target)).astype(ft)
     H1, H2, X = st["H1"], st["H2"], st["X"]
     dW3 = H2.T @ dy
-    G2 = (dy[:, None] * W3.astype(np.float32).reshape(1, -1)) * (H2 > 0)
-    G2h = G2.astype(np.float16).astype(np.float32)
+    G2 = (dy[:, None] * W3.astype(ft).reshape(1, -1)) * (H2 > 0)
+    G2h = _round(G2, half)
     dW2 = H1.T @ G2h
-    G1 = (G2h @ W2.astype(np.float32).T) * (H1 > 0)
-    G1h = G1.astype(np.float16).astype(np.float32)
+    G1 = (G2h @ W2.astype(ft).T) * (H1 > 0)
+    G1h = _round(G1, half)
     dW1 = X.T @ G1h
-    dX = G1h @ W1.astype(np.float32).T
-    dtable = np.zeros(table.shape, np.float32)
+    dX = G1h @ W1.astype(ft).T
+    dtable = np.zeros(table.shape, ft)
     for l in range(L_LEVELS):
         idx, wts = st["cache"][l]
         for k in range(8):
