@@ -66,13 +66,22 @@ def timed(torch, fn, steps, warmup):
     return start.elapsed_time(stop) / steps
 
 
-def variant_direct(torch, vkl, vol, pts, out, steps, warmup, alg_bytes):
-    """The same workload through the direct gather kernel (no binning)."""
-    s = vol.sampler(filter=vkl.VKL_FILTER_LINEAR, binned=False)
-    ms = timed(torch, lambda: s.sample(pts, out=out), steps, warmup)
-    gbs = alg_bytes / (ms * 1e-3) / 1e9
-    return {"msamples_s": round(pts.shape[0] / ms / 1e3, 1), "ms": round(ms, 4), "achieved_gbs": round(gbs, 1),
-            "frac": round(gbs / HBM_PEAK_GBS, 4)}
+def variant_layouts(torch, vkl, dev, vol_t, pts, out, steps, warmup, alg_bytes):
+    """The same workload on the other device layouts of the same voxels: the
+    caller's x-fastest array (direct gather, and spatially binned), 4x4x2
+    bricks (direct gather)."""
+    res = {}
+    for name, kw, binned in (("x_fastest_direct", {}, False), ("x_fastest_binned", {}, True),
+                             ("bricks_4x4x2_direct", {"brick_shift": (2, 2, 1)}, False)):
+        v = dev.structured_regular(vol_t, **kw)
+        s = v.sampler(filter=vkl.VKL_FILTER_LINEAR, binned=binned)
+        ms = timed(torch, lambda: s.sample(pts, out=out), steps, warmup)
+        gbs = alg_bytes / (ms * 1e-3) / 1e9
+        res[name] = {"msamples_s": round(pts.shape[0] / ms / 1e3, 1), "ms": round(ms, 4),
+                     "achieved_gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+        s.release()
+        v.release()
+    return res
 
 
 def variant_batched(torch, vkl, dev, steps, warmup):
@@ -161,12 +170,14 @@ def variant_neural(torch, vkl, dev, res, steps, warmup, train_log2=28, infer_log
                   packed_dense=[leaves])
     s = vol.sampler(filter=vkl.VKL_FILTER_LINEAR)
     span = float(np.diff(vol.value_range())[0])
+    log(f"  vdb {res}^3 committed ({n} leaves); training")
     nf = vkl.NeuralField(s, seed=1)
     train_steps = 1 << (train_log2 - batch_log2)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     _, hist = nf.train(train_steps, batch=1 << batch_log2, lr=1e-2, leaf_fraction=0.9, seed=2)
     train_s = time.perf_counter() - t0
+    log(f"  trained {train_steps} steps in {train_s:.2f} s")
     tol = 1e-2 * span
     approved = nf.finalize(tol, points_per_leaf=64, safety=1.0)
     pts = leaf_points(torch, origins, 1 << infer_log2, SEED + 10)
@@ -180,16 +191,47 @@ def variant_neural(torch, vkl, dev, res, steps, warmup, train_log2=28, infer_log
         fb[0] = nf.sample(pts, fallback=True, out=approx, workspace=ws)[1]
 
     ms_nf = timed(torch, run_nf, steps, warmup)
-    err = (approx - exact).double()
+    # Points whose exact value is undefined (filter taps on the background
+    # outside the domain) are left out of the error.
+    defined = torch.isfinite(exact)
+    err = (approx - exact)[defined].double()
     nrmse = float(torch.sqrt((err ** 2).mean())) / span
+    net_only = torch.empty_like(exact)
+    ms_net = timed(torch, lambda: nf.sample(pts, fallback=False, out=net_only), steps, warmup)
+    nrmse_net = float(torch.sqrt(((net_only - exact)[defined].double() ** 2).mean())) / span
     nf.release()
     return {"volume": f"vdb shell+fBm, {res}^3 index domain, {n} float32 8^3 leaves",
             "exact_msamples_s": round(pts.shape[0] / ms_exact / 1e3, 1),
             "neural_msamples_s": round(pts.shape[0] / ms_nf / 1e3, 1),
             "speedup_vs_exact": round(ms_exact / ms_nf, 3), "nrmse": nrmse, "leaf_tolerance": tol,
+            "network_only_msamples_s": round(pts.shape[0] / ms_net / 1e3, 1), "network_only_nrmse": nrmse_net,
+            "undefined_fraction": round(1 - float(defined.float().mean()), 5),
             "approved_leaves": int(approved), "fallback_fraction": round(fb[0] / pts.shape[0], 4),
             "train_samples": train_steps << batch_log2, "train_s": round(train_s, 2),
             "final_train_mse": float(hist[-16:].mean()), "points": int(pts.shape[0])}
+
+
+def variant_raymarch(torch, vkl, vol, n, steps, warmup, width=1920, height=1080, march=512):
+    """Config-2 ray marching: 1920x1080 orthographic rays along +z through the
+    512^3 volume, 512 fixed steps each (samples at step midpoints), fused in
+    one kernel (vklComputeRayMarchN)."""
+    s = vol.sampler(filter=vkl.VKL_FILTER_LINEAR)
+    ys, xs = torch.meshgrid(torch.arange(height, device="cuda", dtype=torch.float32),
+                            torch.arange(width, device="cuda", dtype=torch.float32), indexing="ij")
+    r = width * height
+    org = torch.stack([(xs.reshape(-1) + 0.5) * ((n - 1) / width), (ys.reshape(-1) + 0.5) * ((n - 1) / height),
+                       torch.zeros(r, device="cuda")], 1).contiguous()
+    d = torch.zeros((r, 3), device="cuda")
+    d[:, 2] = 1.0
+    tr = torch.zeros((r, 2), device="cuda")
+    tr[:, 1] = float(n - 1)
+    out = torch.empty(r, device="cuda")
+    ms = timed(torch, lambda: s.ray_march(org, d, tr, march, out=out), steps, warmup)
+    samples = r * march
+    alg = r * (12 + 12 + 8 + 4) + n ** 3 * 4
+    return {"rays": r, "steps_per_ray": march, "ms": round(ms, 4), "msamples_s": round(samples / ms / 1e3, 1),
+            "mrays_s": round(r / ms / 1e3, 2), "algorithmic_gbs": round(alg / (ms * 1e-3) / 1e9, 1),
+            "note": "coherent gathers hit L1/L2; bound by vector ALU and L1 issue, not HBM"}
 
 
 def boundary_rays(torch, lo, hi, count, seed):
@@ -215,8 +257,10 @@ def variant_iterators(torch, vkl, dev, res, steps, warmup, rays_log2=21):
     (iso 0.5) over 2^21 rays entering the bounding box."""
     origins, leaves = shell_vdb(torch, res)
     n = origins.shape[0]
+    log(f"  vdb {res}^3: {n} leaves generated; committing")
     vol = dev.vdb(np.full(n, 3, np.uint32), origins, np.full(n, vkl.VKL_FORMAT_DENSE_ZYX, np.uint32),
                   packed_dense=[leaves])
+    log("  committed")
     s = vol.sampler(filter=vkl.VKL_FILTER_LINEAR)
     bb = vol.bounding_box()
     org, d = boundary_rays(torch, bb[:3], bb[3:], 1 << rays_log2, SEED + 11)
@@ -226,9 +270,13 @@ def variant_iterators(torch, vkl, dev, res, steps, warmup, rays_log2=21):
 
     def run_i():
         res_i["out"], res_i["cnt"] = ictx.batch(org, d, max_per_ray=256)
+        torch.cuda.synchronize()
+        log("  interval batch done")
 
     def run_h():
         res_h["out"], res_h["cnt"] = hctx.batch(org, d, max_per_ray=16)
+        torch.cuda.synchronize()
+        log("  hit batch done")
 
     ms_i = timed(torch, run_i, max(1, steps // 2), 1)
     ms_h = timed(torch, run_h, max(1, steps // 2), 1)
@@ -266,10 +314,14 @@ def main():
     ap.add_argument("--points-log2", type=int, default=26)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--brick", default="0,0,0", help="device voxel layout: brick shifts sx,sy,sz (0,0,0 = x-fastest)")
+    ap.add_argument("--layout", choices=("linear", "packed"), default="packed",
+                    help="device voxel layout: the caller's x-fastest array (with --brick) or corner-packed cells")
     ap.add_argument("--binned", choices=("auto", "on", "off"), default="auto",
                     help="spatially binned sampling (auto: on for large volumes and batches)")
-    ap.add_argument("--variants", default="direct,batched,neural,iterators",
-                    help="comma list of extra measurements: direct, batched, neural, iterators ('' = none)")
+    ap.add_argument("--variants", default="layouts,raymarch,batched,neural",
+                    help="comma list of extra measurements: layouts, raymarch, batched, neural, iterators ('' = "
+                         "none); iterators (4096^3 vdb, 2^21 rays) is held out of the default run: its last GPU "
+                         "run went silent for 180 s without a cause found yet")
     ap.add_argument("--nf-res", type=int, default=1024, help="index-domain edge of the neural-field vdb")
     ap.add_argument("--vdb-res", type=int, default=4096, help="index-domain edge of the iterator vdb")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -289,9 +341,11 @@ def main():
     log(f"building {n}^3 volume ...")
     vol_t = make_volume(torch, n)
     brick = tuple(int(v) for v in args.brick.split(","))
-    vol = dev.structured_regular(vol_t, brick_shift=brick)
+    packed = args.layout == "packed"
+    vol = dev.structured_regular(vol_t, brick_shift=None if packed else brick, corner_packed=packed or None)
     sampler = vol.sampler(filter=vkl.VKL_FILTER_LINEAR, binned={"auto": None, "on": True, "off": False}[args.binned])
-    binned = args.binned == "on" or (args.binned == "auto" and npts >= (1 << 20) and n ** 3 * 4 >= (64 << 20))
+    binned = args.binned == "on" or (args.binned == "auto" and not packed and npts >= (1 << 20)
+                                     and n ** 3 * 4 >= (64 << 20))
     g = torch.Generator(device="cuda").manual_seed(SEED + 1)
     pts = torch.rand((npts, 3), device="cuda", generator=g, dtype=torch.float32) * float(n - 1)
     out = torch.empty(npts, device="cuda", dtype=torch.float32)
@@ -344,7 +398,8 @@ def main():
         "config": {"workload": f"structuredRegular {n}^3 float32 vertex-centered, VKL_FILTER_LINEAR, "
                                f"vklComputeSampleN, 2^{args.points_log2} uniform random points in bounds",
                    "volume_dims": [n, n, n], "points": npts, "filter": "linear", "exact": True,
-                   "device_layout": "x-fastest" if brick == (0, 0, 0) else "bricks %dx%dx%d" % tuple(1 << b for b in brick),
+                   "device_layout": "corner-packed cells (8 x 4 B per cell)" if packed else
+                   "x-fastest" if brick == (0, 0, 0) else "bricks %dx%dx%d" % tuple(1 << b for b in brick),
                    "sampling_path": "binned (bin scatter + LDS-brick sample)" if binned else "direct gather"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
@@ -354,9 +409,13 @@ def main():
     extra = {}
     for v in variants:
         log(f"variant {v} ...")
-        if v == "direct":
-            extra["direct_gather"] = variant_direct(torch, vkl, vol, pts, out, args.steps, args.warmup,
-                                                    bytes_per_launch)
+        if v == "layouts":
+            extra["layouts"] = variant_layouts(torch, vkl, dev, vol_t, pts, out, args.steps, args.warmup,
+                                               bytes_per_launch)
+        elif v == "raymarch":
+            rv = dev.structured_regular(vol_t)
+            extra["raymarch"] = variant_raymarch(torch, vkl, rv, n, args.steps, args.warmup)
+            rv.release()
         elif v == "batched":
             extra["batched"] = variant_batched(torch, vkl, dev, args.steps, args.warmup)
         elif v == "neural":

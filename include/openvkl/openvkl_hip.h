@@ -23,6 +23,18 @@ OPENVKL_INTERFACE void vklComputeGradientNAsync(const VKLSampler *sampler, unsig
                                                 const vkl_vec3f *objectCoordinates, vkl_vec3f *gradients,
                                                 unsigned int attributeIndex, void *stream);
 
+/* Fixed-step ray marching fused with sampling (a consumer of the sampler, as
+ * the reference's ray-march renderer, examples/interactive/renderer/
+ * ray_march_iterator/RayMarchIterator.ispc:39-137, but without host round
+ * trips): integrals[r] = sum over s < numSteps of sample(origins[r] +
+ * t_s directions[r]) with t_s = t0 + (s + 0.5) (t1 - t0) / numSteps,
+ * [t0, t1] = tRanges[r], binary32 without contraction, summed in step order.
+ * Pointers may be host or device memory. */
+OPENVKL_INTERFACE void vklComputeRayMarchN(const VKLSampler *sampler, unsigned int numRays,
+                                           const vkl_vec3f *origins, const vkl_vec3f *directions,
+                                           const vkl_range1f *tRanges, unsigned int numSteps, float *integrals,
+                                           unsigned int attributeIndex);
+
 /* Iterate the intervals of numRays rays in one launch. Ray r writes up to
  * maxIntervalsPerRay intervals to intervals[r * maxIntervalsPerRay + k] and
  * their count to numIntervals[r]; the sequence equals calling

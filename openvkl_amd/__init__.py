@@ -140,6 +140,7 @@ def lib():
         "vklGetHitIteratorSize": (sz, [C.POINTER(O)]),
         "vklInitHitIterator": (vp, [C.POINTER(O), C.POINTER(vec3f), C.POINTER(vec3f), C.POINTER(range1f), f32, vp]),
         "vklIterateHit": (i32, [vp, C.POINTER(VKLHit)]),
+        "vklComputeRayMarchN": (None, [C.POINTER(O), C.c_uint, vp, vp, vp, C.c_uint, vp, C.c_uint]),
         "vklIterateIntervalsBatch": (None, [C.POINTER(O), C.c_uint, vp, vp, vp, vp, vp, C.c_uint, vp]),
         "vklIterateHitsBatch": (None, [C.POINTER(O), C.c_uint, vp, vp, vp, vp, vp, C.c_uint, vp]),
         "vklNewSampleBatchPlan": (vp, [C.c_uint, vp, vp, vp, vp, vp, i32]),
@@ -265,7 +266,8 @@ class Device:
         return Volume(self, h)
 
     def structured_regular(self, voxels, origin=(0, 0, 0), spacing=(1, 1, 1), filter=None, cell_centered=None,
-                           background=None, index_origin=None, index_to_object=None, brick_shift=None):
+                           background=None, index_origin=None, index_to_object=None, brick_shift=None,
+                           corner_packed=None):
         """voxels: array [nz, ny, nx] (x fastest) or a list of them."""
         arrs = voxels if isinstance(voxels, (list, tuple)) else [voxels]
         v = self.new_volume("structuredRegular")
@@ -292,6 +294,8 @@ class Device:
             v.set_vec3i("indexOrigin", *index_origin)
         if brick_shift is not None:
             v.set_vec3i("brickShift", *brick_shift)
+        if corner_packed is not None:
+            v.set_bool("cornerPacked", corner_packed)
         if index_to_object is not None:
             d = self.data(np.asarray(index_to_object, np.float32))
             v.set_data("indexToObject", d)
@@ -456,6 +460,14 @@ class Sampler(Handle):
         n = _n_points(xyz)
         out = self._out(xyz, (n, 3)) if out is None else out
         lib().vklComputeGradientN(C.byref(self.h), n, _ptr(xyz), _ptr(out), attr, None)
+        self.device.check()
+        return out
+
+    def ray_march(self, org, dir, t_ranges, steps, attr=0, out=None):
+        """vklComputeRayMarchN: per-ray sum of `steps` midpoint samples."""
+        r = _n_points(org)
+        out = self._out(org, (r,)) if out is None else out
+        lib().vklComputeRayMarchN(C.byref(self.h), r, _ptr(org), _ptr(dir), _ptr(t_ranges), steps, _ptr(out), attr)
         self.device.check()
         return out
 

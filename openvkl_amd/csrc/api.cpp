@@ -267,6 +267,8 @@ bool binnedGeometry(Sampler *s, const AttrView &a, uint64_t n, BinOp op, BinGeom
   if (s->binnedMode == 0 || !g.dense) return false;
   const uint32_t es = (uint32_t)sizeOfType((VKLDataType)a.type);
   if (s->binnedMode < 0) {
+    // One record load per stencil already: the direct kernel is the better fit.
+    if (g.layout.cornerPacked) return false;
     const uint64_t bytes = (uint64_t)g.activeSize[0] * g.activeSize[1] * g.activeSize[2] * es;
     if (n < kBinnedMinPoints || bytes < kBinnedMinVolumeBytes) return false;
   }
@@ -1139,6 +1141,29 @@ OPENVKL_INTERFACE void vklComputeSampleNAsync(const VKLSampler *s, unsigned int 
   checkHip(launchSample(sp->view, sp->attr(a), a, N, reinterpret_cast<const float *>(oc), out,
                         static_cast<hipStream_t>(stream)),
            "sample kernel");
+  VKL_CATCH()
+}
+
+OPENVKL_INTERFACE void vklComputeRayMarchN(const VKLSampler *s, unsigned int numRays, const vkl_vec3f *origins,
+                                           const vkl_vec3f *directions, const vkl_range1f *tRanges,
+                                           unsigned int numSteps, float *integrals, unsigned int a)
+{
+  VKL_TRY(samplerOf(s)->device)
+  Sampler *sp = samplerOf(s);
+  dev_->makeCurrent();
+  const AttrView &av = sp->attr(a);
+  if (numRays == 0) return;
+  if (numSteps == 0) throw VklError("numSteps must be positive", VKL_INVALID_ARGUMENT);
+  if (!origins || !directions || !tRanges || !integrals) throw VklError("null ray array", VKL_INVALID_ARGUMENT);
+  hipStream_t st = dev_->stream;
+  Staged o(origins, numRays * 12ull, st, true), d(directions, numRays * 12ull, st, true);
+  Staged t(tRanges, numRays * 8ull, st, true);
+  Staged out(integrals, numRays * 4ull, st, false);
+  checkHip(launchRayMarch(sp->view, av, a, numRays, o.as<float>(), d.as<float>(), t.as<float>(), numSteps,
+                          out.as<float>(), st),
+           "ray march kernel");
+  out.copyOut();
+  syncDevice(dev_);
   VKL_CATCH()
 }
 

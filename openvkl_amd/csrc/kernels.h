@@ -19,10 +19,21 @@ hipError_t launchGradient(const SamplerView &s, const AttrView &a, uint32_t attr
 hipError_t launchSampleStrided(const SamplerView &s, const AttrView &a, uint32_t attr, uint64_t n, const float *xyz,
                                float *out, uint32_t outStride, hipStream_t stream);
 
+// Fixed-step ray marching fused with sampling: out[r] = sum over s < steps of
+// sample(org[r] + t_s dir[r]), t_s = t0 + (s + 0.5) (t1 - t0) / steps with
+// [t0, t1] = tRange[r] (2 floats per ray), summed in step order.
+hipError_t launchRayMarch(const SamplerView &s, const AttrView &a, uint32_t attr, uint32_t numRays, const float *org,
+                          const float *dir, const float *tRange, uint32_t steps, float *out, hipStream_t stream);
+
 // Copy an x-fastest dims[0] x dims[1] x dims[2] array of elemSize-byte
 // elements into brick layout L (total = padded element count).
 hipError_t launchSwizzle(const void *src, void *dst, uint32_t elemSize, const uint32_t dims[3],
                          const DenseLayout &L, uint64_t total, hipStream_t stream);
+
+// Corner-packed copy (8 elements per cell, see DenseLayout) of an x-fastest
+// dims[0] x dims[1] x dims[2] array; dst holds 8 * cells elements.
+hipError_t launchPackCorners(const void *src, void *dst, uint32_t elemSize, const uint32_t dims[3],
+                             hipStream_t stream);
 
 // Per-leaf value ranges over the [-1, +10) voxel window (majorant build).
 // leafOffsets: 3 u32 root-relative leaf origins per leaf.

@@ -54,32 +54,55 @@ __global__ __launch_bounds__(kScatterBlock) void kBinScatter(GridView g, BinGeom
   extern __shared__ uint32_t hist[];
   const uint32_t nb = b.numBins;
   for (uint32_t i = threadIdx.x; i < nb; i += kScatterBlock) hist[i] = 0;
+
+  // Issue every point load of this thread before using any of them.
+  const uint64_t first = (uint64_t)blockIdx.x * (kScatterBlock * kScatterPts) + threadIdx.x;
+  F3 p[kScatterPts];
+#pragma unroll
+  for (int k = 0; k < kScatterPts; k++) {
+    const uint64_t i = first + (uint64_t)k * kScatterBlock;
+    const uint64_t j = i < n ? i : 0;
+    p[k]             = F3{xyz[3 * j], xyz[3 * j + 1], xyz[3 * j + 2]};
+  }
   __syncthreads();
 
-  const uint64_t first = (uint64_t)blockIdx.x * (kScatterBlock * kScatterPts) + threadIdx.x;
   // bin << 14 | rank within the workgroup (both < 2^14: at most 16384 bins
   // and 16384 points per workgroup).
   static_assert(kScatterBlock * kScatterPts <= (1 << 14), "rank field");
-  F3 p[kScatterPts];
   uint32_t binRank[kScatterPts];
 #pragma unroll
   for (int k = 0; k < kScatterPts; k++) {
     const uint64_t i = first + (uint64_t)k * kScatterBlock;
     binRank[k]       = 0xffffffffu;
     if (i < n) {
-      const F3 oc      = {xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]};
-      p[k]             = xfmPoint(g.o2i, oc);
+      p[k]             = xfmPoint(g.o2i, p[k]);
       const uint32_t c = binOf(b, g.rootOrigin, g.activeSize, p[k]);
       binRank[k]       = (c << 14) | atomicAdd(&hist[c], 1u);
     }
   }
   __syncthreads();
+
+  // Reserve segment room: all atomics of this thread in flight together.
+  constexpr int kPerThread = 16384 / kScatterBlock;  // bins per thread at most
   const uint32_t xcd = blockIdx.x % kXcds;
-  for (uint32_t i = threadIdx.x; i < nb; i += kScatterBlock) {
-    const uint32_t c = hist[i];
-    hist[i]          = c ? atomicAdd(&s.segCount[i * kXcds + xcd], c) : 0u;
+  uint32_t base[kPerThread];
+#pragma unroll
+  for (int j = 0; j < kPerThread; j++) {
+    const uint32_t i = threadIdx.x + j * kScatterBlock;
+    const uint32_t c = i < nb ? hist[i] : 0u;
+    base[j]          = c ? atomicAdd(&s.segCount[i * kXcds + xcd], c) : 0u;
   }
   __syncthreads();
+#pragma unroll
+  for (int j = 0; j < kPerThread; j++) {
+    const uint32_t i = threadIdx.x + j * kScatterBlock;
+    if (i < nb) hist[i] = base[j];
+  }
+  __syncthreads();
+
+  // Record stores (no atomics in this loop, so nothing waits on them);
+  // points whose segment is full are collected for the overflow list.
+  uint32_t full = 0;
 #pragma unroll
   for (int k = 0; k < kScatterPts; k++) {
     if (binRank[k] == 0xffffffffu) continue;
@@ -90,8 +113,12 @@ __global__ __launch_bounds__(kScatterBlock) void kBinScatter(GridView g, BinGeom
       float4 r = {p[k].x, p[k].y, p[k].z, __uint_as_float((uint32_t)i)};
       s.records[((uint64_t)c * kXcds + xcd) * s.capacity + pos] = r;
     } else {
-      s.overflow[atomicAdd(s.overflowCount, 1u)] = (uint32_t)i;
+      full |= 1u << k;
     }
+  }
+  if (full) {
+    for (int k = 0; k < kScatterPts; k++)
+      if (full & (1u << k)) s.overflow[atomicAdd(s.overflowCount, 1u)] = (uint32_t)(first + (uint64_t)k * kScatterBlock);
   }
 }
 
@@ -182,25 +209,70 @@ __global__ __launch_bounds__(kSampleBlock) void kBinnedSample(GridView g, AttrVi
                                 (1u << b.shift[0]) + w,
                                 (1u << b.shift[1]) + w,
                                 (1u << b.shift[2]) + w};
-  // Stage the brick row by row; voxels outside the domain are never read
-  // from LDS (their slots stay unwritten).
-  const uint32_t rows = src.ey * src.ez, perPass = kSampleBlock / src.ex, lx = threadIdx.x % src.ex;
-  if (threadIdx.x < perPass * src.ex)
-    for (uint32_t r = threadIdx.x / src.ex; r < rows; r += perPass) {
-      const uint32_t ly = r % src.ey, lz = r / src.ey;
-      const uint32_t x = (uint32_t)(src.bx + (int32_t)lx), y = (uint32_t)(src.by + (int32_t)ly),
-                     z = (uint32_t)(src.bz + (int32_t)lz);
-      if (x < src.global.nx && y < src.global.ny && z < src.global.nz)
-        brick[r * src.ex + lx] = src.global.data[src.global.xTerm(x) + src.global.yTerm(y) + src.global.zTerm(z)];
+  // Stage the brick row by row (a row is ex consecutive x voxels, one per
+  // thread), 8 rows in flight per thread; voxels outside the domain are never
+  // read from LDS, so their slots stay unwritten.
+  const uint32_t perPass = kSampleBlock / src.ex;
+  if (threadIdx.x < perPass * src.ex) {
+    const uint32_t lx = threadIdx.x % src.ex, r0 = threadIdx.x / src.ex;
+    const uint32_t x  = (uint32_t)(src.bx + (int32_t)lx);
+    const bool xIn    = x < src.global.nx;
+    const auto xOff   = src.global.xTerm(x);
+    const uint32_t dly = perPass % src.ey, dlz = perPass / src.ey;
+    uint32_t ly = r0 % src.ey, lz = r0 / src.ey;
+    while (lz < src.ez) {
+      T v[8];
+      uint32_t slot[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        // Unconditional loads (element 0 stands in for skipped voxels) so
+        // that all eight are in flight together.
+        const uint32_t y = (uint32_t)(src.by + (int32_t)ly), z = (uint32_t)(src.bz + (int32_t)lz);
+        const bool ok    = lz < src.ez && xIn && y < src.global.ny && z < src.global.nz;
+        v[u]             = src.global.data[ok ? xOff + src.global.yTerm(y) + src.global.zTerm(z) : 0];
+        slot[u]          = ok ? (lz * src.ey + ly) * src.ex + lx : 0xffffffffu;
+        ly += dly;
+        lz += dlz;
+        if (ly >= src.ey) {
+          ly -= src.ey;
+          lz++;
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < 8; u++)
+        if (slot[u] != 0xffffffffu) brick[slot[u]] = v[u];
     }
+  }
   __syncthreads();
-  for (uint32_t seg = 0; seg < kXcds; seg++) {
-    const uint32_t cnt = min(s.segCount[bin * kXcds + seg], s.capacity);
-    const float4 *rec  = s.records + ((uint64_t)bin * kXcds + seg) * s.capacity;
-    for (uint32_t k = threadIdx.x; k < cnt; k += kSampleBlock) {
-      const float4 r = rec[k];
-      binnedPoint<kOp>(g, src, F3{r.x, r.y, r.z}, noff, __float_as_uint(r.w), out, outStride);
+  // The bin's records: 8 XCD segments, read 4 per thread at a time.
+  uint32_t segEnd[kXcds];
+  uint32_t total = 0;
+#pragma unroll
+  for (int q = 0; q < kXcds; q++) {
+    total += min(s.segCount[bin * kXcds + q], s.capacity);
+    segEnd[q] = total;
+  }
+  const float4 *rec0 = s.records + (uint64_t)bin * kXcds * s.capacity;
+  for (uint32_t k0 = threadIdx.x; k0 < total; k0 += 4 * kSampleBlock) {
+    float4 r[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      // Past the end, re-read record k0 (unconditional loads stay in flight
+      // together); its result is not used.
+      const uint32_t k = k0 + u * kSampleBlock < total ? k0 + u * kSampleBlock : k0;
+      uint32_t q = 0, before = 0;
+#pragma unroll
+      for (int t = 0; t < kXcds - 1; t++)
+        if (k >= segEnd[t]) {
+          q      = t + 1;
+          before = segEnd[t];
+        }
+      r[u] = rec0[(uint64_t)q * s.capacity + (k - before)];
     }
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+      if (k0 + u * kSampleBlock < total)
+        binnedPoint<kOp>(g, src, F3{r[u].x, r[u].y, r[u].z}, noff, __float_as_uint(r[u].w), out, outStride);
   }
   // Overflow records (segments that were full): direct path.
   const uint32_t over = *s.overflowCount;

@@ -28,11 +28,7 @@ __device__ Range1f nodeRange(const GridView &g, const AttrView &a, uint32_t attr
         for (uint32_t z = rz[0]; z < rz[1]; ++z) {
           const uint32_t px = ox + x, py = oy + y, pz = oz + z;
           if (!(px < g.activeSize[0] && py < g.activeSize[1] && pz < g.activeSize[2])) continue;
-          const DenseLayout &L = g.layout;
-          const uint64_t idx   = (((uint64_t)(px >> L.sx)) << (L.sx + L.sy + L.sz)) + (px & ((1u << L.sx) - 1)) +
-                               (uint64_t)(py >> L.sy) * L.strideY + ((py & ((1u << L.sy) - 1)) << L.sx) +
-                               (uint64_t)(pz >> L.sz) * L.strideZ + ((pz & ((1u << L.sz) - 1)) << (L.sx + L.sy));
-          const float v = fetchAny(a.data, a.type, idx);
+          const float v = fetchAny(a.data, a.type, denseOffset(g.layout, px, py, pz));
           r.lower            = vmin(r.lower, v);
           r.upper            = vmax(r.upper, v);
         }

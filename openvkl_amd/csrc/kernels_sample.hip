@@ -54,9 +54,75 @@ __global__ __launch_bounds__(kBlock) void kGradient(GridView g, AttrView a, uint
   out[3 * i + 2] = r.z;
 }
 
+// Fixed-step ray marching fused with sampling (one ray per work-item):
+// out[r] = sum over s of sample(o + t_s d), t_s = t0 + (s + 0.5) dt,
+// dt = (t1 - t0) / steps, summed in step order. Adjacent rays of a
+// wavefront read adjacent voxels, so the gathers hit L1 / L2.
+template <typename Src, int kFilter>
+__global__ __launch_bounds__(kBlock) void kRayMarch(GridView g, AttrView a, uint32_t attr, uint32_t numRays,
+                                                     const float *__restrict__ org, const float *__restrict__ dir,
+                                                     const float *__restrict__ tRange, uint32_t steps, float noff,
+                                                     float *__restrict__ out)
+{
+  const uint32_t r = blockIdx.x * kBlock + threadIdx.x;
+  if (r >= numRays) return;
+  const F3 o = {org[3 * r], org[3 * r + 1], org[3 * r + 2]};
+  const F3 d = {dir[3 * r], dir[3 * r + 1], dir[3 * r + 2]};
+  const float t0 = tRange[2 * r], dt = (tRange[2 * r + 1] - t0) / (float)steps;
+  const Src src(g, a, attr);
+  float acc = 0.f;
+  for (uint32_t k = 0; k < steps; k++) {
+    const float t = t0 + ((float)k + 0.5f) * dt;
+    const F3 p    = xfmPoint(g.o2i, F3{o.x + t * d.x, o.y + t * d.y, o.z + t * d.z});
+    float v;
+    if (kFilter == kFilterNearest)
+      v = sampleNearest(src, p, noff);
+    else if (kFilter == kFilterCubic)
+      v = sampleCubic(src, p);
+    else
+      v = sampleLinear(src, p);
+    acc += v;
+  }
+  out[r] = acc;
+}
+
 namespace {
 
 inline unsigned blocksFor(uint64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+struct MarchArgs
+{
+  uint32_t numRays, steps;
+  const float *org, *dir, *tRange;
+  float *out;
+};
+
+template <typename Src>
+hipError_t marchFilter(const SamplerView &s, const AttrView &a, uint32_t attr, const MarchArgs &m, hipStream_t st)
+{
+  const GridView &g = s.grid;
+  const float noff  = g.constantCellData ? 0.f : 0.5f;
+  const dim3 grid(blocksFor(m.numRays)), block(kBlock);
+  if (s.filter == kFilterNearest)
+    hipLaunchKernelGGL((kRayMarch<Src, kFilterNearest>), grid, block, 0, st, g, a, attr, m.numRays, m.org, m.dir,
+                       m.tRange, m.steps, noff, m.out);
+  else if (s.filter == kFilterCubic)
+    hipLaunchKernelGGL((kRayMarch<Src, kFilterCubic>), grid, block, 0, st, g, a, attr, m.numRays, m.org, m.dir,
+                       m.tRange, m.steps, noff, m.out);
+  else
+    hipLaunchKernelGGL((kRayMarch<Src, kFilterLinear>), grid, block, 0, st, g, a, attr, m.numRays, m.org, m.dir,
+                       m.tRange, m.steps, noff, m.out);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t marchDense(const SamplerView &s, const AttrView &a, uint32_t attr, const MarchArgs &m, hipStream_t st)
+{
+  const GridView &g = s.grid;
+  const uint64_t nbz = ((uint64_t)g.activeSize[2] + (1u << g.layout.sz) - 1) >> g.layout.sz;
+  if (nbz * g.layout.strideZ >= (1ull << 32)) return marchFilter<DenseSource<T, true>>(s, a, attr, m, st);
+  return marchFilter<DenseSource<T, false>>(s, a, attr, m, st);
+}
 
 template <typename Src>
 hipError_t dispatchFilter(const SamplerView &s, const AttrView &a, uint32_t attr, uint64_t n, const float *xyz,
@@ -133,6 +199,27 @@ hipError_t launchGradient(const SamplerView &s, const AttrView &a, uint32_t attr
   return dispatch(s, attr, a, n, xyz, out, 1, stream, true);
 }
 
+hipError_t launchRayMarch(const SamplerView &s, const AttrView &a, uint32_t attr, uint32_t numRays, const float *org,
+                          const float *dir, const float *tRange, uint32_t steps, float *out, hipStream_t stream)
+{
+  if (numRays == 0) return hipSuccess;
+  const MarchArgs m{numRays, steps, org, dir, tRange, out};
+  if (s.grid.dense) {
+    switch (a.type) {
+    case kTypeUchar: return marchDense<uint8_t>(s, a, attr, m, stream);
+    case kTypeShort: return marchDense<int16_t>(s, a, attr, m, stream);
+    case kTypeUshort: return marchDense<uint16_t>(s, a, attr, m, stream);
+    case kTypeHalf: return marchDense<__half>(s, a, attr, m, stream);
+    case kTypeFloat: return marchDense<float>(s, a, attr, m, stream);
+    case kTypeDouble: return marchDense<double>(s, a, attr, m, stream);
+    default: return hipErrorInvalidValue;
+    }
+  }
+  if (a.type == kTypeHalf) return marchFilter<SparseSource<__half>>(s, a, attr, m, stream);
+  if (a.type == kTypeFloat) return marchFilter<SparseSource<float>>(s, a, attr, m, stream);
+  return hipErrorInvalidValue;
+}
+
 hipError_t launchSampleStrided(const SamplerView &s, const AttrView &a, uint32_t attr, uint64_t n, const float *xyz,
                                float *out, uint32_t outStride, hipStream_t stream)
 {
@@ -159,6 +246,54 @@ __global__ __launch_bounds__(kBlock) void kSwizzle(const E *__restrict__ src, E 
   E v{};
   if (x < nx && y < ny && z < nz) v = src[(z * ny + y) * (uint64_t)nx + x];
   dst[d] = v;
+}
+
+// Corner-packed layout: cell (x, y, z) <- the 8 voxels v(x + dx, y + dy,
+// z + dz), record order 4 dx + 2 dy + dz, coordinates past the last voxel
+// clamped to it (those corners are never read through the record: the
+// stencil is then out of the domain). One work-item per cell.
+template <typename E>
+__global__ __launch_bounds__(kBlock) void kPackCorners(const E *__restrict__ src, E *__restrict__ dst, uint32_t nx,
+                                                        uint32_t ny, uint32_t nz)
+{
+  const uint64_t c = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (c >= (uint64_t)nx * ny * nz) return;
+  const uint32_t x = (uint32_t)(c % nx), y = (uint32_t)((c / nx) % ny), z = (uint32_t)(c / ((uint64_t)nx * ny));
+  const uint32_t x1 = x + 1 < nx ? x + 1 : x, y1 = y + 1 < ny ? y + 1 : y, z1 = z + 1 < nz ? z + 1 : z;
+  CellRecord<E> r;
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    const uint32_t xx = (k & 4) ? x1 : x, yy = (k & 2) ? y1 : y, zz = (k & 1) ? z1 : z;
+    r.v[k] = src[((uint64_t)zz * ny + yy) * nx + xx];
+  }
+  reinterpret_cast<CellRecord<E> *>(dst)[c] = r;
+}
+
+hipError_t launchPackCorners(const void *src, void *dst, uint32_t elemSize, const uint32_t dims[3],
+                             hipStream_t stream)
+{
+  const uint64_t cells = (uint64_t)dims[0] * dims[1] * dims[2];
+  const dim3 grid(blocksFor(cells)), block(kBlock);
+  switch (elemSize) {
+  case 1:
+    hipLaunchKernelGGL(kPackCorners<uint8_t>, grid, block, 0, stream, (const uint8_t *)src, (uint8_t *)dst, dims[0],
+                       dims[1], dims[2]);
+    break;
+  case 2:
+    hipLaunchKernelGGL(kPackCorners<uint16_t>, grid, block, 0, stream, (const uint16_t *)src, (uint16_t *)dst,
+                       dims[0], dims[1], dims[2]);
+    break;
+  case 4:
+    hipLaunchKernelGGL(kPackCorners<uint32_t>, grid, block, 0, stream, (const uint32_t *)src, (uint32_t *)dst,
+                       dims[0], dims[1], dims[2]);
+    break;
+  case 8:
+    hipLaunchKernelGGL(kPackCorners<uint64_t>, grid, block, 0, stream, (const uint64_t *)src, (uint64_t *)dst,
+                       dims[0], dims[1], dims[2]);
+    break;
+  default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
 }
 
 hipError_t launchSwizzle(const void *src, void *dst, uint32_t elemSize, const uint32_t dims[3],

@@ -108,6 +108,22 @@ VKL_DEV uint32_t offsetToLinear(uint32_t l, uint32_t x, uint32_t y, uint32_t z)
 // background. Addressing per DenseLayout (x-fastest or brick-swizzled): each
 // axis contributes an independent term, so the 8 trilinear corners cost 6
 // axis terms and 8 adds.
+// Element offset of root-relative voxel (x, y, z) under layout L.
+VKL_DEV uint64_t denseOffset(const DenseLayout &L, uint32_t x, uint32_t y, uint32_t z)
+{
+  const uint32_t S = L.sx + L.sy + L.sz;
+  return (((((uint64_t)(x >> L.sx)) << S) + (x & ((1u << L.sx) - 1))) << (L.cornerPacked ? 3 : 0)) +
+         (uint64_t)(y >> L.sy) * L.strideY + ((y & ((1u << L.sy) - 1)) << L.sx) +
+         (uint64_t)(z >> L.sz) * L.strideZ + ((z & ((1u << L.sz) - 1)) << (L.sx + L.sy));
+}
+
+// The 8 elements of a corner-packed cell record, one vector load.
+template <typename T>
+struct alignas(8 * sizeof(T)) CellRecord
+{
+  T v[8];
+};
+
 template <typename T, bool kIdx64>
 struct DenseSource
 {
@@ -115,7 +131,7 @@ struct DenseSource
   const T *data;
   uint32_t nx, ny, nz;
   int32_t rx, ry, rz;
-  uint32_t sx, sy, sz, S;
+  uint32_t sx, sy, sz, S, pk;
   Off strideY, strideZ;
   float bg;
 
@@ -131,13 +147,14 @@ struct DenseSource
         sy(g.layout.sy),
         sz(g.layout.sz),
         S(g.layout.sx + g.layout.sy + g.layout.sz),
+        pk(g.layout.cornerPacked ? 3u : 0u),
         strideY((Off)g.layout.strideY),
         strideZ((Off)g.layout.strideZ),
         bg(a.background)
   {
   }
 
-  VKL_DEV Off xTerm(uint32_t x) const { return ((Off)(x >> sx) << S) | (Off)(x & ((1u << sx) - 1)); }
+  VKL_DEV Off xTerm(uint32_t x) const { return (((Off)(x >> sx) << S) | (Off)(x & ((1u << sx) - 1))) << pk; }
   VKL_DEV Off yTerm(uint32_t y) const
   {
     return (Off)(y >> sy) * strideY + (Off)((y & ((1u << sy) - 1)) << sx);
@@ -160,6 +177,12 @@ struct DenseSource
   {
     const uint32_t ox = (uint32_t)(ic.x - rx), oy = (uint32_t)(ic.y - ry), oz = (uint32_t)(ic.z - rz);
     if (ox + 1 < nx && oy + 1 < ny && oz + 1 < nz && ox < nx && oy < ny && oz < nz) {
+      if (pk) {  // corner-packed: the cell record holds s[0..7]
+        const CellRecord<T> c = *reinterpret_cast<const CellRecord<T> *>(data + xTerm(ox) + yTerm(oy) + zTerm(oz));
+#pragma unroll
+        for (int i = 0; i < 8; i++) s[i] = toFloat(c.v[i]);
+        return;
+      }
       const Off x0 = xTerm(ox), x1 = xTerm(ox + 1);
       const Off y0 = yTerm(oy), y1 = yTerm(oy + 1);
       const Off z0 = zTerm(oz), z1 = zTerm(oz + 1);
@@ -366,6 +389,32 @@ VKL_DEV void crDerivWeights(float t, float *w)
 template <typename Src>
 VKL_DEV void cubicTaps(const Src &src, I3 ic, float *S)
 {
+#pragma unroll
+  for (int i = 0; i < 64; i++) S[i] = src.at(ic.x + (i >> 4) - 1, ic.y + ((i >> 2) & 3) - 1, ic.z + (i & 3) - 1);
+}
+
+// Corner-packed dense volumes: the 4x4x4 stencil is 8 cell records (cells at
+// ic - 1 + 2 (a, b, c)) when it lies inside the domain; same values as 64
+// single reads.
+template <typename T, bool kIdx64>
+VKL_DEV void cubicTaps(const DenseSource<T, kIdx64> &src, I3 ic, float *S)
+{
+  const uint32_t ox = (uint32_t)(ic.x - src.rx), oy = (uint32_t)(ic.y - src.ry), oz = (uint32_t)(ic.z - src.rz);
+  if (src.pk && ox - 1 < src.nx && ox + 2 < src.nx && oy - 1 < src.ny && oy + 2 < src.ny && oz - 1 < src.nz &&
+      oz + 2 < src.nz) {
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+      const uint32_t a = (c >> 2) & 1, b = (c >> 1) & 1, d = c & 1;
+      const CellRecord<T> r = *reinterpret_cast<const CellRecord<T> *>(
+          src.data + src.xTerm(ox - 1 + 2 * a) + src.yTerm(oy - 1 + 2 * b) + src.zTerm(oz - 1 + 2 * d));
+#pragma unroll
+      for (int k = 0; k < 8; k++) {
+        const uint32_t i = 2 * a + ((k >> 2) & 1), j = 2 * b + ((k >> 1) & 1), l = 2 * d + (k & 1);
+        S[16 * i + 4 * j + l] = toFloat(r.v[k]);
+      }
+    }
+    return;
+  }
 #pragma unroll
   for (int i = 0; i < 64; i++) S[i] = src.at(ic.x + (i >> 4) - 1, ic.y + ((i >> 2) & 3) - 1, ic.z + (i & 3) - 1);
 }

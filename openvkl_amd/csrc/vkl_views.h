@@ -56,9 +56,17 @@ struct LevelView
 // for bricks of 2^sx x 2^sy x 2^sz voxels (S = sx + sy + sz, bricks x-fastest).
 // sx = sy = sz = 0 is the reference's x-fastest array (strideY = nx,
 // strideZ = nx * ny).
+//
+// Corner-packed layout (cornerPacked = 1, with sx = sy = sz = 0): cell
+// (x, y, z) holds its 8 trilinear corners v(x + dx, y + dy, z + dz) as one
+// record of 8 elements in the order 4 dx + 2 dy + dz (corners past the last
+// voxel repeat it); X(x) = 8 x, strideY = 8 nx, strideZ = 8 nx ny. Voxel
+// (x, y, z) is element 0 of its cell, so every voxel-addressed read works
+// unchanged, and a trilinear stencil is one aligned 8-element load.
 struct DenseLayout
 {
   uint32_t sx, sy, sz;
+  uint32_t cornerPacked;
   uint64_t strideY, strideZ;
 };
 

@@ -359,16 +359,22 @@ void DenseVolume::commit()
   getVec3i("brickShift", shift);
   for (int i = 0; i < 3; i++)
     if (shift[i] < 0 || shift[i] > 4) throw VklError("brickShift components must be in [0, 4]");
+  // Or "cornerPacked" (bool): each cell stores its 8 trilinear corners
+  // (8x the memory; one aligned load per trilinear stencil).
+  const bool packed = getBool("cornerPacked", false);
+  if (packed && (shift[0] | shift[1] | shift[2]))
+    throw VklError("cornerPacked and brickShift cannot be combined", VKL_INVALID_ARGUMENT);
   DenseLayout &L = grid.layout;
-  L.sx = (uint32_t)shift[0];
-  L.sy = (uint32_t)shift[1];
-  L.sz = (uint32_t)shift[2];
+  L.sx           = (uint32_t)shift[0];
+  L.sy           = (uint32_t)shift[1];
+  L.sz           = (uint32_t)shift[2];
+  L.cornerPacked = packed ? 1u : 0u;
   const uint64_t nbx = ((uint64_t)dims[0] + (1u << L.sx) - 1) >> L.sx;
   const uint64_t nby = ((uint64_t)dims[1] + (1u << L.sy) - 1) >> L.sy;
   const uint64_t nbz = ((uint64_t)dims[2] + (1u << L.sz) - 1) >> L.sz;
   const uint32_t S   = L.sx + L.sy + L.sz;
-  L.strideY          = nbx << S;
-  L.strideZ          = (nbx * nby) << S;
+  L.strideY          = (nbx << S) << (packed ? 3 : 0);
+  L.strideZ          = ((nbx * nby) << S) << (packed ? 3 : 0);
   const bool bricked = S != 0;
   const uint32_t udims[3] = {(uint32_t)dims[0], (uint32_t)dims[1], (uint32_t)dims[2]};
 
@@ -376,7 +382,22 @@ void DenseVolume::commit()
   attrViews.clear();
   for (size_t a = 0; a < attrs.size(); a++) {
     AttrView av;
-    if (bricked) {
+    if (packed) {
+      const Data &d = *attrs[a];
+      std::unique_ptr<DeviceBuffer> tmp;
+      const void *src = d.base;
+      if (!(d.onDevice && d.compact())) {
+        std::vector<uint8_t> h = d.hostCopy();
+        tmp.reset(new DeviceBuffer(h.size()));
+        tmp->upload(h.data(), h.size(), st);
+        src = tmp->ptr;
+      }
+      const size_t es   = sizeOfType(d.dataType);
+      DeviceBuffer *dst = newBuffer(nvox * 8 * es);
+      checkHip(launchPackCorners(src, dst->ptr, (uint32_t)es, udims, st), "corner packing");
+      checkHip(hipStreamSynchronize(st), "corner packing");
+      av.data = dst->ptr;
+    } else if (bricked) {
       // Swizzle from the caller's device buffer, or from a temporary upload.
       const Data &d = *attrs[a];
       std::unique_ptr<DeviceBuffer> tmp;

@@ -185,6 +185,44 @@ def test_brick_layouts_match_reference_order(device, shift):
         np.testing.assert_array_equal(r.view(np.uint32), orng.view(np.uint32))
 
 
+@pytest.mark.parametrize("dtype", [np.float32, np.float16, np.uint8, np.float64])
+def test_corner_packed_layout_matches_reference(device, dtype):
+    """Corner-packed cell records (8 corners per cell) change addressing only:
+    samples, gradients and majorants equal the oracle's exactly, including
+    stencils on the domain's last planes and outside it."""
+    rng = np.random.default_rng(57)
+    data = rng.standard_normal((33, 26, 41)) * 30
+    data = data.astype(dtype) if np.dtype(dtype).kind == "f" else np.abs(data).astype(dtype)
+    vol = device.structured_regular(data, corner_packed=True)
+    grid = O.Grid.dense(data)
+    pts = V.uniform_points(200000, (-1.5, -1.5, -1.5), (41.5, 26.5, 33.5))
+    for f in (vkl.VKL_FILTER_NEAREST, vkl.VKL_FILTER_LINEAR):
+        grid.set_sampler(f)
+        s = vol.sampler(filter=f)
+        assert_same(s.sample(pts), grid.sample(pts))
+        assert_same(s.gradient(pts), grid.gradient(pts))
+    grid.set_sampler(O.CUBIC)
+    check_cubic(grid, vol.sampler(filter=vkl.VKL_FILTER_CUBIC).sample(pts[:50000]), pts[:50000])
+    for l in range(3):
+        _, _, r = vol.level(l)
+        _, _, orng = grid.level(l)
+        np.testing.assert_array_equal(r.view(np.uint32), orng.view(np.uint32))
+
+
+def test_corner_packed_rejects_brick_shift(device):
+    v = device.new_volume("structuredRegular")
+    v.set_vec3i("dimensions", 4, 4, 4)
+    d = device.data(np.zeros(64, np.float32))
+    v.set_data("data", d)
+    d.release()
+    v.set_vec3i("brickShift", 1, 1, 1)
+    v.set_bool("cornerPacked", True)
+    vkl.lib().vklCommit(v.h)
+    code, msg = device.last_error()
+    device.errors.clear()
+    assert code == vkl.VKL_INVALID_ARGUMENT and "cornerPacked" in msg
+
+
 def test_node_table_bitexact(device):
     """Node table (voxel codes + value ranges) equals the oracle's bit for bit."""
     rng = np.random.default_rng(37)
