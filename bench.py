@@ -1,10 +1,15 @@
 """Benchmark: exact FP32 trilinear sampling of a 512^3 structuredRegular
 volume at 2^26 uniform random points (vklComputeSampleN, device-resident
-inputs), on one MI355X.
+inputs), on one MI355X. The volume uses the device's corner-packed layout
+(volume parameter "cornerPacked"; --layout linear selects the caller's
+x-fastest array).
 
 Prints one JSON line: throughput in Msamples/s, the roofline of the sampling
-kernel (algorithmic bytes / kernel time vs 8 TB/s HBM), and the CPU oracle
-timed on this host's cores as a reported baseline.
+kernel (algorithmic bytes / kernel time vs 8 TB/s HBM, plus the measured HBM
+traffic from the committed rocprofv3 counters), the CPU oracle timed on this
+host's cores as a reported baseline, and the other workloads (other layouts,
+ray marching, batched multi-volume sampling, the neural-field sampler) as
+"variants".
 """
 import argparse
 import json
@@ -382,6 +387,13 @@ def main():
     log(f"U = {U} distinct voxels ({U / n ** 3:.3f} of V); {bytes_per_launch / npts:.2f} B/sample; "
         f"kernel avg {ms_event:.3f} ms; {achieved:.0f} GB/s")
 
+    # HBM traffic of the headline kernel from the committed rocprofv3 FETCH_SIZE /
+    # WRITE_SIZE runs (copy-calibrated), for the configuration they measured.
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", "kSample_dense512_f32_linear_packed.traffic.txt")
+    if packed and n == 512 and npts == 1 << 26 and os.path.exists(tfile):
+        kv = dict(line.split()[:2] for line in open(tfile) if line.strip() and not line.startswith("#"))
+        traffic = int(kv["fetch_bytes_per_launch"]) + int(kv["write_bytes_per_launch"])
     result = {
         "metric": METRIC,
         "value": round(value, 1),
@@ -402,7 +414,7 @@ def main():
                    "x-fastest" if brick == (0, 0, 0) else "bricks %dx%dx%d" % tuple(1 << b for b in brick),
                    "sampling_path": "binned (bin scatter + LDS-brick sample)" if binned else "direct gather"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes_per_launch": bytes_per_launch, "kernel_ms": round(ms_event, 4)},
     }
     variants = [v for v in args.variants.split(",") if v]
