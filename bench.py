@@ -89,7 +89,38 @@ def variant_layouts(torch, vkl, dev, vol_t, pts, out, steps, warmup, alg_bytes):
     return res
 
 
-def variant_batched(torch, vkl, dev, steps, warmup):
+def variant_config2(torch, vkl, dev, vol_t, pts, steps, warmup, U):
+    """The rest of the 512^3 / 2^26-point matrix on corner-packed volumes:
+    half and uint8 (raw codes; the scale is applied by the caller) linear,
+    float32 cubic, float32 linear gradient."""
+    npts = pts.shape[0]
+    out = torch.empty(npts, device="cuda")
+    gout = torch.empty((npts, 3), device="cuda")
+    res = {}
+    cases = (("half_linear", vol_t.half(), vkl.VKL_FILTER_LINEAR, False, 2),
+             ("uint8_linear", (vol_t * 100.0 + 100.0).clamp(0, 255).to(torch.uint8), vkl.VKL_FILTER_LINEAR, False, 1),
+             ("float32_cubic", vol_t, vkl.VKL_FILTER_CUBIC, False, 4),
+             ("float32_linear_gradient", vol_t, vkl.VKL_FILTER_LINEAR, True, 4))
+    for name, data, filt, grad, es in cases:
+        v = dev.structured_regular(data.contiguous(), corner_packed=True)
+        sm = v.sampler(filter=filt)
+        if grad:
+            ms = timed(torch, lambda: sm.gradient(pts, out=gout), steps, warmup)
+            alg = npts * (12 + 12) + es * U
+        else:
+            ms = timed(torch, lambda: sm.sample(pts, out=out), steps, warmup)
+            alg = npts * (12 + 4) + es * (U if filt != vkl.VKL_FILTER_CUBIC else data.numel())
+        gbs = alg / (ms * 1e-3) / 1e9
+        res[name] = {"msamples_s": round(npts / ms / 1e3, 1), "ms": round(ms, 4), "algorithmic_gbs": round(gbs, 1),
+                     "frac": round(gbs / HBM_PEAK_GBS, 4)}
+        log(f"  {name}: {res[name]}")
+        sm.release()
+        v.release()
+        del data
+    return res
+
+
+def variant_batched(torch, vkl, dev, steps, warmup, packed=False):
     """64 structuredRegular volumes, cube sides seeded in [32, 384], float32 and
     half alternating, point counts log-uniform in [1e4, 1e7]: one persistent,
     graph-captured launch with an autotuned shape (vklNewSampleBatchPlan)."""
@@ -101,7 +132,7 @@ def variant_batched(torch, vkl, dev, steps, warmup):
         side = int(rng.integers(32, 385))
         dt = torch.float32 if i % 2 == 0 else torch.float16
         data = torch.rand((side, side, side), device="cuda", generator=g).to(dt)
-        v = dev.structured_regular(data)
+        v = dev.structured_regular(data, corner_packed=packed or None)
         s = v.sampler(filter=vkl.VKL_FILTER_LINEAR)
         count = int(np.exp(rng.uniform(np.log(1e4), np.log(1e7))))
         p = torch.rand((count, 3), device="cuda", generator=g) * float(side - 1)
@@ -118,7 +149,8 @@ def variant_batched(torch, vkl, dev, steps, warmup):
     blocks, threads, tile = plan.launch_shape()
     plan.release()
     gbs = alg / (ms * 1e-3) / 1e9
-    return {"msamples_s": round(total / ms / 1e3, 1), "ms": round(ms, 4), "points": total, "volumes": 64,
+    return {"layout": "corner-packed" if packed else "x-fastest",
+            "msamples_s": round(total / ms / 1e3, 1), "ms": round(ms, 4), "points": total, "volumes": 64,
             "achieved_gbs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
             "launch": {"blocks": blocks, "threads": threads, "tile": tile}}
 
@@ -190,6 +222,15 @@ def variant_neural(torch, vkl, dev, res, steps, warmup, train_log2=28, infer_log
     approx = torch.empty_like(exact)
     ws = torch.empty(pts.shape[0] + 1, device="cuda", dtype=torch.int32)
     ms_exact = timed(torch, lambda: s.sample(pts, out=exact), steps, warmup)
+    # the exact sampler on the corner-packed form of the same volume
+    vol_p = dev.vdb(np.full(n, 3, np.uint32), origins, np.full(n, vkl.VKL_FORMAT_DENSE_ZYX, np.uint32),
+                    packed_dense=[leaves], corner_packed=True)
+    s_p = vol_p.sampler(filter=vkl.VKL_FILTER_LINEAR)
+    exact_p = torch.empty_like(exact)
+    ms_exact_p = timed(torch, lambda: s_p.sample(pts, out=exact_p), steps, warmup)
+    same = bool(((exact_p == exact) | (torch.isnan(exact_p) & torch.isnan(exact))).all())
+    s_p.release()
+    vol_p.release()
     fb = [0]
 
     def run_nf():
@@ -207,6 +248,8 @@ def variant_neural(torch, vkl, dev, res, steps, warmup, train_log2=28, infer_log
     nf.release()
     return {"volume": f"vdb shell+fBm, {res}^3 index domain, {n} float32 8^3 leaves",
             "exact_msamples_s": round(pts.shape[0] / ms_exact / 1e3, 1),
+            "exact_corner_packed_msamples_s": round(pts.shape[0] / ms_exact_p / 1e3, 1),
+            "exact_corner_packed_bitexact": same,
             "neural_msamples_s": round(pts.shape[0] / ms_nf / 1e3, 1),
             "speedup_vs_exact": round(ms_exact / ms_nf, 3), "nrmse": nrmse, "leaf_tolerance": tol,
             "network_only_msamples_s": round(pts.shape[0] / ms_net / 1e3, 1), "network_only_nrmse": nrmse_net,
@@ -323,8 +366,8 @@ def main():
                     help="device voxel layout: the caller's x-fastest array (with --brick) or corner-packed cells")
     ap.add_argument("--binned", choices=("auto", "on", "off"), default="auto",
                     help="spatially binned sampling (auto: on for large volumes and batches)")
-    ap.add_argument("--variants", default="layouts,raymarch,batched,neural",
-                    help="comma list of extra measurements: layouts, raymarch, batched, neural, iterators ('' = "
+    ap.add_argument("--variants", default="layouts,config2,raymarch,batched,neural",
+                    help="comma list of extra measurements: layouts, config2, raymarch, batched, neural, iterators ('' = "
                          "none); iterators (4096^3 vdb, 2^21 rays) is held out of the default run: its last GPU "
                          "run went silent for 180 s without a cause found yet")
     ap.add_argument("--nf-res", type=int, default=1024, help="index-domain edge of the neural-field vdb")
@@ -428,8 +471,11 @@ def main():
             rv = dev.structured_regular(vol_t)
             extra["raymarch"] = variant_raymarch(torch, vkl, rv, n, args.steps, args.warmup)
             rv.release()
+        elif v == "config2":
+            extra["config2"] = variant_config2(torch, vkl, dev, vol_t, pts, args.steps, args.warmup, U)
         elif v == "batched":
-            extra["batched"] = variant_batched(torch, vkl, dev, args.steps, args.warmup)
+            extra["batched"] = variant_batched(torch, vkl, dev, args.steps, args.warmup, packed=True)
+            extra["batched_x_fastest"] = variant_batched(torch, vkl, dev, args.steps, args.warmup)
         elif v == "neural":
             extra["neural_field"] = variant_neural(torch, vkl, dev, args.nf_res, args.steps, args.warmup)
         elif v == "iterators":

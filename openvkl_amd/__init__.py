@@ -304,7 +304,7 @@ class Device:
         return v
 
     def vdb(self, level, origin, fmt, packed_dense=None, packed_tile=None, leaf_data=None, background=None,
-            index_to_object=None, filter=None):
+            index_to_object=None, filter=None, corner_packed=None, max_sampling_depth=None):
         """Sparse vdb volume from node arrays (packed or per-leaf data)."""
         v = self.new_volume("vdb")
         tmp = []
@@ -343,6 +343,10 @@ class Device:
             v.set_int("filter", filter)
         if index_to_object is not None:
             v.set_data("indexToObject", d(np.asarray(index_to_object, np.float32)))
+        if corner_packed is not None:
+            v.set_bool("cornerPacked", corner_packed)
+        if max_sampling_depth is not None:
+            v.set_int("maxSamplingDepth", max_sampling_depth)
         v.commit()
         for h in tmp:
             h.release()

@@ -542,7 +542,8 @@ int iterateHit(HitBuffer *buf, VKLHit *out)
   d->makeCurrent();
   uint8_t *s = stepScratch(d);
   checkHip(hipMemcpyAsync(s + 64, &buf->state, sizeof(HitState), hipMemcpyHostToDevice, d->stream), "copy");
-  checkHip(launchHitNext(ctx->deviceView.as<IteratorContextView>(), reinterpret_cast<HitState *>(s + 64), reinterpret_cast<HitOut *>(s + 1024),
+  checkHip(launchHitNext(ctx->view, ctx->deviceView.as<IteratorContextView>(),
+                         ctx->sampler->attr(ctx->view.attributeIndex), reinterpret_cast<HitState *>(s + 64), reinterpret_cast<HitOut *>(s + 1024),
                          reinterpret_cast<int *>(s), d->stream),
            "hit step");
   int result = 0;
@@ -1218,7 +1219,8 @@ OPENVKL_INTERFACE void vklIterateHitsBatch(const VKLHitIteratorContext *c, unsig
   Staged tm(times, times ? numRays * 4ull : 0, st, true);
   Staged out(hits, (uint64_t)numRays * maxHitsPerRay * sizeof(VKLHit), st, false);
   Staged cnt(numHits, numRays * 4ull, st, false);
-  checkHip(launchHitBatch(ctx->deviceView.as<IteratorContextView>(), numRays, o.as<float>(), d.as<float>(), t.as<float>(), tm.as<float>(),
+  checkHip(launchHitBatch(ctx->view, ctx->deviceView.as<IteratorContextView>(),
+                          ctx->sampler->attr(ctx->view.attributeIndex), numRays, o.as<float>(), d.as<float>(), t.as<float>(), tm.as<float>(),
                           out.as<HitOut>(), maxHitsPerRay, cnt.as<uint32_t>(), st),
            "hit batch");
   out.copyOut();

@@ -1,13 +1,15 @@
 // Batched sampling plans: a device descriptor table for many (sampler,
-// ragged point batch) pairs, one persistent kernel launch captured in a
-// hipGraph, and a small autotuner for the launch shape.
+// ragged point batch) pairs, grouped by kernel specialisation (voxel source
+// type x filter); one persistent kernel launch per group, all captured in one
+// hipGraph; a small autotuner for the launch shape.
 //
 // Autotuning: on plan creation the candidate shapes (threads per workgroup x
 // points per tile x workgroups per CU) are timed with HIP events over a few
-// launches and the fastest is kept. Results are cached in memory per workload
-// signature (batch count, total points rounded to a power of two, fast-path
-// mix) and, if OPENVKL_HIP_TUNE_CACHE names a file, persisted there as text
-// lines "signature blocks threads tile".
+// executions of all groups and the fastest is kept. Results are cached in
+// memory per workload signature (batch count and, per group, its
+// specialisation key, batch count and total points rounded to a power of two)
+// and, if OPENVKL_HIP_TUNE_CACHE names a file, persisted there as text lines
+// "signature blocks threads tile".
 
 #include <hip/hip_runtime.h>
 
@@ -29,7 +31,8 @@ struct SampleBatchPlan
 {
   Device *device = nullptr;
   std::vector<Sampler *> samplers;  // retained
-  DeviceBuffer items, begins;
+  DeviceBuffer items, begins;       // all groups' items / offsets, group after group
+  std::vector<BatchGroup> groups;
   uint32_t numItems = 0;
   uint64_t total    = 0;
   BatchLaunch launch{};
@@ -96,12 +99,10 @@ BatchLaunch tune(SampleBatchPlan &p, hipStream_t st)
   BatchLaunch best{};
   float bestMs = 1e30f;
   for (const BatchLaunch &c : candidates(cus)) {
-    checkHip(launchSampleBatch(p.items.as<BatchItem>(), p.begins.as<uint64_t>(), p.numItems, p.total, c, st),
-             "batch tune");
+    checkHip(launchSampleBatch(p.groups.data(), (uint32_t)p.groups.size(), c, st), "batch tune");
     checkHip(hipEventRecord(a, st), "hipEventRecord");
     for (int r = 0; r < 3; r++)
-      checkHip(launchSampleBatch(p.items.as<BatchItem>(), p.begins.as<uint64_t>(), p.numItems, p.total, c, st),
-               "batch tune");
+      checkHip(launchSampleBatch(p.groups.data(), (uint32_t)p.groups.size(), c, st), "batch tune");
     checkHip(hipEventRecord(b, st), "hipEventRecord");
     checkHip(hipEventSynchronize(b), "hipEventSynchronize");
     float ms = 0.f;
@@ -115,16 +116,6 @@ BatchLaunch tune(SampleBatchPlan &p, hipStream_t st)
   (void)hipEventDestroy(b);
   storeTune(p.signature, best);
   return best;
-}
-
-int32_t fastPathOf(const Sampler &s, const AttrView &a)
-{
-  const GridView &g = s.view.grid;
-  const uint64_t nbz = ((uint64_t)g.activeSize[2] + (1u << g.layout.sz) - 1) >> g.layout.sz;
-  if (!g.dense || s.view.filter != kFilterLinear || nbz * g.layout.strideZ >= (1ull << 32)) return kBatchGeneric;
-  if (a.type == kTypeFloat) return kBatchDenseLinearF32;
-  if (a.type == kTypeHalf) return kBatchDenseLinearF16;
-  return kBatchGeneric;
 }
 
 }  // namespace
@@ -150,10 +141,9 @@ OPENVKL_INTERFACE VKLSampleBatchPlan vklNewSampleBatchPlan(unsigned int numBatch
     dev->makeCurrent();
     auto *p   = new SampleBatchPlan();
     p->device = dev;
-    std::vector<BatchItem> items(numBatches);
-    std::vector<uint64_t> begins(numBatches);
-    uint64_t total = 0;
-    int fast[3]    = {0, 0, 0};
+    // Group the batches by kernel specialisation (order kept within a group).
+    std::map<uint32_t, std::vector<unsigned>> byKey;
+    std::vector<BatchItem> all(numBatches);
     for (unsigned i = 0; i < numBatches; i++) {
       Sampler *s = static_cast<Sampler *>(samplers[i].host);
       if (!s || !s->committed) {
@@ -163,22 +153,47 @@ OPENVKL_INTERFACE VKLSampleBatchPlan vklNewSampleBatchPlan(unsigned int numBatch
       s->retain();
       p->samplers.push_back(s);
       const uint32_t a = attributeIndices ? attributeIndices[i] : 0;
-      BatchItem &it    = items[i];
+      BatchItem &it    = all[i];
       it.sampler       = s->view;
       it.attr          = s->attr(a);
       it.attrIndex     = a;
       it.xyz           = reinterpret_cast<const float *>(objectCoordinates[i]);
       it.out           = samples[i];
+      it.count         = counts[i];
       if (counts[i] && (!isDevicePointer(it.xyz) || !isDevicePointer(it.out))) {
         delete p;
         throw VklError("sample batch: point and sample arrays must be device memory");
       }
-      it.begin    = total;
-      it.count    = counts[i];
-      it.fastPath = fastPathOf(*s, it.attr);
-      fast[it.fastPath]++;
-      begins[i] = total;
-      total += counts[i];
+      byKey[sourceKey(s->view.grid, it.attr, s->view.filter)].push_back(i);
+    }
+    std::vector<BatchItem> items;
+    std::vector<uint64_t> begins;
+    std::vector<size_t> firstOf;
+    std::ostringstream sig;
+    sig << "b" << numBatches;
+    uint64_t total = 0;
+    for (const auto &kv : byKey) {
+      BatchGroup g{};
+      const BatchItem &first = all[kv.second.front()];
+      g.grid                 = first.sampler.grid;
+      g.attr                 = first.attr;
+      g.filter               = first.sampler.filter;
+      g.numItems             = (uint32_t)kv.second.size();
+      firstOf.push_back(items.size());
+      uint64_t gt = 0;
+      for (unsigned i : kv.second) {
+        BatchItem it = all[i];
+        it.begin     = gt;
+        begins.push_back(gt);
+        items.push_back(it);
+        gt += it.count;
+      }
+      g.total = gt;
+      total += gt;
+      p->groups.push_back(g);
+      uint64_t pow2 = 1;
+      while (pow2 < gt) pow2 <<= 1;
+      sig << "_k" << kv.first << "x" << kv.second.size() << "n" << pow2;
     }
     p->numItems = numBatches;
     p->total    = total;
@@ -186,10 +201,10 @@ OPENVKL_INTERFACE VKLSampleBatchPlan vklNewSampleBatchPlan(unsigned int numBatch
     p->items.upload(items.data(), items.size() * sizeof(BatchItem), dev->stream);
     p->begins.allocate(begins.size() * sizeof(uint64_t));
     p->begins.upload(begins.data(), begins.size() * sizeof(uint64_t), dev->stream);
-    uint64_t pow2 = 1;
-    while (pow2 < total) pow2 <<= 1;
-    std::ostringstream sig;
-    sig << "b" << numBatches << "_n" << pow2 << "_f" << fast[0] << "." << fast[1] << "." << fast[2];
+    for (size_t g = 0; g < p->groups.size(); g++) {
+      p->groups[g].items  = p->items.as<BatchItem>() + firstOf[g];
+      p->groups[g].begins = p->begins.as<uint64_t>() + firstOf[g];
+    }
     p->signature = sig.str();
     int cus = 256;
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev->hipDevice);
@@ -198,9 +213,7 @@ OPENVKL_INTERFACE VKLSampleBatchPlan vklNewSampleBatchPlan(unsigned int numBatch
     hipStream_t cap;
     checkHip(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking), "hipStreamCreate");
     checkHip(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-    checkHip(launchSampleBatch(p->items.as<BatchItem>(), p->begins.as<uint64_t>(), p->numItems, p->total,
-                               p->launch, cap),
-             "batch capture");
+    checkHip(launchSampleBatch(p->groups.data(), (uint32_t)p->groups.size(), p->launch, cap), "batch capture");
     checkHip(hipStreamEndCapture(cap, &p->graph), "hipStreamEndCapture");
     checkHip(hipGraphInstantiate(&p->graphExec, p->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
     (void)hipStreamDestroy(cap);

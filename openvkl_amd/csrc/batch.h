@@ -1,4 +1,5 @@
-// Batched multi-volume sampling: descriptor table entries and launch shape.
+// Batched multi-volume sampling: descriptor table entries, specialisation
+// groups and launch shape.
 #pragma once
 
 #include <hip/hip_runtime_api.h>
@@ -7,8 +8,6 @@
 
 namespace vklhip {
 
-enum : int32_t { kBatchGeneric = 0, kBatchDenseLinearF32 = 1, kBatchDenseLinearF16 = 2 };
-
 // One (sampler, points, outputs) batch; lives in device memory.
 struct BatchItem
 {
@@ -16,13 +15,27 @@ struct BatchItem
   AttrView attr;
   const float *xyz;  // count * 3 floats (device)
   float *out;        // count floats (device)
-  uint64_t begin;    // offset of this batch in the flattened point space
+  uint64_t begin;    // offset of this batch in its group's flattened point space
   uint64_t count;
   uint32_t attrIndex;
-  int32_t fastPath;
+  uint32_t pad;
 };
 
-// Launch shape chosen by the autotuner.
+// Batches sharing one kernel specialisation. grid / attr / filter (host
+// copies from the group's first batch) select the instantiation; items and
+// begins are device arrays.
+struct BatchGroup
+{
+  GridView grid;
+  AttrView attr;
+  int32_t filter;
+  uint32_t numItems;
+  uint64_t total;
+  const BatchItem *items;
+  const uint64_t *begins;
+};
+
+// Launch shape chosen by the autotuner (the same for every group).
 struct BatchLaunch
 {
   uint32_t blocks;
@@ -30,7 +43,19 @@ struct BatchLaunch
   uint32_t tile;  // points per workgroup per iteration
 };
 
-hipError_t launchSampleBatch(const BatchItem *items, const uint64_t *begins, uint32_t numItems, uint64_t total,
-                             const BatchLaunch &cfg, hipStream_t stream);
+hipError_t launchSampleBatch(const BatchGroup *groups, uint32_t numGroups, const BatchLaunch &cfg,
+                             hipStream_t stream);
+
+// Specialisation key of (grid, attribute, filter): batches with equal keys
+// run through the same kernel instantiation.
+inline uint32_t sourceKey(const GridView &g, const AttrView &a, int filter)
+{
+  uint32_t big = 0;
+  if (g.dense) {
+    const uint64_t nbz = ((uint64_t)g.activeSize[2] + (1u << g.layout.sz) - 1) >> g.layout.sz;
+    big                = nbz * g.layout.strideZ >= (1ull << 32) ? 1u : 0u;
+  }
+  return (a.type << 12) | ((uint32_t)filter << 2) | (g.dense ? 2u : 0u) | big;
+}
 
 }  // namespace vklhip

@@ -35,6 +35,15 @@ hipError_t launchSwizzle(const void *src, void *dst, uint32_t elemSize, const ui
 hipError_t launchPackCorners(const void *src, void *dst, uint32_t elemSize, const uint32_t dims[3],
                              hipStream_t stream);
 
+// Corner-packed vdb (commit): the leaf index grid over g.leafDims (filled
+// with 0xffffffff, then the slot of each dense leaf; origins absolute, 3
+// int32 per leaf, packed-slot order) and the cell records (8 floats per cell,
+// 512 cells per dense leaf) of one attribute.
+hipError_t launchBuildLeafIndex(const GridView &g, const int32_t *origins, uint64_t numLeaves, uint32_t *index,
+                                hipStream_t stream);
+hipError_t launchPackLeafCells(const GridView &g, const AttrView &a, uint32_t attr, const int32_t *origins,
+                               uint64_t numLeaves, float *cells, hipStream_t stream);
+
 // Per-leaf value ranges over the [-1, +10) voxel window (majorant build).
 // leafOffsets: 3 u32 root-relative leaf origins per leaf.
 hipError_t launchLeafRanges(const GridView &g, uint32_t attr, uint64_t numLeaves, const uint32_t *leafOffsets,
@@ -45,9 +54,12 @@ hipError_t launchLeafRanges(const GridView &g, uint32_t attr, uint64_t numLeaves
 hipError_t launchIntervalInit(const IteratorContextView &c, IntervalState *state, const float *ray, hipStream_t stream);
 hipError_t launchIntervalNext(const IteratorContextView &c, IntervalState *state, IntervalOut *out, int *result,
                               hipStream_t stream);
-// Hit steps take the context in device memory (IteratorContext::deviceView).
+// Hit steps take the context in device memory (IteratorContext::deviceView);
+// hostView (the same context on the host) and the attribute's view select
+// the kernel specialisation.
 hipError_t launchHitInit(const IteratorContextView *c, HitState *state, const float *ray, hipStream_t stream);
-hipError_t launchHitNext(const IteratorContextView *c, HitState *state, HitOut *out, int *result, hipStream_t stream);
+hipError_t launchHitNext(const IteratorContextView &hostView, const IteratorContextView *c, const AttrView &a,
+                         HitState *state, HitOut *out, int *result, hipStream_t stream);
 
 // Batched iteration: one ray per thread; ray r writes up to maxOut results at
 // out[r*maxOut], count[r] = number written. org / dir: 3 floats per ray;
@@ -55,8 +67,8 @@ hipError_t launchHitNext(const IteratorContextView *c, HitState *state, HitOut *
 hipError_t launchIntervalBatch(const IteratorContextView &c, uint32_t numRays, const float *org, const float *dir,
                                const float *tRanges, IntervalOut *out, uint32_t maxOut, uint32_t *count,
                                hipStream_t stream);
-hipError_t launchHitBatch(const IteratorContextView *c, uint32_t numRays, const float *org, const float *dir,
-                          const float *tRanges, const float *times, HitOut *out, uint32_t maxOut, uint32_t *count,
-                          hipStream_t stream);
+hipError_t launchHitBatch(const IteratorContextView &hostView, const IteratorContextView *c, const AttrView &a,
+                          uint32_t numRays, const float *org, const float *dir, const float *tRanges,
+                          const float *times, HitOut *out, uint32_t maxOut, uint32_t *count, hipStream_t stream);
 
 }  // namespace vklhip

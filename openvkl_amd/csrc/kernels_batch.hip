@@ -1,13 +1,13 @@
-// Batched sampling: many volumes with ragged point batches in one persistent
-// launch. The flattened point space [0, total) is split into tiles of
-// `tile` points; workgroups stride over tiles; each work-item finds its batch
-// by binary search over the batch prefix offsets (staged in LDS).
-//
-// Results equal one vklComputeSampleN per batch: the same per-point device
-// functions run (dense linear float / half inline, everything else through
-// the runtime-dispatched sampler).
+// Batched sampling: many volumes with ragged point batches. The batches of a
+// plan are grouped by sampler specialisation (voxel source type and filter);
+// each group is one persistent launch over its flattened point space, split
+// into tiles of `tile` points that workgroups stride over; each work-item
+// finds its batch by binary search over the group's batch offsets (staged in
+// LDS). Results equal one vklComputeSampleN per batch: the same per-point
+// device functions run.
 
 #include "batch.h"
+#include "kernels_dispatch.hh"
 #include "vkl_device.hh"
 
 namespace vklhip {
@@ -28,14 +28,7 @@ __device__ __forceinline__ uint32_t findItem(const uint64_t *begins, uint32_t n,
   return lo;
 }
 
-template <typename T>
-__device__ __forceinline__ float denseLinear(const BatchItem &it, F3 oc)
-{
-  const F3 p = xfmPoint(it.sampler.grid.o2i, oc);
-  const DenseSource<T, false> src(it.sampler.grid, it.attr, it.attrIndex);
-  return sampleLinear(src, p);
-}
-
+template <typename Src, int kFilter>
 __global__ void kSampleBatch(const BatchItem *__restrict__ items, const uint64_t *__restrict__ begins,
                              uint32_t numItems, uint64_t total, uint32_t tile)
 {
@@ -49,29 +42,42 @@ __global__ void kSampleBatch(const BatchItem *__restrict__ items, const uint64_t
   for (uint64_t base = (uint64_t)blockIdx.x * tile; base < total; base += (uint64_t)gridDim.x * tile) {
     const uint64_t end = base + tile < total ? base + tile : total;
     for (uint64_t p = base + threadIdx.x; p < end; p += blockDim.x) {
-      const uint32_t k   = findItem(b, numItems, p);
+      const uint32_t k    = findItem(b, numItems, p);
       const BatchItem &it = items[k];
-      const uint64_t i   = p - it.begin;
-      const F3 oc        = {it.xyz[3 * i], it.xyz[3 * i + 1], it.xyz[3 * i + 2]};
-      float r;
-      if (it.fastPath == kBatchDenseLinearF32)
-        r = denseLinear<float>(it, oc);
-      else if (it.fastPath == kBatchDenseLinearF16)
-        r = denseLinear<__half>(it, oc);
-      else
-        r = sampleObject(it.sampler, it.attrIndex, oc);
-      it.out[i] = r;
+      const uint64_t i    = p - it.begin;
+      const F3 oc         = {it.xyz[3 * i], it.xyz[3 * i + 1], it.xyz[3 * i + 2]};
+      const PointSampler<Src, kFilter> smp(it.sampler.grid, it.attr, it.attrIndex);
+      it.out[i] = smp(oc);
     }
   }
 }
 
-hipError_t launchSampleBatch(const BatchItem *items, const uint64_t *begins, uint32_t numItems, uint64_t total,
-                             const BatchLaunch &cfg, hipStream_t stream)
+namespace {
+struct BatchGroupLaunch
 {
-  if (total == 0 || numItems == 0) return hipSuccess;
-  hipLaunchKernelGGL(kSampleBatch, dim3(cfg.blocks), dim3(cfg.threads), 0, stream, items, begins, numItems, total,
-                     cfg.tile);
-  return hipGetLastError();
+  const BatchGroup &grp;
+  const BatchLaunch &cfg;
+  hipStream_t st;
+  template <typename Src, int kFilter>
+  hipError_t operator()() const
+  {
+    hipLaunchKernelGGL((kSampleBatch<Src, kFilter>), dim3(cfg.blocks), dim3(cfg.threads), 0, st, grp.items,
+                       grp.begins, grp.numItems, grp.total, cfg.tile);
+    return hipGetLastError();
+  }
+};
+}  // namespace
+
+hipError_t launchSampleBatch(const BatchGroup *groups, uint32_t numGroups, const BatchLaunch &cfg,
+                             hipStream_t stream)
+{
+  for (uint32_t g = 0; g < numGroups; g++) {
+    const BatchGroup &grp = groups[g];
+    if (grp.total == 0 || grp.numItems == 0) continue;
+    const hipError_t e = dispatchSource(grp.grid, grp.attr, grp.filter, BatchGroupLaunch{grp, cfg, stream});
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 }  // namespace vklhip

@@ -1,0 +1,47 @@
+// Host-side selection of a kernel specialisation: voxel source type (dense
+// element type and 32- / 64-bit offsets, or sparse vdb float / half) and
+// filter. `f` is a functor with
+//   template <typename Src, int kFilter> hipError_t operator()() const;
+// that launches the matching instantiation.
+#pragma once
+
+#include "vkl_device.hh"
+
+namespace vklhip {
+
+template <typename Src, typename F>
+hipError_t withFilter(int filter, const F &f)
+{
+  if (filter == kFilterNearest) return f.template operator()<Src, kFilterNearest>();
+  if (filter == kFilterCubic) return f.template operator()<Src, kFilterCubic>();
+  return f.template operator()<Src, kFilterLinear>();
+}
+
+template <typename T, typename F>
+hipError_t withDense(const GridView &g, int filter, const F &f)
+{
+  const uint64_t nbz = ((uint64_t)g.activeSize[2] + (1u << g.layout.sz) - 1) >> g.layout.sz;
+  if (nbz * g.layout.strideZ >= (1ull << 32)) return withFilter<DenseSource<T, true>>(filter, f);
+  return withFilter<DenseSource<T, false>>(filter, f);
+}
+
+template <typename F>
+hipError_t dispatchSource(const GridView &g, const AttrView &a, int filter, const F &f)
+{
+  if (g.dense) {
+    switch (a.type) {
+    case kTypeUchar: return withDense<uint8_t>(g, filter, f);
+    case kTypeShort: return withDense<int16_t>(g, filter, f);
+    case kTypeUshort: return withDense<uint16_t>(g, filter, f);
+    case kTypeHalf: return withDense<__half>(g, filter, f);
+    case kTypeFloat: return withDense<float>(g, filter, f);
+    case kTypeDouble: return withDense<double>(g, filter, f);
+    default: return hipErrorInvalidValue;
+    }
+  }
+  if (a.type == kTypeHalf) return withFilter<SparseSource<__half>>(filter, f);
+  if (a.type == kTypeFloat) return withFilter<SparseSource<float>>(filter, f);
+  return hipErrorInvalidValue;
+}
+
+}  // namespace vklhip

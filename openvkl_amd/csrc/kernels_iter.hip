@@ -6,6 +6,7 @@
 //    batched kernels (one ray per work-item) for vklIterate*Batch.
 
 #include "kernels.h"
+#include "kernels_dispatch.hh"
 #include "vkl_device.hh"
 
 namespace vklhip {
@@ -115,21 +116,24 @@ __global__ void kIntervalNext(IteratorContextView c, IntervalState *s, IntervalO
   *result = r ? 1 : 0;
 }
 
-// The hit kernels read their context from device memory: the exact sampler
-// they call takes it by reference, and a kernel-argument copy would have to
-// be spilled to the private segment first.
+// The hit kernels read their context from device memory and sample through
+// a PointSampler specialised on the volume's source type and filter (chosen
+// on the host from the same context).
 __global__ void kHitInit(const IteratorContextView *__restrict__ cp, HitState *s, const float *ray)
 {
   const IteratorContextView &c = *cp;
   hitInit(c, *s, F3{ray[0], ray[1], ray[2]}, F3{ray[3], ray[4], ray[5]}, Range1f{ray[6], ray[7]}, ray[8]);
 }
 
-__global__ void kHitNext(const IteratorContextView *__restrict__ cp, HitState *s, HitOut *out, int *result)
+template <typename Src, int kFilter>
+__global__ void kHitNext(const IteratorContextView *__restrict__ cp, AttrView a, HitState *s, HitOut *out,
+                         int *result)
 {
   const IteratorContextView &c = *cp;
+  const PointSampler<Src, kFilter> smp(c.sampler.grid, a, c.attributeIndex);
   HitState st = *s;
   HitOut o;
-  const bool r = hitNext(c, st, o);
+  const bool r = hitNext(c, smp, st, o);
   *s           = st;
   if (r) *out = o;
   *result = r ? 1 : 0;
@@ -151,10 +155,30 @@ hipError_t launchHitInit(const IteratorContextView *c, HitState *state, const fl
   hipLaunchKernelGGL(kHitInit, dim3(1), dim3(1), 0, st, c, state, ray);
   return hipGetLastError();
 }
-hipError_t launchHitNext(const IteratorContextView *c, HitState *state, HitOut *out, int *result, hipStream_t st)
+
+namespace {
+struct HitNextLaunch
 {
-  hipLaunchKernelGGL(kHitNext, dim3(1), dim3(1), 0, st, c, state, out, result);
-  return hipGetLastError();
+  const IteratorContextView *c;
+  AttrView a;
+  HitState *state;
+  HitOut *out;
+  int *result;
+  hipStream_t st;
+  template <typename Src, int kFilter>
+  hipError_t operator()() const
+  {
+    hipLaunchKernelGGL((kHitNext<Src, kFilter>), dim3(1), dim3(1), 0, st, c, a, state, out, result);
+    return hipGetLastError();
+  }
+};
+}  // namespace
+
+hipError_t launchHitNext(const IteratorContextView &hostView, const IteratorContextView *c, const AttrView &a,
+                         HitState *state, HitOut *out, int *result, hipStream_t st)
+{
+  return dispatchSource(hostView.sampler.grid, a, hostView.sampler.filter,
+                        HitNextLaunch{c, a, state, out, result, st});
 }
 
 // ---- batches ---------------------------------------------------------------------
@@ -184,8 +208,10 @@ __global__ __launch_bounds__(kIterBlock) void kIntervalBatch(IteratorContextView
   count[r] = n;
 }
 
-__global__ __launch_bounds__(kIterBlock) void kHitBatch(const IteratorContextView *__restrict__ cp, uint32_t numRays,
-                                                         const float *__restrict__ org, const float *__restrict__ dir,
+template <typename Src, int kFilter>
+__global__ __launch_bounds__(kIterBlock) void kHitBatch(const IteratorContextView *__restrict__ cp, AttrView a,
+                                                         uint32_t numRays, const float *__restrict__ org,
+                                                         const float *__restrict__ dir,
                                                          const float *__restrict__ tRanges,
                                                          const float *__restrict__ times, HitOut *__restrict__ out,
                                                          uint32_t maxOut, uint32_t *__restrict__ count)
@@ -193,13 +219,14 @@ __global__ __launch_bounds__(kIterBlock) void kHitBatch(const IteratorContextVie
   const uint32_t r = blockIdx.x * kIterBlock + threadIdx.x;
   if (r >= numRays) return;
   const IteratorContextView &c = *cp;
+  const PointSampler<Src, kFilter> smp(c.sampler.grid, a, c.attributeIndex);
   HitState st;
   hitInit(c, st, F3{org[3 * r], org[3 * r + 1], org[3 * r + 2]}, F3{dir[3 * r], dir[3 * r + 1], dir[3 * r + 2]},
           rayRange(tRanges, r), times ? times[r] : 0.f);
   uint32_t n = 0;
   HitOut o;
   HitOut *dst = out + (uint64_t)r * maxOut;
-  while (n < maxOut && hitNext(c, st, o)) dst[n++] = o;
+  while (n < maxOut && hitNext(c, smp, st, o)) dst[n++] = o;
   count[r] = n;
 }
 
@@ -213,14 +240,34 @@ hipError_t launchIntervalBatch(const IteratorContextView &c, uint32_t numRays, c
   return hipGetLastError();
 }
 
-hipError_t launchHitBatch(const IteratorContextView *c, uint32_t numRays, const float *org, const float *dir,
-                          const float *tRanges, const float *times, HitOut *out, uint32_t maxOut, uint32_t *count,
-                          hipStream_t st)
+namespace {
+struct HitBatchLaunch
+{
+  const IteratorContextView *c;
+  AttrView a;
+  uint32_t numRays;
+  const float *org, *dir, *tRanges, *times;
+  HitOut *out;
+  uint32_t maxOut;
+  uint32_t *count;
+  hipStream_t st;
+  template <typename Src, int kFilter>
+  hipError_t operator()() const
+  {
+    hipLaunchKernelGGL((kHitBatch<Src, kFilter>), dim3((numRays + kIterBlock - 1) / kIterBlock), dim3(kIterBlock), 0,
+                       st, c, a, numRays, org, dir, tRanges, times, out, maxOut, count);
+    return hipGetLastError();
+  }
+};
+}  // namespace
+
+hipError_t launchHitBatch(const IteratorContextView &hostView, const IteratorContextView *c, const AttrView &a,
+                          uint32_t numRays, const float *org, const float *dir, const float *tRanges,
+                          const float *times, HitOut *out, uint32_t maxOut, uint32_t *count, hipStream_t st)
 {
   if (numRays == 0) return hipSuccess;
-  hipLaunchKernelGGL(kHitBatch, dim3((numRays + kIterBlock - 1) / kIterBlock), dim3(kIterBlock), 0, st, c, numRays,
-                     org, dir, tRanges, times, out, maxOut, count);
-  return hipGetLastError();
+  return dispatchSource(hostView.sampler.grid, a, hostView.sampler.filter,
+                        HitBatchLaunch{c, a, numRays, org, dir, tRanges, times, out, maxOut, count, st});
 }
 
 }  // namespace vklhip

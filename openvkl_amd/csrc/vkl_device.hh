@@ -212,8 +212,11 @@ struct SparseSource
   const uint64_t *leafSlot;
   const T *dense;
   const T *tile;
+  const float *cells;         // corner-packed cell records (or null)
+  const uint32_t *leafIndex;  // leaf cell -> dense-leaf slot (corner-packed)
   int32_t root[3];
   uint32_t size[3];
+  uint32_t ldim[3];
   uint32_t maxDepth, numAttr, attr;
   float bg;
 
@@ -221,6 +224,8 @@ struct SparseSource
       : leafSlot(g.leafSlot),
         dense((const T *)a.data),
         tile((const T *)a.tile),
+        cells(g.maxSamplingDepth == g.packedDepth ? a.cells : nullptr),
+        leafIndex(g.leafIndex),
         maxDepth(g.maxSamplingDepth),
         numAttr(g.numAttributes),
         attr(attributeIndex),
@@ -231,6 +236,7 @@ struct SparseSource
       lv[i]   = g.levels[i];
       root[i] = g.rootOrigin[i];
       size[i] = g.activeSize[i];
+      ldim[i] = g.leafDims[i];
     }
   }
 
@@ -291,11 +297,23 @@ struct SparseSource
     return 0.f;
   }
 
-  // When the 2x2x2 stencil lies in one leaf (7/8 per axis), one traversal
-  // serves all corners; the values equal eight separate traversals.
+  // Corner-packed volumes: a cell inside a dense leaf reads its record (the
+  // 8 values at() returns for its corners, baked at commit). Otherwise, when
+  // the 2x2x2 stencil lies in one leaf (7/8 per axis), one traversal serves
+  // all corners; the values equal eight separate traversals.
   VKL_DEV void corners8(I3 ic, float *s) const
   {
     const uint32_t ox = (uint32_t)(ic.x - root[0]), oy = (uint32_t)(ic.y - root[1]), oz = (uint32_t)(ic.z - root[2]);
+    if (cells && inDomain(ox, oy, oz)) {
+      const uint32_t slot = leafIndex[((uint64_t)(oz >> 3) * ldim[1] + (oy >> 3)) * ldim[0] + (ox >> 3)];
+      if (slot != 0xffffffffu) {
+        const CellRecord<float> c = *reinterpret_cast<const CellRecord<float> *>(
+            cells + ((uint64_t)slot * 512u + (((ox & 7u) << 6) | ((oy & 7u) << 3) | (oz & 7u))) * 8u);
+#pragma unroll
+        for (int i = 0; i < 8; i++) s[i] = c.v[i];
+        return;
+      }
+    }
     if (inDomain(ox, oy, oz) && inDomain(ox + 1, oy + 1, oz + 1) && (ox & 7u) != 7u && (oy & 7u) != 7u &&
         (oz & 7u) != 7u) {
       uint32_t el = 0, eo = 0;
@@ -485,36 +503,32 @@ VKL_DEV float sampleFiltered(const Src &src, int filter, F3 p, float nearestOffs
   return sampleLinear(src, p);
 }
 
-// Runtime-dispatched sample at an object-space point (the sampler entry of
-// VdbSampler_iterator_computeSample_varying, VdbSampler.ispc:341-374). Not
-// inlined: it covers every element type x filter, and iterator code calls it
-// from several places.
-static __device__ __noinline__ float sampleObject(const SamplerView &s, uint32_t attr, F3 oc)
+// Object-space sample with one source type and filter per instantiation (the
+// sampler entry of VdbSampler_iterator_computeSample_varying,
+// VdbSampler.ispc:341-374). Every caller gets its own inlined copy: the
+// library has no non-inlined device functions, because branch relaxation in
+// a large callee on gfx950 rewrote the return-address register pair s[30:31]
+// for its long jumps, which sent the callee's return back into its own body.
+template <typename Src, int kFilter>
+struct PointSampler
 {
-  const GridView &g = s.grid;
-  const F3 p        = xfmPoint(g.o2i, oc);
-  const AttrView a  = g.attrs[attr];
-  const float noff  = g.constantCellData ? 0.f : 0.5f;
-  if (g.dense) {
-    const bool big = denseNeeds64(g);
-    switch (a.type) {
-#define VKL_DENSE_CASE(CODE, T)                                                       \
-  case CODE:                                                                          \
-    if (big) return sampleFiltered(DenseSource<T, true>(g, a, attr), s.filter, p, noff);   \
-    return sampleFiltered(DenseSource<T, false>(g, a, attr), s.filter, p, noff);
-      VKL_DENSE_CASE(kTypeUchar, uint8_t)
-      VKL_DENSE_CASE(kTypeShort, int16_t)
-      VKL_DENSE_CASE(kTypeUshort, uint16_t)
-      VKL_DENSE_CASE(kTypeHalf, __half)
-      VKL_DENSE_CASE(kTypeDouble, double)
-      VKL_DENSE_CASE(kTypeFloat, float)
-#undef VKL_DENSE_CASE
-    }
-    return 0.f;
+  Src src;
+  const float *o2i;
+  float noff;
+
+  VKL_DEV PointSampler(const GridView &g, const AttrView &a, uint32_t attr)
+      : src(g, a, attr), o2i(g.o2i), noff(g.constantCellData ? 0.f : 0.5f)
+  {
   }
-  if (a.type == kTypeHalf) return sampleFiltered(SparseSource<__half>(g, a, attr), s.filter, p, noff);
-  return sampleFiltered(SparseSource<float>(g, a, attr), s.filter, p, noff);
-}
+
+  VKL_DEV float operator()(F3 oc) const
+  {
+    const F3 p = xfmPoint(o2i, oc);
+    if (kFilter == kFilterNearest) return sampleNearest(src, p, noff);
+    if (kFilter == kFilterCubic) return sampleCubic(src, p);
+    return sampleLinear(src, p);
+  }
+};
 
 // ---- interval iteration (VdbIterator.ispc:36-180, Dda.ih, HDDA.ih.in,
 //      VdbQueryVoxel.ih.in, VdbQueryVoxelDense.ih) -----------------------------
@@ -782,21 +796,22 @@ VKL_DEV bool intervalNext(const IteratorContextView &c, IntervalState &it, Inter
 
 // ---- hit iteration (DefaultIterator.ispc:141-258, Iterator.ih:17-200) --------
 
-VKL_DEV float sampleAlong(const IteratorContextView &c, F3 o, F3 d, float t)
+template <typename S>
+VKL_DEV float sampleAlong(const S &smp, F3 o, F3 d, float t)
 {
-  return sampleObject(c.sampler, c.attributeIndex, F3{o.x + t * d.x, o.y + t * d.y, o.z + t * d.z});
+  return smp(F3{o.x + t * d.x, o.y + t * d.y, o.z + t * d.z});
 }
 
 // bisect, Iterator.ih:23-85.
-VKL_DEV float bisect(const IteratorContextView &c, F3 o, F3 d, float t0, float s0, float t, float s, float iso,
-                     float tol)
+template <typename S>
+VKL_DEV float bisect(const S &smp, F3 o, F3 d, float t0, float s0, float t, float s, float iso, float tol)
 {
   float tMid = 0.f;
   for (int iter = 0; iter < 10; ++iter) {
     tMid            = 0.5f * (t0 + t);
     const float err = 0.5f * (t - t0);
     if (err < tol) break;
-    const float sm = sampleAlong(c, o, d, tMid);
+    const float sm = sampleAlong(smp, o, d, tMid);
     if (isnan(sm)) return (fabsf(iso - s0) < fabsf(iso - s)) ? t0 : t;
     if (sm == iso || (iso - s0) * (iso - sm) < 0.f) {
       t = tMid;
@@ -812,24 +827,25 @@ VKL_DEV float bisect(const IteratorContextView &c, F3 o, F3 d, float t0, float s
 }
 
 // intersectSurfacesBisection, Iterator.ih:87-197.
-VKL_DEV bool intersectBisection(const IteratorContextView &c, F3 o, F3 d, float tLo, float tHi, float step,
-                                HitOut &hit)
+template <typename S>
+VKL_DEV bool intersectBisection(const IteratorContextView &c, const S &smp, F3 o, F3 d, float tLo, float tHi,
+                                float step, HitOut &hit)
 {
   float t0 = tLo;
-  float s0 = sampleAlong(c, o, d, t0);
+  float s0 = sampleAlong(smp, o, d, t0);
   for (int it = 0; it < 10 && isnan(s0) && t0 < tHi; ++it) {
     t0 += vmax(1e-5f, 1e-5f * step);
-    s0 = sampleAlong(c, o, d, t0);
+    s0 = sampleAlong(smp, o, d, t0);
   }
   while (t0 < tHi) {
     const float h = vmin(step, tHi - t0);
     const float t = t0 + h;
     if (!(t > t0)) break;  // no float progress: the march cannot continue
-    float s       = sampleAlong(c, o, d, t);
+    float s       = sampleAlong(smp, o, d, t);
     float ts      = t;
     for (int it = 0; it < 10 && isnan(s) && ts > t0; ++it) {
       ts -= vmax(1e-5f, 1e-5f * step);
-      s = sampleAlong(c, o, d, ts);
+      s = sampleAlong(smp, o, d, ts);
     }
     float tHit = vinf(), eps = vinf(), value = vinf();
     for (int i = 0; i < c.numValues; i++) {
@@ -841,7 +857,7 @@ VKL_DEV bool intersectBisection(const IteratorContextView &c, F3 o, F3 d, float 
           eps   = 0.125f * step;
         }
       } else if (!isnan(s0 + s) && (v - s0) * (v - s) < 0.f) {
-        const float tIso = bisect(c, o, d, t0, s0, ts, s, v, vmin(1e-6f, 0.01f * h));
+        const float tIso = bisect(smp, o, d, t0, s0, ts, s, v, vmin(1e-6f, 0.01f * h));
         if (tIso < tHit && tIso <= tHi) {
           tHit  = tIso;
           value = v;
@@ -885,7 +901,8 @@ VKL_DEV void hitInit(const IteratorContextView &c, HitState &h, F3 origin, F3 di
 }
 
 // DefaultHitIterator_iterateHit, DefaultIterator.ispc:168-258.
-VKL_DEV bool hitNext(const IteratorContextView &c, HitState &h, HitOut &hit)
+template <typename S>
+VKL_DEV bool hitNext(const IteratorContextView &c, const S &smp, HitState &h, HitOut &hit)
 {
   if (c.numValues == 0) return false;
   if (h.curLower == vinf()) return false;
@@ -912,7 +929,7 @@ VKL_DEV bool hitNext(const IteratorContextView &c, HitState &h, HitOut &hit)
       return false;
     }
     hit.t = vinf();
-    if (intersectBisection(c, o, d, h.curLower, h.curUpper, h.curNominalDeltaT, hit)) {
+    if (intersectBisection(c, smp, o, d, h.curLower, h.curUpper, h.curNominalDeltaT, hit)) {
       h.curLower       = hit.t + hit.epsilon;
       h.lastHitOffsetT = hit.t + hit.epsilon;
       return true;

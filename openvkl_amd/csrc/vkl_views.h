@@ -38,6 +38,10 @@ struct AttrView
   const void *tile;   // vdb: packed tile values (one per tile leaf), may be null
   uint32_t type;      // VKLDataType of the elements
   float background;   // value outside the domain
+  // vdb with "cornerPacked": per dense leaf (packed-slot order) 512 cell
+  // records of 8 floats, the values the sampler reads at the cell's 8
+  // trilinear corners (record order 4 dx + 2 dy + dz); null otherwise.
+  const float *cells;
 };
 
 // One inner level of the node table (levels 0..2).
@@ -84,6 +88,12 @@ struct GridView
   const AttrView *attrs;      // device array [numAttributes]
   const uint64_t *leafSlot;   // vdb with per-leaf node.data: leaf index -> packed slot
   LevelView levels[3];
+  // vdb with "cornerPacked": dense-leaf slot (or 0xffffffff) per 8^3 leaf
+  // cell of the domain, x fastest, leafDims = ceil(activeSize / 8); the cell
+  // records hold values for maxSamplingDepth == packedDepth.
+  const uint32_t *leafIndex;
+  uint32_t leafDims[3];
+  uint32_t packedDepth;
 };
 
 // Sampler = grid + filters.

@@ -381,7 +381,7 @@ void DenseVolume::commit()
   const std::vector<float> bg = parseBackground(*this, grid.numAttributes);
   attrViews.clear();
   for (size_t a = 0; a < attrs.size(); a++) {
-    AttrView av;
+    AttrView av{};
     if (packed) {
       const Data &d = *attrs[a];
       std::unique_ptr<DeviceBuffer> tmp;
@@ -619,6 +619,31 @@ void VdbVolume::commit()
   std::vector<uint64_t> leafId(n);
   for (uint64_t i = 0; i < n; i++) leafId[i] = packedD ? slot[i] : i;
   finishGrid(*this, level, format, offs, leafId, st);
+
+  // "cornerPacked" (bool): a leaf index grid over the domain plus, per dense
+  // leaf, 512 cell records of the 8 corner values the sampler reads (8 floats
+  // per cell; baked for this volume's maxSamplingDepth).
+  grid.leafIndex   = nullptr;
+  grid.packedDepth = 0xffffffffu;
+  if (getBool("cornerPacked", false)) {
+    for (int c = 0; c < 3; c++) grid.leafDims[c] = (grid.activeSize[c] + 7) / 8;
+    const uint64_t cellsOfGrid = (uint64_t)grid.leafDims[0] * grid.leafDims[1] * grid.leafDims[2];
+    const uint64_t numDense    = denseLeafOrigins.size() / 3;
+    DeviceBuffer origins(denseLeafOrigins.size() * sizeof(int32_t));
+    origins.upload(denseLeafOrigins.data(), denseLeafOrigins.size() * sizeof(int32_t), st);
+    DeviceBuffer *index = newBuffer(cellsOfGrid * sizeof(uint32_t));
+    checkHip(launchBuildLeafIndex(grid, origins.as<int32_t>(), numDense, index->as<uint32_t>(), st), "leaf index");
+    for (uint32_t a = 0; a < grid.numAttributes; a++) {
+      DeviceBuffer *cells = newBuffer(numDense * 512 * 8 * sizeof(float));
+      checkHip(launchPackLeafCells(grid, attrViews[a], a, origins.as<int32_t>(), numDense, cells->as<float>(), st),
+               "cell records");
+      attrViews[a].cells = cells->as<float>();
+    }
+    checkHip(hipStreamSynchronize(st), "corner packing");
+    uploadAttrViews(*this, st);
+    grid.leafIndex   = index->as<uint32_t>();
+    grid.packedDepth = grid.maxSamplingDepth;
+  }
   committed = true;
 }
 

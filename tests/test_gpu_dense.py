@@ -144,8 +144,6 @@ def test_cell_centered_background_transform(device):
     np.testing.assert_array_equal(vol.value_range(), grid.value_range())
 
 
-@pytest.mark.skip(reason="a GPU run of this case stalled without a shown cause; the runtime-dispatched "
-                  "multi-attribute kernel it used is removed and the case stays off the GPU until a cause is shown")
 def test_multi_attribute_sample_m(device):
     rng = np.random.default_rng(31)
     a0 = rng.standard_normal((20, 20, 20)).astype(np.float32)

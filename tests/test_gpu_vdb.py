@@ -67,9 +67,10 @@ def test_max_sampling_depth_error_voxels(shell_vdb, depth):
     assert_same(s.sample(pts[:100000]), grid.sample(pts[:100000]))
 
 
-def test_node_data_half_tiles_multi_attribute(device):
+@pytest.mark.parametrize("packed", [False, True])
+def test_node_data_half_tiles_multi_attribute(device, packed):
     """Per-leaf node.data (no repack), half + float attributes, tiles on
-    levels 1..3, background."""
+    levels 1..3, background; also through corner-packed cell records."""
     origins, leaves = V.sparse_shell_leaves(res=128, radius=0.3, width=0.08)
     n = len(origins)
     rng = np.random.default_rng(43)
@@ -83,7 +84,7 @@ def test_node_data_half_tiles_multi_attribute(device):
     a1 = rng.standard_normal((n, 512)).astype(np.float16)
     leaf_data = [[leaves[i], a1[i]] for i in range(n)]
     leaf_data += [[np.array([0.75 + k], np.float32), np.array([-1.5 - k], np.float16)] for k in range(3)]
-    vol = device.vdb(lv, org, fmt, leaf_data=leaf_data, background=0.5)
+    vol = device.vdb(lv, org, fmt, leaf_data=leaf_data, background=0.5, corner_packed=packed)
     grid = O.Grid.vdb(lv, org, fmt, leaf_data=leaf_data, background=[0.5, 0.5])
     box = grid.bounding_box()
     np.testing.assert_array_equal(vol.bounding_box(), box)
@@ -100,6 +101,45 @@ def test_node_data_half_tiles_multi_attribute(device):
         _, ovox, orng = grid.level(l)
         np.testing.assert_array_equal(vox, ovox)
         np.testing.assert_array_equal(r.view(np.uint32), orng.view(np.uint32))
+
+
+def test_corner_packed_vdb_bitexact(device):
+    """Corner-packed vdb (leaf index grid + cell records baked at commit):
+    samples and gradients equal the oracle's for every filter, points inside
+    leaves, at leaf faces, in empty space and outside the domain."""
+    origins, leaves = V.sparse_shell_leaves(res=128, radius=0.3, width=0.08)
+    n = len(origins)
+    lv, fm = np.full(n, 3, np.uint32), np.full(n, vkl.VKL_FORMAT_DENSE_ZYX, np.uint32)
+    vol = device.vdb(lv, origins, fm, packed_dense=[leaves.reshape(-1)], corner_packed=True)
+    grid = O.Grid.vdb(lv, origins, fm, packed_dense=[leaves.reshape(-1)])
+    rng = np.random.default_rng(81)
+    k = rng.integers(0, n, 150000)
+    inside = (origins[k] + rng.random((150000, 3)) * 8.0).astype(np.float32)
+    pts = np.concatenate([inside, V.uniform_points(100000, (-3, -3, -3), (131, 131, 131))])
+    for f in (vkl.VKL_FILTER_NEAREST, vkl.VKL_FILTER_LINEAR):
+        grid.set_sampler(f)
+        s = vol.sampler(filter=f)
+        assert_same(s.sample(pts), grid.sample(pts))
+        assert_same(s.gradient(pts), grid.gradient(pts))
+    grid.set_sampler(O.CUBIC)
+    check_cubic(grid, vol.sampler(filter=vkl.VKL_FILTER_CUBIC).sample(pts[:60000]), pts[:60000])
+
+
+@pytest.mark.parametrize("volume_depth,sampler_depth", [(2, None), (2, 3), (3, 1)])
+def test_corner_packed_vdb_sampling_depth(device, volume_depth, sampler_depth):
+    """Cell records hold the volume's maxSamplingDepth values; a sampler with
+    another depth reads through the node table. Both equal the oracle."""
+    origins, leaves = V.sparse_shell_leaves(res=128, radius=0.3, width=0.08)
+    n = len(origins)
+    lv, fm = np.full(n, 3, np.uint32), np.full(n, vkl.VKL_FORMAT_DENSE_ZYX, np.uint32)
+    vol = device.vdb(lv, origins, fm, packed_dense=[leaves.reshape(-1)], corner_packed=True,
+                     max_sampling_depth=volume_depth)
+    grid = O.Grid.vdb(lv, origins, fm, packed_dense=[leaves.reshape(-1)])
+    depth = volume_depth if sampler_depth is None else sampler_depth
+    grid.set_sampler(O.LINEAR, max_sampling_depth=depth)
+    pts = V.uniform_points(200000, (-2, -2, -2), (130, 130, 130))
+    s = vol.sampler(max_sampling_depth=sampler_depth)
+    assert_same(s.sample(pts), grid.sample(pts))
 
 
 def test_index_to_object_transform(device):

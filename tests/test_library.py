@@ -39,3 +39,19 @@ def test_struct_layouts_match_reference():
     assert ctypes.sizeof(vkl.VKLHit) == 12
     assert ctypes.sizeof(vkl.VKLObject) == 16
     assert ctypes.sizeof(vkl.WIDE[16][2]) == 16 * 5 * 4
+
+
+def test_no_non_inlined_device_functions():
+    """The device code has no non-inlined device functions: on gfx950 the
+    compiler's branch relaxation in a large callee reused the return-address
+    register pair s[30:31] for long jumps, and the callee's return then
+    re-entered its own body (a kernel that never ends). Every device function
+    is inlined (kernels have no return address)."""
+    import glob
+    import os
+    import re
+    here = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "openvkl_amd", "csrc")
+    for path in glob.glob(os.path.join(here, "*.hip")) + glob.glob(os.path.join(here, "*.hh")):
+        text = re.sub(r"//.*", "", open(path).read())
+        assert "__noinline__" not in text, path
+        assert "noinline" not in text, path
