@@ -366,10 +366,9 @@ def main():
                     help="device voxel layout: the caller's x-fastest array (with --brick) or corner-packed cells")
     ap.add_argument("--binned", choices=("auto", "on", "off"), default="auto",
                     help="spatially binned sampling (auto: on for large volumes and batches)")
-    ap.add_argument("--variants", default="layouts,config2,raymarch,batched,neural",
-                    help="comma list of extra measurements: layouts, config2, raymarch, batched, neural, iterators ('' = "
-                         "none); iterators (4096^3 vdb, 2^21 rays) is held out of the default run: its last GPU "
-                         "run went silent for 180 s without a cause found yet")
+    ap.add_argument("--variants", default="layouts,config2,raymarch,batched,neural,iterators",
+                    help="comma list of extra measurements: layouts, config2, raymarch, batched, neural, iterators "
+                         "('' = none)")
     ap.add_argument("--nf-res", type=int, default=1024, help="index-domain edge of the neural-field vdb")
     ap.add_argument("--vdb-res", type=int, default=4096, help="index-domain edge of the iterator vdb")
     ap.add_argument("--copy-calibration", action="store_true",
