@@ -12,6 +12,7 @@
 #include "data.h"
 #include "device.h"
 #include "iterator.h"
+#include "observer.h"
 #include "parameters.h"
 #include "sampler.h"
 #include "vdb.h"

@@ -20,7 +20,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "lib", "libopenvkl_hip.so")
 
 # ---- enums (values fixed by the reference headers) -----------------------------------
-VKL_DATA, VKL_VOLUME = 0x8000000 + 100, 0x8000000 + 101
+VKL_OBJECT, VKL_DATA, VKL_VOLUME = 0x8000000, 0x8000000 + 100, 0x8000000 + 101
 VKL_BOOL, VKL_STRING, VKL_VOID_PTR = 250, 1500, 200
 VKL_UCHAR, VKL_SHORT, VKL_USHORT, VKL_INT, VKL_UINT = 2500, 3000, 3500, 4000, 4500
 VKL_HALF, VKL_FLOAT, VKL_DOUBLE = 5800, 6000, 7000
@@ -118,6 +118,13 @@ def lib():
         "vklGetValueRange": (range1f, [O, C.c_uint]),
         "vklNewSampler": (O, [O]),
         "vklNewIntervalIteratorContext": (O, [O]),
+        "vklNewVolumeObserver": (O, [O, C.c_char_p]),
+        "vklNewSamplerObserver": (O, [O, C.c_char_p]),
+        "vklMapObserver": (vp, [O]),
+        "vklUnmapObserver": (None, [O]),
+        "vklGetObserverElementType": (u32, [O]),
+        "vklGetObserverElementSize": (sz, [O]),
+        "vklGetObserverNumElements": (sz, [O]),
         "vklNewHitIteratorContext": (O, [O]),
         "vklSetParam": (None, [O, C.c_char_p, u32, vp]),
         "vklSetBool": (None, [O, C.c_char_p, i32]),
@@ -420,6 +427,16 @@ class Volume(Handle):
              if count else np.zeros((0, na, 2), np.float32))
         return n.value, v, r
 
+    def observer(self, type_name="InnerNode", max_depth=None):
+        """vklNewVolumeObserver ("InnerNode" on vdb volumes; int "maxDepth")."""
+        h = lib().vklNewVolumeObserver(self.h, type_name.encode())
+        self.device.check()
+        o = Observer(self.device, h)
+        if max_depth is not None:
+            o.set_int("maxDepth", max_depth)
+            o.commit()
+        return o
+
     def sampler(self, filter=None, gradient_filter=None, max_sampling_depth=None, binned=None):
         """binned: None = automatic, False / True = \"binnedSampling\" 0 / 1 (spatially
         binned stream sampling of dense volumes, linear filter)."""
@@ -439,7 +456,40 @@ class Volume(Handle):
         return s
 
 
+class Observer(Handle):
+    """vklMapObserver / vklUnmapObserver and the element queries."""
+
+    def map(self):
+        """Maps, copies the elements out and unmaps: uint32 array for
+        LeafNodeAccess, float32 [n, elementSize / 4] for InnerNode."""
+        L = lib()
+        p = L.vklMapObserver(self.h)
+        self.device.check()
+        try:
+            t, es, n = (L.vklGetObserverElementType(self.h), L.vklGetObserverElementSize(self.h),
+                        L.vklGetObserverNumElements(self.h))
+            self.device.check()
+            if t == VKL_UINT:
+                return np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_uint32)), shape=(n,)).copy() if n else \
+                    np.zeros(0, np.uint32)
+            nf = es // 4
+            if not n:
+                return np.zeros((0, nf), np.float32)
+            return np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_float)), shape=(n * nf,)).reshape(n, nf).copy()
+        finally:
+            L.vklUnmapObserver(self.h)
+
+    def element_type(self):
+        return int(lib().vklGetObserverElementType(self.h))
+
+
 class Sampler(Handle):
+    def observer(self, type_name="LeafNodeAccess"):
+        """vklNewSamplerObserver ("LeafNodeAccess" on samplers of vdb volumes)."""
+        h = lib().vklNewSamplerObserver(self.h, type_name.encode())
+        self.device.check()
+        return Observer(self.device, h)
+
     def _out(self, like, shape, dtype=np.float32):
         if isinstance(like, np.ndarray):
             return np.empty(shape, dtype)

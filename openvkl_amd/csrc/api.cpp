@@ -48,10 +48,20 @@ void Sampler::commit()
   view.filter                = f;
   view.gradientFilter        = gf;
   binnedMode                 = getInt("binnedSampling", -1);
+  applyObservation();
+  committed = true;
+}
+
+void Sampler::applyObservation()
+{
+  const bool observed    = liveObservers > 0 && accessBuf.ptr;
+  view.grid.leafAccess   = observed ? accessBuf.as<uint32_t>() : nullptr;
+  view.grid.accessStamp  = accessEpoch;
+  view.grid.packedDepth  = observed ? 0xffffffffu : volume->grid.packedDepth;
+  viewVersion++;
   device->makeCurrent();
   if (!deviceView.ptr) deviceView.allocate(sizeof(SamplerView));
   deviceView.upload(&view, sizeof(view), device->stream);
-  committed = true;
 }
 
 const AttrView &Sampler::attr(uint32_t a) const
@@ -69,6 +79,15 @@ IteratorContext::IteratorContext(Device *d, Sampler *s, bool hit) : ManagedObjec
 }
 
 IteratorContext::~IteratorContext() { sampler->release(); }
+
+void IteratorContext::refreshSampler()
+{
+  if (samplerVersion == sampler->viewVersion) return;
+  view.sampler   = sampler->view;
+  samplerVersion = sampler->viewVersion;
+  device->makeCurrent();
+  deviceView.upload(&view, sizeof(IteratorContextView), device->stream);
+}
 
 namespace {
 // mapToMaxIteratorDepth for VDB volumes, IteratorContext.cpp:49-174.
@@ -95,6 +114,7 @@ void IteratorContext::commit()
   if (attrIndex < 0 || (uint32_t)attrIndex >= sampler->volume->numAttributes())
     throw VklError("illegal attributeIndex requested on volume");
   view.sampler        = sampler->view;
+  samplerVersion      = sampler->viewVersion;
   view.attributeIndex = (uint32_t)attrIndex;
   std::vector<Range1f> ranges;
   std::vector<float> values;
@@ -471,6 +491,7 @@ IteratorContext *contextOf(const VKLObject *c, bool hit)
   IteratorContext *ctx = static_cast<IteratorContext *>(c->host);
   if (ctx->isHit != hit) throw VklError("wrong iterator context type");
   if (!ctx->committed) throw VklError("iterator context is not committed");
+  ctx->refreshSampler();
   return ctx;
 }
 
@@ -496,6 +517,7 @@ int iterateInterval(IntervalBuffer *buf, VKLInterval *out)
 {
   if (buf->magic != kIntervalMagic) throw VklError("invalid interval iterator");
   IteratorContext *ctx = buf->ctx;
+  ctx->refreshSampler();
   Device *d            = ctx->device;
   d->makeCurrent();
   uint8_t *s = stepScratch(d);
@@ -538,6 +560,7 @@ int iterateHit(HitBuffer *buf, VKLHit *out)
 {
   if (buf->magic != kHitMagic) throw VklError("invalid hit iterator");
   IteratorContext *ctx = buf->ctx;
+  ctx->refreshSampler();
   Device *d            = ctx->device;
   d->makeCurrent();
   uint8_t *s = stepScratch(d);
@@ -818,6 +841,76 @@ OPENVKL_INTERFACE VKLSampler vklNewSampler(VKLVolume v)
 }
 
 OPENVKL_INTERFACE VKLFeatureFlags vklGetFeatureFlags(VKLSampler) { return (VKLFeatureFlags)VKL_FEATURE_FLAGS_DEFAULT; }
+
+// ---- observers (openvkl/api/API.cpp:126-186) -----------------------------------------
+
+OPENVKL_INTERFACE VKLObserver vklNewVolumeObserver(VKLVolume v, const char *type)
+{
+  VKL_TRY(deviceOf(v))
+  if (!type) throw VklError("null type provided to vklNewVolumeObserver");
+  Volume *vol = as<Volume>(v, "volume");
+  Observer *o = nullptr;
+  if (dynamic_cast<VdbVolume *>(vol) && std::string(type) == "InnerNode") {
+    if (!vol->committed) throw VklError("Trying to create an observer on a vdb volume that was not committed.");
+    o = new InnerNodeObserver(dev_, vol);
+  }
+  if (!o) throw VklError(std::string("unsupported observer type: ") + type);
+  VKLObserver h;
+  h.host   = o;
+  h.device = nullptr;
+  return h;
+  VKL_CATCH(VKLObserver())
+}
+
+OPENVKL_INTERFACE VKLObserver vklNewSamplerObserver(VKLSampler s, const char *type)
+{
+  VKL_TRY(deviceOf(s))
+  if (!type) throw VklError("null type provided to vklNewSamplerObserver");
+  Sampler *sp = as<Sampler>(s, "sampler");
+  Observer *o = nullptr;
+  if (dynamic_cast<VdbVolume *>(sp->volume) && std::string(type) == "LeafNodeAccess") o = new LeafAccessObserver(dev_, sp);
+  if (!o) throw VklError(std::string("unsupported observer type: ") + type);
+  VKLObserver h;
+  h.host   = o;
+  h.device = nullptr;
+  return h;
+  VKL_CATCH(VKLObserver())
+}
+
+OPENVKL_INTERFACE const void *vklMapObserver(VKLObserver o)
+{
+  VKL_TRY(deviceOf(o))
+  return as<Observer>(o, "observer")->map();
+  VKL_CATCH(nullptr)
+}
+
+OPENVKL_INTERFACE void vklUnmapObserver(VKLObserver o)
+{
+  VKL_TRY(deviceOf(o))
+  as<Observer>(o, "observer")->unmap();
+  VKL_CATCH()
+}
+
+OPENVKL_INTERFACE VKLDataType vklGetObserverElementType(VKLObserver o)
+{
+  VKL_TRY(deviceOf(o))
+  return as<Observer>(o, "observer")->elementType();
+  VKL_CATCH(VKL_UNKNOWN)
+}
+
+OPENVKL_INTERFACE size_t vklGetObserverElementSize(VKLObserver o)
+{
+  VKL_TRY(deviceOf(o))
+  return as<Observer>(o, "observer")->elementSize();
+  VKL_CATCH(0)
+}
+
+OPENVKL_INTERFACE size_t vklGetObserverNumElements(VKLObserver o)
+{
+  VKL_TRY(deviceOf(o))
+  return as<Observer>(o, "observer")->numElements();
+  VKL_CATCH(0)
+}
 
 OPENVKL_INTERFACE VKLIntervalIteratorContext vklNewIntervalIteratorContext(VKLSampler s)
 {

@@ -47,15 +47,18 @@ hipError_t launchSampleBatch(const BatchGroup *groups, uint32_t numGroups, const
                              hipStream_t stream);
 
 // Specialisation key of (grid, attribute, filter): batches with equal keys
-// run through the same kernel instantiation.
+// run through the same kernel instantiation (dense offsets 32 / 64 bit,
+// sparse observed or not).
 inline uint32_t sourceKey(const GridView &g, const AttrView &a, int filter)
 {
-  uint32_t big = 0;
+  uint32_t variant = 0;
   if (g.dense) {
     const uint64_t nbz = ((uint64_t)g.activeSize[2] + (1u << g.layout.sz) - 1) >> g.layout.sz;
-    big                = nbz * g.layout.strideZ >= (1ull << 32) ? 1u : 0u;
+    variant            = nbz * g.layout.strideZ >= (1ull << 32) ? 1u : 0u;
+  } else {
+    variant = g.leafAccess ? 1u : 0u;
   }
-  return (a.type << 12) | ((uint32_t)filter << 2) | (g.dense ? 2u : 0u) | big;
+  return (a.type << 12) | ((uint32_t)filter << 2) | (g.dense ? 2u : 0u) | variant;
 }
 
 }  // namespace vklhip

@@ -1,6 +1,7 @@
 // Host-side selection of a kernel specialisation: voxel source type (dense
 // element type and 32- / 64-bit offsets, or sparse vdb float / half) and
-// filter. `f` is a functor with
+// filter; sparse sources of observed samplers get their own instantiation.
+// `f` is a functor with
 //   template <typename Src, int kFilter> hipError_t operator()() const;
 // that launches the matching instantiation.
 #pragma once
@@ -39,8 +40,11 @@ hipError_t dispatchSource(const GridView &g, const AttrView &a, int filter, cons
     default: return hipErrorInvalidValue;
     }
   }
-  if (a.type == kTypeHalf) return withFilter<SparseSource<__half>>(filter, f);
-  if (a.type == kTypeFloat) return withFilter<SparseSource<float>>(filter, f);
+  const bool obs = g.leafAccess != nullptr;
+  if (a.type == kTypeHalf)
+    return obs ? withFilter<SparseSource<__half, true>>(filter, f) : withFilter<SparseSource<__half>>(filter, f);
+  if (a.type == kTypeFloat)
+    return obs ? withFilter<SparseSource<float, true>>(filter, f) : withFilter<SparseSource<float>>(filter, f);
   return hipErrorInvalidValue;
 }
 

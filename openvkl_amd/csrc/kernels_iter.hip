@@ -36,11 +36,12 @@ __device__ Range1f nodeRange(const GridView &g, const AttrView &a, uint32_t attr
     return r;
   }
   uint64_t voxel = 0;
-  uint32_t el = 0, eo = 0;
+  const Range1f *er = nullptr;
+  uint32_t eo       = 0;
   if (inDom) {
     SparseSource<float> src(g, a, attr);
     src.maxDepth = 3;
-    voxel        = src.traverse(ox & ~7u, oy & ~7u, oz & ~7u, el, eo);
+    voxel        = src.traverse(ox & ~7u, oy & ~7u, oz & ~7u, er, eo);
   }
   if ((voxel & 3u) != 3u) {
     if (!inDom) return {a.background, a.background};

@@ -178,8 +178,13 @@ hipError_t dispatch(const SamplerView &s, uint32_t attr, const AttrView &a, uint
     default: return hipErrorInvalidValue;
     }
   }
-  if (a.type == kTypeHalf) return dispatchFilter<SparseSource<__half>>(s, a, attr, n, xyz, out, stride, st, gradient);
-  if (a.type == kTypeFloat) return dispatchFilter<SparseSource<float>>(s, a, attr, n, xyz, out, stride, st, gradient);
+  const bool obs = s.grid.leafAccess != nullptr;
+  if (a.type == kTypeHalf)
+    return obs ? dispatchFilter<SparseSource<__half, true>>(s, a, attr, n, xyz, out, stride, st, gradient)
+               : dispatchFilter<SparseSource<__half>>(s, a, attr, n, xyz, out, stride, st, gradient);
+  if (a.type == kTypeFloat)
+    return obs ? dispatchFilter<SparseSource<float, true>>(s, a, attr, n, xyz, out, stride, st, gradient)
+               : dispatchFilter<SparseSource<float>>(s, a, attr, n, xyz, out, stride, st, gradient);
   return hipErrorInvalidValue;
 }
 
@@ -215,8 +220,13 @@ hipError_t launchRayMarch(const SamplerView &s, const AttrView &a, uint32_t attr
     default: return hipErrorInvalidValue;
     }
   }
-  if (a.type == kTypeHalf) return marchFilter<SparseSource<__half>>(s, a, attr, m, stream);
-  if (a.type == kTypeFloat) return marchFilter<SparseSource<float>>(s, a, attr, m, stream);
+  const bool obs = s.grid.leafAccess != nullptr;
+  if (a.type == kTypeHalf)
+    return obs ? marchFilter<SparseSource<__half, true>>(s, a, attr, m, stream)
+               : marchFilter<SparseSource<__half>>(s, a, attr, m, stream);
+  if (a.type == kTypeFloat)
+    return obs ? marchFilter<SparseSource<float, true>>(s, a, attr, m, stream)
+               : marchFilter<SparseSource<float>>(s, a, attr, m, stream);
   return hipErrorInvalidValue;
 }
 

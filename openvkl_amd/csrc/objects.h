@@ -30,6 +30,9 @@ struct VklError : std::runtime_error
 };
 
 void checkHip(hipError_t e, const char *what);
+// Object-space bounds of an index-space box: the 8 corners through i2o
+// (row-major 3x4), component-wise min / max.
+void transformBounds(const float i2o[12], const float box[6], float out[6]);
 size_t sizeOfType(VKLDataType t);
 bool isDevicePointer(const void *p);
 
@@ -175,12 +178,25 @@ struct Sampler : ManagedObject
   int binnedMode = -1;
   std::mutex binMutex;
   DeviceBuffer binBuf;
+  // LeafNodeAccess observation: one uint32 per leaf index holding the stamp
+  // of the last traversal that ended on that leaf. accessEpoch counts the
+  // observers ever created (each observer keeps its own epoch and reads an
+  // entry as accessed when its stamp is at least that epoch); traversals
+  // store the current epoch while any observer is alive. The buffer lives as
+  // long as the sampler, so views copied into plans stay valid.
+  DeviceBuffer accessBuf;
+  uint32_t accessEpoch = 0;
+  uint32_t liveObservers = 0;
+  uint64_t viewVersion   = 0;  // bumped whenever view changes
 
   Sampler(Device *d, Volume *v);
   ~Sampler() override;
   void commit() override;
   std::string toString() const override { return "openvkl::Sampler"; }
   const AttrView &attr(uint32_t a) const;
+  // Applies the observation state to view (observed samplers traverse the
+  // node table: corner-packed cell records bypass it) and uploads deviceView.
+  void applyObservation();
 };
 
 struct IteratorContext : ManagedObject
@@ -189,13 +205,62 @@ struct IteratorContext : ManagedObject
   IteratorContextView view{};
   DeviceBuffer rangesBuf, valuesBuf;
   DeviceBuffer deviceView;  // copy of `view` in device memory (hit kernels)
+  uint64_t samplerVersion = 0;  // Sampler::viewVersion that view.sampler copies
   bool isHit;
   bool committed = false;
 
   IteratorContext(Device *d, Sampler *s, bool hit);
   ~IteratorContext() override;
   void commit() override;
+  // Picks up sampler state changed since commit (re-commit, observers).
+  void refreshSampler();
   std::string toString() const override { return isHit ? "openvkl::HitIteratorContext" : "openvkl::IntervalIteratorContext"; }
+};
+
+// vklNew*Observer (openvkl/include/openvkl/observer.h:21-75). Observers
+// retain their target.
+struct Observer : ManagedObject
+{
+  using ManagedObject::ManagedObject;
+  virtual const void *map()           = 0;
+  virtual void unmap() {}
+  virtual VKLDataType elementType() const = 0;
+  virtual size_t elementSize() const      = 0;
+  virtual size_t numElements() const      = 0;
+  std::string toString() const override { return "openvkl::Observer"; }
+};
+
+// "InnerNode" on vdb volumes (VdbInnerNodeObserver.cpp:96-186): built on
+// commit, or on the first map after creation.
+struct InnerNodeObserver : Observer
+{
+  Volume *volume;
+  std::vector<float> nodes;
+  uint32_t numFloats = 0;
+  bool built         = false;
+
+  InnerNodeObserver(Device *d, Volume *v);
+  ~InnerNodeObserver() override;
+  void commit() override;
+  const void *map() override;
+  VKLDataType elementType() const override { return VKL_OBJECT; }
+  size_t elementSize() const override { return sizeof(float) * numFloats; }
+  size_t numElements() const override { return numFloats ? nodes.size() / numFloats : 0; }
+};
+
+// "LeafNodeAccess" on samplers of vdb volumes (VdbLeafAccessObserver.cpp).
+struct LeafAccessObserver : Observer
+{
+  Sampler *sampler;
+  uint32_t epoch;
+  std::vector<uint32_t> snapshot;
+
+  LeafAccessObserver(Device *d, Sampler *s);
+  ~LeafAccessObserver() override;
+  const void *map() override;
+  VKLDataType elementType() const override { return VKL_UINT; }
+  size_t elementSize() const override { return sizeof(uint32_t); }
+  size_t numElements() const override { return snapshot.size(); }
 };
 
 struct Device

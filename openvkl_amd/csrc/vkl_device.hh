@@ -204,8 +204,12 @@ struct DenseSource
 
 // Sparse VDB node table (VdbSampler_traverseAndSample.ih:85-202,
 // VdbSampleInner.ih.in:33-77). Leaf data is packed: dense leaves of 512
-// z-fastest voxels, tiles of one value.
-template <typename T>
+// z-fastest voxels, tiles of one value. kObserve: the sampler has a
+// LeafNodeAccess observer; traversals record their leaf and the corner-packed
+// cell records (which bypass traversal) are not used. A separate
+// instantiation keeps the unobserved kernels free of the marking code, which
+// costs the hit iterator about 27 VGPRs.
+template <typename T, bool kObserve = false>
 struct SparseSource
 {
   LevelView lv[3];
@@ -214,21 +218,24 @@ struct SparseSource
   const T *tile;
   const float *cells;         // corner-packed cell records (or null)
   const uint32_t *leafIndex;  // leaf cell -> dense-leaf slot (corner-packed)
+  uint32_t *leafAccess;       // observed samplers: per leaf index, last access stamp
   int32_t root[3];
   uint32_t size[3];
   uint32_t ldim[3];
-  uint32_t maxDepth, numAttr, attr;
+  uint32_t maxDepth, numAttr, attr, accessStamp;
   float bg;
 
   VKL_DEV SparseSource(const GridView &g, const AttrView &a, uint32_t attributeIndex)
       : leafSlot(g.leafSlot),
         dense((const T *)a.data),
         tile((const T *)a.tile),
-        cells(g.maxSamplingDepth == g.packedDepth ? a.cells : nullptr),
+        cells(!kObserve && g.maxSamplingDepth == g.packedDepth ? a.cells : nullptr),
         leafIndex(g.leafIndex),
+        leafAccess(g.leafAccess),
         maxDepth(g.maxSamplingDepth),
         numAttr(g.numAttributes),
         attr(attributeIndex),
+        accessStamp(g.accessStamp),
         bg(a.background)
   {
 #pragma unroll
@@ -241,34 +248,48 @@ struct SparseSource
   }
 
   // Voxel code for an in-domain root-relative offset; error voxels report
-  // their level and voxel offset (VdbSampleInner.ih.in:46-54).
-  VKL_DEV uint64_t traverse(uint32_t ox, uint32_t oy, uint32_t oz, uint32_t &errLevel, uint32_t &errOffset) const
+  // their level's value ranges and voxel offset (VdbSampleInner.ih.in:46-54).
+  // The three levels are written out: a loop over lv[l] that the compiler
+  // does not unroll indexes lv at run time, which places the whole source in
+  // scratch memory.
+  VKL_DEV uint64_t traverse(uint32_t ox, uint32_t oy, uint32_t oz, const Range1f *&errRanges, uint32_t &errOffset) const
   {
-    uint64_t node = 0;
-#pragma unroll
-    for (uint32_t l = 0; l < 3; ++l) {
-      const uint32_t vo  = (uint32_t)(node * kNumVoxels[l]) + offsetToLinear(l, ox, oy, oz);
-      const uint64_t v   = lv[l].voxels[vo];
-      const uint32_t tag = (uint32_t)v & 3u;
-      if (tag >= 2u && l + 1 > maxDepth) {
-        errLevel  = l;
-        errOffset = vo;
-        return 1u;
-      }
-      if (tag == 2u && l < 2) {
-        node = v >> 2;
-        continue;
-      }
-      return v;
-    }
-    return 0;
+    const uint32_t vo0 = (uint32_t)offsetToLinear(0, ox, oy, oz);
+    const uint64_t v0  = lv[0].voxels[vo0];
+    if (((uint32_t)v0 & 3u) >= 2u && maxDepth < 1u) return stop(lv[0].valueRange, vo0, errRanges, errOffset);
+    if (((uint32_t)v0 & 3u) != 2u) return found(v0);
+    const uint32_t vo1 = (uint32_t)((v0 >> 2) * kNumVoxels[1]) + offsetToLinear(1, ox, oy, oz);
+    const uint64_t v1  = lv[1].voxels[vo1];
+    if (((uint32_t)v1 & 3u) >= 2u && maxDepth < 2u) return stop(lv[1].valueRange, vo1, errRanges, errOffset);
+    if (((uint32_t)v1 & 3u) != 2u) return found(v1);
+    const uint32_t vo2 = (uint32_t)((v1 >> 2) * kNumVoxels[2]) + offsetToLinear(2, ox, oy, oz);
+    const uint64_t v2  = lv[2].voxels[vo2];
+    if (((uint32_t)v2 & 3u) >= 2u && maxDepth < 3u) return stop(lv[2].valueRange, vo2, errRanges, errOffset);
+    return found(v2);
+  }
+
+  // Error voxel: the stopping level's value ranges and the voxel offset.
+  VKL_DEV static uint64_t stop(const Range1f *ranges, uint32_t vo, const Range1f *&errRanges, uint32_t &errOffset)
+  {
+    errRanges = ranges;
+    errOffset = vo;
+    return 1u;
+  }
+
+  // Leaf access observation (VdbSampleInner.ih.in:69-74): every traversal
+  // that ends on a leaf pointer records the leaf index. Concurrent lanes
+  // store the same stamp, so the races are benign.
+  VKL_DEV uint64_t found(uint64_t v) const
+  {
+    if (kObserve && ((uint32_t)v & 3u) == 3u) leafAccess[v >> 6] = accessStamp;
+    return v;
   }
 
   VKL_DEV uint64_t slotOf(uint64_t v) const { return leafSlot ? leafSlot[v >> 6] : (v >> 6); }
 
-  VKL_DEV float errorValue(uint32_t el, uint32_t eo) const
+  VKL_DEV float errorValue(const Range1f *er, uint32_t eo) const
   {
-    const Range1f r = lv[el].valueRange[(uint64_t)eo * numAttr + attr];
+    const Range1f r = er[(uint64_t)eo * numAttr + attr];
     return 0.5f * (r.lower + r.upper);
   }
 
@@ -289,11 +310,12 @@ struct SparseSource
   {
     const uint32_t ox = (uint32_t)(x - root[0]), oy = (uint32_t)(y - root[1]), oz = (uint32_t)(z - root[2]);
     if (!inDomain(ox, oy, oz)) return bg;
-    uint32_t el = 0, eo = 0;
-    const uint64_t v   = traverse(ox, oy, oz, el, eo);
+    const Range1f *er  = nullptr;
+    uint32_t eo        = 0;
+    const uint64_t v   = traverse(ox, oy, oz, er, eo);
     const uint32_t tag = (uint32_t)v & 3u;
     if (tag == 3u) return leafValue(v, ox, oy, oz);
-    if (tag == 1u) return errorValue(el, eo);
+    if (tag == 1u) return errorValue(er, eo);
     return 0.f;
   }
 
@@ -316,28 +338,37 @@ struct SparseSource
     }
     if (inDomain(ox, oy, oz) && inDomain(ox + 1, oy + 1, oz + 1) && (ox & 7u) != 7u && (oy & 7u) != 7u &&
         (oz & 7u) != 7u) {
-      uint32_t el = 0, eo = 0;
-      const uint64_t v   = traverse(ox, oy, oz, el, eo);
-      const uint32_t tag = (uint32_t)v & 3u;
-      if (tag == 3u && ((v >> 2) & 3u) == 1u) {
-        const T *p = dense + slotOf(v) * 512u + (((ox & 7u) << 6) | ((oy & 7u) << 3) | (oz & 7u));
-        s[0] = toFloat(p[0]);
-        s[1] = toFloat(p[1]);
-        s[2] = toFloat(p[8]);
-        s[3] = toFloat(p[9]);
-        s[4] = toFloat(p[64]);
-        s[5] = toFloat(p[65]);
-        s[6] = toFloat(p[72]);
-        s[7] = toFloat(p[73]);
-        return;
-      }
-      const float c = (tag == 3u) ? toFloat(tile[slotOf(v)]) : (tag == 1u) ? errorValue(el, eo) : 0.f;
-#pragma unroll
-      for (int i = 0; i < 8; i++) s[i] = c;
+      const Range1f *er = nullptr;
+      uint32_t eo       = 0;
+      const uint64_t v  = traverse(ox, oy, oz, er, eo);
+      leafCorners(v, er, eo, ox, oy, oz, s);
       return;
     }
 #pragma unroll
     for (int i = 0; i < 8; i++) s[i] = at(ic.x + ((i >> 2) & 1), ic.y + ((i >> 1) & 1), ic.z + (i & 1));
+  }
+
+  // The 8 corners of a stencil inside one 8^3 leaf cell, given that cell's
+  // traversal result.
+  VKL_DEV void leafCorners(uint64_t v, const Range1f *er, uint32_t eo, uint32_t ox, uint32_t oy, uint32_t oz,
+                           float *s) const
+  {
+    const uint32_t tag = (uint32_t)v & 3u;
+    if (tag == 3u && ((v >> 2) & 3u) == 1u) {
+      const T *p = dense + slotOf(v) * 512u + (((ox & 7u) << 6) | ((oy & 7u) << 3) | (oz & 7u));
+      s[0] = toFloat(p[0]);
+      s[1] = toFloat(p[1]);
+      s[2] = toFloat(p[8]);
+      s[3] = toFloat(p[9]);
+      s[4] = toFloat(p[64]);
+      s[5] = toFloat(p[65]);
+      s[6] = toFloat(p[72]);
+      s[7] = toFloat(p[73]);
+      return;
+    }
+    const float c = (tag == 3u) ? toFloat(tile[slotOf(v)]) : (tag == 1u) ? errorValue(er, eo) : 0.f;
+#pragma unroll
+    for (int i = 0; i < 8; i++) s[i] = c;
   }
 };
 

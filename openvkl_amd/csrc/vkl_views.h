@@ -94,6 +94,10 @@ struct GridView
   const uint32_t *leafIndex;
   uint32_t leafDims[3];
   uint32_t packedDepth;
+  // vdb samplers with a LeafNodeAccess observer: traversals ending on a leaf
+  // pointer store accessStamp at leafAccess[leaf index] (null: unobserved).
+  uint32_t *leafAccess;
+  uint32_t accessStamp;
 };
 
 // Sampler = grid + filters.

@@ -127,20 +127,6 @@ void buildTransforms(const float c[12], float i2o[12], float o2i[12])
   o2i[11] = -lz;
 }
 
-void transformBounds(const float i2o[12], const float box[6], float out[6])
-{
-  out[0] = out[1] = out[2] = kInf;
-  out[3] = out[4] = out[5] = -kInf;
-  for (int i = 0; i < 8; i++) {
-    const float v[3] = {(i & 1) ? box[3] : box[0], (i & 2) ? box[4] : box[1], (i & 4) ? box[5] : box[2]};
-    for (int r = 0; r < 3; r++) {
-      const float w = i2o[3 * r] * v[0] + i2o[3 * r + 1] * v[1] + i2o[3 * r + 2] * v[2] + i2o[9 + r];
-      out[r]        = std::min(out[r], w);
-      out[3 + r]    = std::max(out[3 + r], w);
-    }
-  }
-}
-
 float undefinedBackground()
 {
   const uint32_t bits = 0xFFC068B5u;
@@ -182,6 +168,20 @@ void rejectTemporal(const ManagedObject &o)
 }
 
 }  // namespace
+
+void transformBounds(const float i2o[12], const float box[6], float out[6])
+{
+  out[0] = out[1] = out[2] = kInf;
+  out[3] = out[4] = out[5] = -kInf;
+  for (int i = 0; i < 8; i++) {
+    const float v[3] = {(i & 1) ? box[3] : box[0], (i & 2) ? box[4] : box[1], (i & 4) ? box[5] : box[2]};
+    for (int r = 0; r < 3; r++) {
+      const float w = i2o[3 * r] * v[0] + i2o[3 * r + 1] * v[1] + i2o[3 * r + 2] * v[2] + i2o[9 + r];
+      out[r]        = std::min(out[r], w);
+      out[3 + r]    = std::max(out[3 + r], w);
+    }
+  }
+}
 
 DeviceBuffer *Volume::newBuffer(size_t bytes)
 {

@@ -71,6 +71,9 @@ def lib():
         L.oracle_iterate_hits.restype = u32
         L.oracle_iterate_hits.argtypes = [vp, u32, i32, i32, vp, vp, vp, vp, f32, vp, u32]
         L.oracle_tricubic_matrix.argtypes = [vp]
+        L.oracle_set_leaf_access.argtypes = [vp, vp]
+        L.oracle_inner_nodes.restype = u64
+        L.oracle_inner_nodes.argtypes = [vp, i32, vp, u64]
         L.oracle_half_to_float.restype = f32
         L.oracle_half_to_float.argtypes = [C.c_uint16]
         _lib = L
@@ -155,7 +158,30 @@ class Grid:
         keep += [dt, bg, cl]
         h = lib().oracle_vdb_new(n, _p(level), _p(origin), _p(fmt), na, _p(dt), packed, pdp, ptp, ldp,
                                  _p(m), _p(bg), _p(cl))
-        return cls(h, keep)
+        g = cls(h, keep)
+        g.num_leaves = n
+        return g
+
+    # -- observers ----------------------------------------------------------------
+    def observe_leaf_access(self, on=True):
+        """LeafNodeAccess: returns the uint32 array (one entry per input node)
+        that later sampling calls mark with 1; on=False stops marking."""
+        if not on:
+            lib().oracle_set_leaf_access(self.h, None)
+            self._access = None
+            return None
+        self._access = np.zeros(self.num_leaves, np.uint32)
+        lib().oracle_set_leaf_access(self.h, self._access.ctypes.data)
+        return self._access
+
+    def inner_nodes(self, max_depth=1):
+        """InnerNode observer: array [count, 6 + 2 * numAttributes] (object-
+        space box, then one value range per attribute)."""
+        count = lib().oracle_inner_nodes(self.h, int(max_depth), None, 0)
+        nf = 6 + 2 * lib().oracle_num_attributes(self.h)
+        out = np.empty((count, nf), np.float32)
+        lib().oracle_inner_nodes(self.h, int(max_depth), out.ctypes.data, count)
+        return out
 
     # -- sampler ------------------------------------------------------------------
     def set_sampler(self, filter=LINEAR, gradient_filter=None, max_sampling_depth=3):

@@ -262,6 +262,10 @@ struct OracleGrid
   int filter = kFilterLinear, gradientFilter = kFilterLinear;
   uint32_t maxSamplingDepth = 3;
   bool cubicDouble = false;
+
+  // LeafNodeAccess observation (VdbLeafAccessObserver.ih): when set, every
+  // sampling traversal that ends on a leaf pointer sets entry [leaf index].
+  uint32_t *leafAccess = nullptr;
 };
 
 namespace {
@@ -398,6 +402,9 @@ float sparseCorner(const OracleGrid &g, uint32_t attr, V3i ic)
   const V3u o = toDomainOffset(ic, g.rootOrigin);
   if (!isInDomain(g.activeSize, o)) return g.background[attr];
   const uint64_t v = traverse(g, o, g.maxSamplingDepth);
+  // VdbSampleInner.ih.in:69-74: a non-empty voxel that ends the traversal
+  // (error voxels return earlier, :48-54) marks its leaf index.
+  if (g.leafAccess && voxelIsLeaf(v)) __atomic_store_n(&g.leafAccess[leafIndexOf(v)], 1u, __ATOMIC_RELAXED);
   return sampleVoxel(g, v, o, attr);
 }
 
@@ -1385,6 +1392,57 @@ void oracle_gradient(const OracleGrid *g, uint64_t n, const float *xyz, float *o
 }
 
 void oracle_bounding_box(const OracleGrid *g, float out[6]) { std::memcpy(out, g->bbox, sizeof(g->bbox)); }
+
+void oracle_set_leaf_access(OracleGrid *g, uint32_t *buffer) { g->leafAccess = buffer; }
+
+// VdbInnerNodeObserver.cpp:19-42 (visitVoxels) and :96-186 (commit), in the
+// sequential order level, node, x, y, z. maxDepth is clamped to 2, the
+// deepest level with voxels.
+uint64_t oracle_inner_nodes(const OracleGrid *g, int maxDepth, float *out, uint64_t capacity)
+{
+  const uint32_t md   = std::min<uint32_t>((uint32_t)maxDepth, 2u);
+  const uint32_t nf   = 6 + 2 * g->numAttributes;
+  uint64_t count      = 0;
+  for (uint32_t l = 0; l <= md && !g->dense; ++l) {
+    const OracleLevel &L   = g->levels[l];
+    const uint32_t logRes  = kLogRes[l];
+    const uint32_t res     = 1u << logRes;
+    const uint32_t ext     = kRes[l + 1];
+    for (uint64_t n = 0; n < L.numNodes; ++n)
+      for (uint32_t vx = 0; vx < res; ++vx)
+        for (uint32_t vy = 0; vy < res; ++vy)
+          for (uint32_t vz = 0; vz < res; ++vz) {
+            const uint64_t vidx = (n << (3 * logRes)) + ((uint64_t)vx << (2 * logRes)) + ((uint64_t)vy << logRes) + vz;
+            const uint64_t voxel = L.voxels[vidx];
+            if (!(voxelIsLeaf(voxel) || (l == md && !voxelIsEmpty(voxel)))) continue;
+            if (out && count < capacity) {
+              float *node = out + count * nf;
+              const int32_t ox = g->rootOrigin.x + (int32_t)(L.origin[n].x + ext * vx);
+              const int32_t oy = g->rootOrigin.y + (int32_t)(L.origin[n].y + ext * vy);
+              const int32_t oz = g->rootOrigin.z + (int32_t)(L.origin[n].z + ext * vz);
+              const V3f lo{(float)ox, (float)oy, (float)oz};
+              const V3f hi{lo.x + (float)ext, lo.y + (float)ext, lo.z + (float)ext};
+              float b[6] = {kInf, kInf, kInf, -kInf, -kInf, -kInf};
+              for (int i = 0; i < 8; ++i) {
+                const V3f c = xfmPoint(g->i2o, V3f{(i & 1) ? hi.x : lo.x, (i & 2) ? hi.y : lo.y, (i & 4) ? hi.z : lo.z});
+                b[0] = fmin_(b[0], c.x);
+                b[1] = fmin_(b[1], c.y);
+                b[2] = fmin_(b[2], c.z);
+                b[3] = fmax_(b[3], c.x);
+                b[4] = fmax_(b[4], c.y);
+                b[5] = fmax_(b[5], c.z);
+              }
+              std::memcpy(node, b, sizeof(b));
+              for (uint32_t a = 0; a < g->numAttributes; ++a) {
+                node[6 + 2 * a] = L.valueRange[vidx * g->numAttributes + a].lower;
+                node[7 + 2 * a] = L.valueRange[vidx * g->numAttributes + a].upper;
+              }
+            }
+            ++count;
+          }
+  }
+  return count;
+}
 
 void oracle_value_range(const OracleGrid *g, uint32_t attr, float out[2])
 {

@@ -76,6 +76,13 @@ void oracle_grid_info(const OracleGrid *g, int32_t rootOrigin[3], uint32_t activ
 uint64_t oracle_level_num_nodes(const OracleGrid *g, int l);
 void oracle_level_copy(const OracleGrid *g, int l, uint32_t *origin, uint64_t *voxels, float *ranges);
 
+/* LeafNodeAccess: with buffer non-NULL (numLeaves entries), sampling sets
+ * buffer[leaf index] = 1 for every traversal that ends on a leaf pointer. */
+void oracle_set_leaf_access(OracleGrid *g, uint32_t *buffer);
+/* InnerNode observer at maxDepth: returns the node count and writes up to
+ * capacity (6 + 2 * numAttributes)-float tuples to out (may be NULL). */
+uint64_t oracle_inner_nodes(const OracleGrid *g, int maxDepth, float *out, uint64_t capacity);
+
 /* intervalResolutionHint -> iterator depth (IteratorContext.cpp:49-174). */
 int oracle_map_depth(float hint);
 
