@@ -9,7 +9,8 @@ import pytest
 import oracle as O
 import openvkl_amd as vkl
 from tests.test_gpu_dense import assert_same
-from tests.test_oracle_observers import mixed_node_arrays
+from tests.test_oracle_observers import (KA_CASES, check_inner_nodes_known_answer, known_answer_volume,
+                                         mixed_node_arrays)
 
 pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
@@ -205,3 +206,30 @@ def test_unsupported_observers(device):
     code, msg = device.last_error()
     device.errors.clear()
     assert not h.host and "unsupported observer type: LeafNodeAccess" in msg
+
+
+@pytest.mark.parametrize("origin,spacing", KA_CASES)
+@pytest.mark.parametrize("packed,nattr", [(True, 1), (True, 3), (False, 1)])
+def test_inner_nodes_reference_known_answer(device, origin, spacing, packed, nattr):
+    """vdb_volume_inner_node_observer.cpp at its own size (256^3), packed and
+    per-node input, one and three attributes."""
+    level, origins, fmt, dense, i2o = known_answer_volume(origin, spacing, nattr, dims=256)
+    i2o_l = [float(v) for v in i2o]
+    if packed:
+        vol = device.vdb(level, origins, fmt, packed_dense=dense, index_to_object=i2o_l)
+    else:
+        per_node = [[d[i * 512:(i + 1) * 512] for d in dense] for i in range(len(origins))]
+        vol = device.vdb(level, origins, fmt, leaf_data=per_node, index_to_object=i2o_l)
+    s = vol.sampler(filter=vkl.VKL_FILTER_NEAREST)
+
+    def inner(d):
+        o = vol.observer("InnerNode", max_depth=d)
+        try:
+            return o.map()
+        finally:
+            o.release()
+
+    def sample(p, a):
+        return s.sample(torch.from_numpy(p).cuda(), attr=a).cpu().numpy()
+
+    check_inner_nodes_known_answer(inner, sample, vol.bounding_box(), 256, nattr)

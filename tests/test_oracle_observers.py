@@ -9,10 +9,13 @@ LeafNodeAccess: the marked entries are exactly the input nodes that contain a
 filter-stencil voxel inside the domain whose node level is within
 maxSamplingDepth, found by brute force over the stencil voxels; packed input
 marks the per-format packed index instead of the input index."""
+import warnings
+
 import numpy as np
 import pytest
 
 import oracle as O
+from tests import volumes as V
 
 RES = {1: 4096, 2: 128, 3: 8}
 
@@ -201,3 +204,63 @@ def test_leaf_access_off_marks_nothing():
     g.observe_leaf_access(False)
     g.sample(np.random.default_rng(12).random((100, 3)).astype(np.float32) * 256)
     assert acc.sum() == 0
+
+
+# ---- the reference's own known answers -----------------------------------------------
+
+KA_CASES = [((0.0, 0.0, 0.0), (1.0, 1.0, 1.0)), ((100.0, 200.0, 300.0), (1.0, 2.0, 3.0))]
+KA_FIELDS = [V.xyz, V.wavelet, V.zramp]
+
+
+def known_answer_volume(origin, spacing, nattr, dims=256):
+    """testing/apps/tests/vdb_volume_inner_node_observer.cpp:138-168: a vdb
+    volume filled with the xyz field (and further fields for the
+    multi-attribute case) on a dims^3 grid with the given origin / spacing."""
+    origins, leaves = V.dense_vdb_leaves(dims, KA_FIELDS[:nattr], origin, spacing)
+    n = len(origins)
+    i2o = np.array([spacing[0], 0, 0, 0, spacing[1], 0, 0, 0, spacing[2], *origin], np.float32)
+    return (np.full(n, 3, np.uint32), origins, np.full(n, O.FORMAT_DENSE_ZYX, np.uint32),
+            [a.reshape(-1) for a in leaves], i2o)
+
+
+def check_inner_nodes_known_answer(inner_nodes, sample_nearest, bbox, dims, nattr, max_points=None):
+    """vdb_volume_inner_node_observer.cpp:13-134: for maxDepth 0..2 the node
+    count is ceil(dims / node resolution)^3, values sampled (nearest) on a
+    step-2 grid over each node (step 16 at depth 0) lie inside the node's
+    range for every attribute (NaN background samples are ignored, as
+    range1f::extend does), and the union of the node boxes contains the
+    volume's bounding box (equals it at depth 2)."""
+    for max_depth in range(3):
+        res = {0: 4096, 1: 128, 2: 8}[max_depth]
+        nodes = inner_nodes(max_depth)
+        assert nodes.shape == ((-(-dims // res)) ** 3, 6 + 2 * nattr)
+        step = 16 if max_depth == 0 else 2
+        k = np.arange(res // step, dtype=np.float32) * step / np.float32(res)
+        gx, gy, gz = np.meshgrid(k, k, k, indexing="ij")
+        frac = np.stack([gx.ravel(), gy.ravel(), gz.ravel()], 1)
+        if max_points and len(frac) > max_points:
+            frac = frac[np.random.default_rng(max_depth).choice(len(frac), max_points, replace=False)]
+        lo, hi = nodes[:, None, :3], nodes[:, None, 3:6]
+        pts = (lo + frac[None] * (hi - lo)).astype(np.float32)
+        for a in range(nattr):
+            s = sample_nearest(pts.reshape(-1, 3), a).reshape(len(nodes), -1)
+            with warnings.catch_warnings():  # all-NaN rows: nodes outside the domain
+                warnings.simplefilter("ignore", RuntimeWarning)
+                smin, smax = np.nanmin(s, axis=1), np.nanmax(s, axis=1)
+            ok = np.isnan(smin) | ((smin >= nodes[:, 6 + 2 * a]) & (smax <= nodes[:, 7 + 2 * a]))
+            assert ok.all(), (max_depth, a)
+        ext = np.concatenate([nodes[:, :3].min(0), nodes[:, 3:6].max(0)])
+        if max_depth == 2:
+            np.testing.assert_array_equal(ext, bbox)
+        else:
+            assert np.all(bbox[:3] >= ext[:3]) and np.all(bbox[3:] <= ext[3:])
+
+
+@pytest.mark.parametrize("origin,spacing", KA_CASES)
+@pytest.mark.parametrize("nattr", [1, 3])
+def test_inner_nodes_reference_known_answer(origin, spacing, nattr):
+    level, origins, fmt, dense, i2o = known_answer_volume(origin, spacing, nattr, dims=128)
+    g = O.Grid.vdb(level, origins, fmt, packed_dense=dense, i2o=i2o)
+    g.set_sampler(O.NEAREST)
+    check_inner_nodes_known_answer(g.inner_nodes, lambda p, a: g.sample(p, attr=a, threads=8),
+                                   g.bounding_box(), 128, nattr, max_points=4096)

@@ -81,3 +81,25 @@ def sparse_shell_leaves(res=256, radius=0.35, width=0.06, seed=SEED + 2, fbm_amp
              + 0.5 * np.sin(px * 0.23 + phase[3]) * np.sin(py * 0.29 + phase[4]) * np.sin(pz * 0.31 + phase[5]))
     vals = shell * (1.0 + fbm_amp * noise)
     return origins, vals.astype(np.float32)
+
+
+def dense_vdb_leaves(dims, fns, origin=(0, 0, 0), spacing=(1, 1, 1)):
+    """Every 8^3 leaf of a dims^3 index domain (dims a multiple of 8), with
+    attribute a = fns[a](x, y, z) at the voxel object positions (binary32
+    coordinates). Returns (origins [n,3] int32, [leaves [n,512] float32 per
+    attribute]), leaf-local order x*64 + y*8 + z, leaves in x-slowest order
+    (testing/volume/ProceduralVdbVolume.h:213-259)."""
+    nl = dims // 8
+    k = np.arange(nl)
+    kx, ky, kz = np.meshgrid(k, k, k, indexing="ij")
+    origins = np.stack([kx.ravel(), ky.ravel(), kz.ravel()], 1).astype(np.int32) * 8
+    l = np.arange(8)
+    lx, ly, lz = np.meshgrid(l, l, l, indexing="ij")
+    ix = origins[:, 0:1] + lx.reshape(1, -1)
+    iy = origins[:, 1:2] + ly.reshape(1, -1)
+    iz = origins[:, 2:3] + lz.reshape(1, -1)
+    o, sp = np.asarray(origin, np.float32), np.asarray(spacing, np.float32)
+    x = (o[0] + ix.astype(np.float32) * sp[0]).astype(np.float64)
+    y = (o[1] + iy.astype(np.float32) * sp[1]).astype(np.float64)
+    z = (o[2] + iz.astype(np.float32) * sp[2]).astype(np.float64)
+    return origins, [fn(x, y, z).astype(np.float32) for fn in fns]
