@@ -23,7 +23,7 @@ LIB_PATH = os.path.join(HERE, "lib", "libopenvkl_hip.so")
 VKL_OBJECT, VKL_DATA, VKL_VOLUME = 0x8000000, 0x8000000 + 100, 0x8000000 + 101
 VKL_BOOL, VKL_STRING, VKL_VOID_PTR = 250, 1500, 200
 VKL_UCHAR, VKL_SHORT, VKL_USHORT, VKL_INT, VKL_UINT = 2500, 3000, 3500, 4000, 4500
-VKL_HALF, VKL_FLOAT, VKL_DOUBLE = 5800, 6000, 7000
+VKL_HALF, VKL_FLOAT, VKL_DOUBLE, VKL_ULONG = 5800, 6000, 7000, 5550
 VKL_VEC3I, VKL_VEC3F, VKL_BOX1F, VKL_BOX3I, VKL_AFFINE3F = 4002, 6002, 10000, 8002, 12003
 VKL_DATA_DEFAULT, VKL_DATA_SHARED_BUFFER = 0, 1
 VKL_FILTER_NEAREST, VKL_FILTER_LINEAR, VKL_FILTER_CUBIC = 0, 100, 200
@@ -274,12 +274,26 @@ class Device:
 
     def structured_regular(self, voxels, origin=(0, 0, 0), spacing=(1, 1, 1), filter=None, cell_centered=None,
                            background=None, index_origin=None, index_to_object=None, brick_shift=None,
-                           corner_packed=None):
-        """voxels: array [nz, ny, nx] (x fastest) or a list of them."""
+                           corner_packed=None, dims=None, structured_steps=None, tu_indices=None, tu_times=None):
+        """voxels: array [nz, ny, nx] (x fastest) or a list of them. Motion
+        blur: pass dims (nx, ny, nz) and 1-D voxels with structured_steps
+        values per voxel, or tu_indices / tu_times (temporallyUnstructured*;
+        uint32 or uint64 indices)."""
         arrs = voxels if isinstance(voxels, (list, tuple)) else [voxels]
         v = self.new_volume("structuredRegular")
-        nz, ny, nx = arrs[0].shape
+        nx, ny, nz = dims if dims is not None else arrs[0].shape[::-1]
         v.set_vec3i("dimensions", nx, ny, nz)
+        if structured_steps is not None:
+            v.set_int("temporallyStructuredNumTimesteps", structured_steps)
+        if tu_indices is not None:
+            ti = np.ascontiguousarray(tu_indices)
+            d = self.data(ti, VKL_ULONG if ti.dtype == np.uint64 else VKL_UINT)
+            v.set_data("temporallyUnstructuredIndices", d)
+            d.release()
+        if tu_times is not None:
+            d = self.data(np.ascontiguousarray(tu_times, np.float32))
+            v.set_data("temporallyUnstructuredTimes", d)
+            d.release()
         v.set_vec3f("gridOrigin", *origin)
         v.set_vec3f("gridSpacing", *spacing)
         datas = [self.data(a) if isinstance(a, np.ndarray) else self.data(a.reshape(-1)) for a in arrs]
@@ -497,11 +511,12 @@ class Sampler(Handle):
 
         return torch.empty(shape, dtype=torch.float32, device=like.device)
 
-    def sample(self, xyz, attr=0, out=None):
-        """vklComputeSampleN; xyz: [N,3] float32 numpy (host) or torch (device)."""
+    def sample(self, xyz, attr=0, out=None, times=None):
+        """vklComputeSampleN; xyz: [N,3] float32 numpy (host) or torch (device);
+        times: [N] float32 like xyz, or None (time 0)."""
         n = _n_points(xyz)
         out = self._out(xyz, (n,)) if out is None else out
-        lib().vklComputeSampleN(C.byref(self.h), n, _ptr(xyz), _ptr(out), attr, None)
+        lib().vklComputeSampleN(C.byref(self.h), n, _ptr(xyz), _ptr(out), attr, _ptr(times))
         self.device.check()
         return out
 
@@ -510,10 +525,10 @@ class Sampler(Handle):
         self.device.check()
         return out
 
-    def gradient(self, xyz, attr=0, out=None):
+    def gradient(self, xyz, attr=0, out=None, times=None):
         n = _n_points(xyz)
         out = self._out(xyz, (n, 3)) if out is None else out
-        lib().vklComputeGradientN(C.byref(self.h), n, _ptr(xyz), _ptr(out), attr, None)
+        lib().vklComputeGradientN(C.byref(self.h), n, _ptr(xyz), _ptr(out), attr, _ptr(times))
         self.device.check()
         return out
 
@@ -525,17 +540,17 @@ class Sampler(Handle):
         self.device.check()
         return out
 
-    def sample_m(self, xyz, attrs):
+    def sample_m(self, xyz, attrs, times=None):
         n = _n_points(xyz)
         at = np.ascontiguousarray(attrs, np.uint32)
         out = self._out(xyz, (n, at.size))
-        lib().vklComputeSampleMN(C.byref(self.h), n, _ptr(xyz), _ptr(out), at.size, at.ctypes.data, None)
+        lib().vklComputeSampleMN(C.byref(self.h), n, _ptr(xyz), _ptr(out), at.size, at.ctypes.data, _ptr(times))
         self.device.check()
         return out
 
-    def sample1(self, p, attr=0):
+    def sample1(self, p, attr=0, time=0.0):
         v = vec3f(*[float(x) for x in p])
-        r = lib().vklComputeSample(C.byref(self.h), C.byref(v), attr, 0.0)
+        r = lib().vklComputeSample(C.byref(self.h), C.byref(v), attr, float(time))
         self.device.check()
         return r
 
@@ -545,7 +560,7 @@ class Sampler(Handle):
         self.device.check()
         return np.array([r.x, r.y, r.z], np.float32)
 
-    def sample_wide(self, pts, attr=0, width=4, valid=None):
+    def sample_wide(self, pts, attr=0, width=4, valid=None, times=None):
         VV = WIDE[width][0]
         vv = VV()
         k = len(pts)
@@ -553,7 +568,11 @@ class Sampler(Handle):
             vv.x[i], vv.y[i], vv.z[i] = [float(c) for c in pts[i]]
         val = (C.c_int * width)(*([1] * k + [0] * (width - k))) if valid is None else (C.c_int * width)(*valid)
         out = (C.c_float * width)()
-        getattr(lib(), f"vklComputeSample{width}")(C.addressof(val), C.byref(self.h), C.byref(vv), C.addressof(out), attr, None)
+        tm = None
+        if times is not None:
+            tm = (C.c_float * width)(*([float(x) for x in times] + [0.0] * (width - len(times))))
+        getattr(lib(), f"vklComputeSample{width}")(C.addressof(val), C.byref(self.h), C.byref(vv), C.addressof(out),
+                                                   attr, None if tm is None else C.addressof(tm))
         self.device.check()
         return np.array(out[:k], np.float32)
 

@@ -284,7 +284,8 @@ int maxLdsPerBlock(int device)
 bool binnedGeometry(Sampler *s, const AttrView &a, uint64_t n, BinOp op, BinGeometry &geo)
 {
   const GridView &g = s->view.grid;
-  if (s->binnedMode == 0 || !g.dense) return false;
+  // Temporal volumes sample through the time-interpolating source only.
+  if (s->binnedMode == 0 || !g.dense || g.temporalFormat) return false;
   const uint32_t es = (uint32_t)sizeOfType((VKLDataType)a.type);
   if (s->binnedMode < 0) {
     // One record load per stencil already: the direct kernel is the better fit.
@@ -323,7 +324,8 @@ BinScratch binnedScratch(Sampler *s, const BinGeometry &geo, uint64_t n)
 }
 
 // One attribute into out[i * stride]: binned or direct.
-void sampleInto(Sampler *s, uint32_t attr, uint64_t n, const float *xyz, float *out, uint32_t stride)
+void sampleInto(Sampler *s, uint32_t attr, uint64_t n, const float *xyz, float *out, uint32_t stride,
+                const float *times)
 {
   Device *d         = s->device;
   const AttrView &a = s->attr(attr);
@@ -337,29 +339,35 @@ void sampleInto(Sampler *s, uint32_t attr, uint64_t n, const float *xyz, float *
     syncDevice(d);
     return;
   }
-  checkHip(launchSampleStrided(s->view, a, attr, n, xyz, out, stride, d->stream), "sample kernel");
+  checkHip(launchSampleStrided(s->view, a, attr, n, xyz, out, stride, d->stream, times), "sample kernel");
 }
 
-void sampleN(Sampler *s, uint64_t n, const vkl_vec3f *oc, float *out, uint32_t attr)
+// times: n floats (host or device) or null (time 0); only temporal volumes
+// read them.
+void sampleN(Sampler *s, uint64_t n, const vkl_vec3f *oc, float *out, uint32_t attr, const float *times)
 {
   Device *d = s->device;
   d->makeCurrent();
   (void)s->attr(attr);
   if (n == 0) return;
+  const bool temporal = s->view.grid.temporalFormat != 0 && times;
   Staged in(oc, n * 12, d->stream, true);
+  Staged tm(temporal ? times : nullptr, temporal ? n * 4 : 0, d->stream, true);
   Staged o(out, n * 4, d->stream, false);
-  sampleInto(s, attr, n, in.as<float>(), o.as<float>(), 1);
+  sampleInto(s, attr, n, in.as<float>(), o.as<float>(), 1, tm.as<float>());
   o.copyOut();
   syncDevice(d);
 }
 
-void gradientN(Sampler *s, uint64_t n, const vkl_vec3f *oc, vkl_vec3f *out, uint32_t attr)
+void gradientN(Sampler *s, uint64_t n, const vkl_vec3f *oc, vkl_vec3f *out, uint32_t attr, const float *times)
 {
   Device *d = s->device;
   d->makeCurrent();
   const AttrView &a = s->attr(attr);
   if (n == 0) return;
+  const bool temporal = s->view.grid.temporalFormat != 0 && times;
   Staged in(oc, n * 12, d->stream, true);
+  Staged tm(temporal ? times : nullptr, temporal ? n * 4 : 0, d->stream, true);
   Staged o(out, n * 12, d->stream, false);
   BinGeometry geo;
   const int gf   = s->view.gradientFilter;
@@ -370,23 +378,27 @@ void gradientN(Sampler *s, uint64_t n, const vkl_vec3f *oc, vkl_vec3f *out, uint
     checkHip(launchBinned(s->view, a, geo, op, n, in.as<float>(), o.as<float>(), 1, sc, d->stream),
              "binned gradient kernels");
   } else {
-    checkHip(launchGradient(s->view, a, attr, n, in.as<float>(), o.as<float>(), d->stream), "gradient kernel");
+    checkHip(launchGradient(s->view, a, attr, n, in.as<float>(), o.as<float>(), d->stream, tm.as<float>()),
+             "gradient kernel");
   }
   o.copyOut();
   syncDevice(d);
 }
 
-void sampleMN(Sampler *s, uint64_t n, const vkl_vec3f *oc, float *out, uint32_t m, const unsigned int *attrs)
+void sampleMN(Sampler *s, uint64_t n, const vkl_vec3f *oc, float *out, uint32_t m, const unsigned int *attrs,
+              const float *times)
 {
   Device *d = s->device;
   d->makeCurrent();
   if (n == 0 || m == 0) return;
   for (uint32_t k = 0; k < m; k++) (void)s->attr(attrs[k]);
+  const bool temporal = s->view.grid.temporalFormat != 0 && times;
   Staged in(oc, n * 12, d->stream, true);
+  Staged tm(temporal ? times : nullptr, temporal ? n * 4 : 0, d->stream, true);
   Staged o(out, n * m * 4, d->stream, false);
   // Sampler.h:226-244 layout samples[i * M + k]: one specialised launch per
   // attribute (attributes may differ in element type), strided by M.
-  for (uint32_t k = 0; k < m; k++) sampleInto(s, attrs[k], n, in.as<float>(), o.as<float>() + k, m);
+  for (uint32_t k = 0; k < m; k++) sampleInto(s, attrs[k], n, in.as<float>(), o.as<float>() + k, m, tm.as<float>());
   o.copyOut();
   syncDevice(d);
 }
@@ -404,39 +416,53 @@ std::vector<vkl_vec3f> gatherLanes(const int *valid, const VV *v, std::vector<in
   return p;
 }
 
+// Times of the valid lanes (empty when times is null).
+std::vector<float> laneTimes(const float *times, const std::vector<int> &lanes)
+{
+  std::vector<float> t;
+  if (times)
+    for (int l : lanes) t.push_back(times[l]);
+  return t;
+}
+
 template <int W, typename VV>
-void sampleW(const int *valid, const VKLSampler *sampler, const VV *oc, float *samples, unsigned attr)
+void sampleW(const int *valid, const VKLSampler *sampler, const VV *oc, float *samples, unsigned attr,
+             const float *times)
 {
   Sampler *s = samplerOf(sampler);
   std::vector<int> lanes;
   std::vector<vkl_vec3f> p = gatherLanes<W>(valid, oc, lanes);
   std::vector<float> r(p.size());
-  sampleN(s, p.size(), p.data(), r.data(), attr);
+  const std::vector<float> t = laneTimes(times, lanes);
+  sampleN(s, p.size(), p.data(), r.data(), attr, times ? t.data() : nullptr);
   for (size_t k = 0; k < lanes.size(); k++) samples[lanes[k]] = r[k];
 }
 
 template <int W, typename VV>
 void sampleMW(const int *valid, const VKLSampler *sampler, const VV *oc, float *samples, unsigned m,
-              const unsigned *attrs)
+              const unsigned *attrs, const float *times)
 {
   Sampler *s = samplerOf(sampler);
   std::vector<int> lanes;
   std::vector<vkl_vec3f> p = gatherLanes<W>(valid, oc, lanes);
   std::vector<float> r(p.size() * m);
-  sampleMN(s, p.size(), p.data(), r.data(), m, attrs);
+  const std::vector<float> t = laneTimes(times, lanes);
+  sampleMN(s, p.size(), p.data(), r.data(), m, attrs, times ? t.data() : nullptr);
   // Sampler.h:226-244: samples[a * W + lane] for the vector calls.
   for (size_t k = 0; k < lanes.size(); k++)
     for (unsigned a = 0; a < m; a++) samples[a * W + lanes[k]] = r[k * m + a];
 }
 
 template <int W, typename VV>
-void gradientW(const int *valid, const VKLSampler *sampler, const VV *oc, VV *out, unsigned attr)
+void gradientW(const int *valid, const VKLSampler *sampler, const VV *oc, VV *out, unsigned attr,
+               const float *times)
 {
   Sampler *s = samplerOf(sampler);
   std::vector<int> lanes;
   std::vector<vkl_vec3f> p = gatherLanes<W>(valid, oc, lanes);
   std::vector<vkl_vec3f> r(p.size());
-  gradientN(s, p.size(), p.data(), r.data(), attr);
+  const std::vector<float> t = laneTimes(times, lanes);
+  gradientN(s, p.size(), p.data(), r.data(), attr, times ? t.data() : nullptr);
   for (size_t k = 0; k < lanes.size(); k++) {
     out->x[lanes[k]] = r[k].x;
     out->y[lanes[k]] = r[k].y;
@@ -967,115 +993,115 @@ OPENVKL_INTERFACE void vklSetVoidPtr(VKLObject o, const char *name, void *v) { v
 
 // ---- sampling ----------------------------------------------------------------------------
 
-OPENVKL_INTERFACE float vklComputeSample(const VKLSampler *s, const vkl_vec3f *oc, unsigned int a, float)
+OPENVKL_INTERFACE float vklComputeSample(const VKLSampler *s, const vkl_vec3f *oc, unsigned int a, float time)
 {
   VKL_TRY(samplerOf(s)->device)
   float r = kNaN;
-  sampleN(samplerOf(s), 1, oc, &r, a);
+  sampleN(samplerOf(s), 1, oc, &r, a, &time);
   return r;
   VKL_CATCH(kNaN)
 }
 
 OPENVKL_INTERFACE void vklComputeSample4(const int *valid, const VKLSampler *s, const vkl_vvec3f4 *oc, float *out,
-                                         unsigned int a, const float *)
+                                         unsigned int a, const float *times)
 {
   VKL_TRY(samplerOf(s)->device)
-  sampleW<4>(valid, s, oc, out, a);
+  sampleW<4>(valid, s, oc, out, a, times);
   VKL_CATCH()
 }
 OPENVKL_INTERFACE void vklComputeSample8(const int *valid, const VKLSampler *s, const vkl_vvec3f8 *oc, float *out,
-                                         unsigned int a, const float *)
+                                         unsigned int a, const float *times)
 {
   VKL_TRY(samplerOf(s)->device)
-  sampleW<8>(valid, s, oc, out, a);
+  sampleW<8>(valid, s, oc, out, a, times);
   VKL_CATCH()
 }
 OPENVKL_INTERFACE void vklComputeSample16(const int *valid, const VKLSampler *s, const vkl_vvec3f16 *oc, float *out,
-                                          unsigned int a, const float *)
+                                          unsigned int a, const float *times)
 {
   VKL_TRY(samplerOf(s)->device)
-  sampleW<16>(valid, s, oc, out, a);
+  sampleW<16>(valid, s, oc, out, a, times);
   VKL_CATCH()
 }
 
 OPENVKL_INTERFACE void vklComputeSampleN(const VKLSampler *s, unsigned int N, const vkl_vec3f *oc, float *out,
-                                         unsigned int a, const float *)
+                                         unsigned int a, const float *times)
 {
   VKL_TRY(samplerOf(s)->device)
-  sampleN(samplerOf(s), N, oc, out, a);
+  sampleN(samplerOf(s), N, oc, out, a, times);
   VKL_CATCH()
 }
 
 OPENVKL_INTERFACE void vklComputeSampleM(const VKLSampler *s, const vkl_vec3f *oc, float *out, unsigned int M,
-                                         const unsigned int *attrs, float)
+                                         const unsigned int *attrs, float time)
 {
   VKL_TRY(samplerOf(s)->device)
-  sampleMN(samplerOf(s), 1, oc, out, M, attrs);
+  sampleMN(samplerOf(s), 1, oc, out, M, attrs, &time);
   VKL_CATCH()
 }
 OPENVKL_INTERFACE void vklComputeSampleM4(const int *valid, const VKLSampler *s, const vkl_vvec3f4 *oc, float *out,
-                                          unsigned int M, const unsigned int *attrs, const float *)
+                                          unsigned int M, const unsigned int *attrs, const float *times)
 {
   VKL_TRY(samplerOf(s)->device)
-  sampleMW<4>(valid, s, oc, out, M, attrs);
+  sampleMW<4>(valid, s, oc, out, M, attrs, times);
   VKL_CATCH()
 }
 OPENVKL_INTERFACE void vklComputeSampleM8(const int *valid, const VKLSampler *s, const vkl_vvec3f8 *oc, float *out,
-                                          unsigned int M, const unsigned int *attrs, const float *)
+                                          unsigned int M, const unsigned int *attrs, const float *times)
 {
   VKL_TRY(samplerOf(s)->device)
-  sampleMW<8>(valid, s, oc, out, M, attrs);
+  sampleMW<8>(valid, s, oc, out, M, attrs, times);
   VKL_CATCH()
 }
 OPENVKL_INTERFACE void vklComputeSampleM16(const int *valid, const VKLSampler *s, const vkl_vvec3f16 *oc, float *out,
-                                           unsigned int M, const unsigned int *attrs, const float *)
+                                           unsigned int M, const unsigned int *attrs, const float *times)
 {
   VKL_TRY(samplerOf(s)->device)
-  sampleMW<16>(valid, s, oc, out, M, attrs);
+  sampleMW<16>(valid, s, oc, out, M, attrs, times);
   VKL_CATCH()
 }
 OPENVKL_INTERFACE void vklComputeSampleMN(const VKLSampler *s, unsigned int N, const vkl_vec3f *oc, float *out,
-                                          unsigned int M, const unsigned int *attrs, const float *)
+                                          unsigned int M, const unsigned int *attrs, const float *times)
 {
   VKL_TRY(samplerOf(s)->device)
-  sampleMN(samplerOf(s), N, oc, out, M, attrs);
+  sampleMN(samplerOf(s), N, oc, out, M, attrs, times);
   VKL_CATCH()
 }
 
-OPENVKL_INTERFACE vkl_vec3f vklComputeGradient(const VKLSampler *s, const vkl_vec3f *oc, unsigned int a, float)
+OPENVKL_INTERFACE vkl_vec3f vklComputeGradient(const VKLSampler *s, const vkl_vec3f *oc, unsigned int a, float time)
 {
   vkl_vec3f r = {kNaN, kNaN, kNaN};
   VKL_TRY(samplerOf(s)->device)
-  gradientN(samplerOf(s), 1, oc, &r, a);
+  gradientN(samplerOf(s), 1, oc, &r, a, &time);
   return r;
   VKL_CATCH(r)
 }
 OPENVKL_INTERFACE void vklComputeGradient4(const int *valid, const VKLSampler *s, const vkl_vvec3f4 *oc,
-                                           vkl_vvec3f4 *g, unsigned int a, const float *)
+                                           vkl_vvec3f4 *g, unsigned int a, const float *times)
 {
   VKL_TRY(samplerOf(s)->device)
-  gradientW<4>(valid, s, oc, g, a);
+  gradientW<4>(valid, s, oc, g, a, times);
   VKL_CATCH()
 }
 OPENVKL_INTERFACE void vklComputeGradient8(const int *valid, const VKLSampler *s, const vkl_vvec3f8 *oc,
-                                           vkl_vvec3f8 *g, unsigned int a, const float *)
+                                           vkl_vvec3f8 *g, unsigned int a, const float *times)
 {
   VKL_TRY(samplerOf(s)->device)
-  gradientW<8>(valid, s, oc, g, a);
+  gradientW<8>(valid, s, oc, g, a, times);
   VKL_CATCH()
 }
 OPENVKL_INTERFACE void vklComputeGradient16(const int *valid, const VKLSampler *s, const vkl_vvec3f16 *oc,
-                                            vkl_vvec3f16 *g, unsigned int a, const float *)
+                                            vkl_vvec3f16 *g, unsigned int a, const float *times)
 {
   VKL_TRY(samplerOf(s)->device)
-  gradientW<16>(valid, s, oc, g, a);
+  gradientW<16>(valid, s, oc, g, a, times);
   VKL_CATCH()
 }
 OPENVKL_INTERFACE void vklComputeGradientN(const VKLSampler *s, unsigned int N, const vkl_vec3f *oc, vkl_vec3f *g,
-                                           unsigned int a, const float *)
+                                           unsigned int a, const float *times)
 {
   VKL_TRY(samplerOf(s)->device)
-  gradientN(samplerOf(s), N, oc, g, a);
+  gradientN(samplerOf(s), N, oc, g, a, times);
   VKL_CATCH()
 }
 

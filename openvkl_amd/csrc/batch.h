@@ -47,12 +47,14 @@ hipError_t launchSampleBatch(const BatchGroup *groups, uint32_t numGroups, const
                              hipStream_t stream);
 
 // Specialisation key of (grid, attribute, filter): batches with equal keys
-// run through the same kernel instantiation (dense offsets 32 / 64 bit,
-// sparse observed or not).
+// run through the same kernel instantiation (dense offsets 32 / 64 bit or
+// temporal, sparse observed or not).
 inline uint32_t sourceKey(const GridView &g, const AttrView &a, int filter)
 {
   uint32_t variant = 0;
-  if (g.dense) {
+  if (g.dense && g.temporalFormat) {
+    variant = 1u << 11;
+  } else if (g.dense) {
     const uint64_t nbz = ((uint64_t)g.activeSize[2] + (1u << g.layout.sz) - 1) >> g.layout.sz;
     variant            = nbz * g.layout.strideZ >= (1ull << 32) ? 1u : 0u;
   } else {

@@ -9,15 +9,16 @@ namespace vklhip {
 
 // vklComputeSampleN: out[i] = sample(xyz[i]) of attribute `attr`, whose view
 // (a host copy of s.grid.attrs[attr]) selects the kernel specialisation.
+// times (device, n floats) or null (time 0) for temporal volumes.
 hipError_t launchSample(const SamplerView &s, const AttrView &a, uint32_t attr, uint64_t n, const float *xyz,
-                        float *out, hipStream_t stream);
+                        float *out, hipStream_t stream, const float *times = nullptr);
 // vklComputeGradientN: out[3i..3i+2] = object-space gradient.
 hipError_t launchGradient(const SamplerView &s, const AttrView &a, uint32_t attr, uint64_t n, const float *xyz,
-                          float *out, hipStream_t stream);
+                          float *out, hipStream_t stream, const float *times = nullptr);
 // out[i * outStride] = sample(xyz[i]) of attribute `attr` (vklComputeSampleMN
 // issues one launch per requested attribute with outStride = M).
 hipError_t launchSampleStrided(const SamplerView &s, const AttrView &a, uint32_t attr, uint64_t n, const float *xyz,
-                               float *out, uint32_t outStride, hipStream_t stream);
+                               float *out, uint32_t outStride, hipStream_t stream, const float *times = nullptr);
 
 // Fixed-step ray marching fused with sampling: out[r] = sum over s < steps of
 // sample(org[r] + t_s dir[r]), t_s = t0 + (s + 0.5) (t1 - t0) / steps with
@@ -70,5 +71,12 @@ hipError_t launchIntervalBatch(const IteratorContextView &c, uint32_t numRays, c
 hipError_t launchHitBatch(const IteratorContextView &hostView, const IteratorContextView *c, const AttrView &a,
                           uint32_t numRays, const float *org, const float *dir, const float *tRanges,
                           const float *times, HitOut *out, uint32_t maxOut, uint32_t *count, hipStream_t stream);
+// The same for temporal structuredRegular and for vdb volumes (compiled in
+// kernels_hit_other.hip); launchHitNext / launchHitBatch forward to them.
+hipError_t launchHitNextOther(const IteratorContextView &hostView, const IteratorContextView *c, const AttrView &a,
+                              HitState *state, HitOut *out, int *result, hipStream_t stream);
+hipError_t launchHitBatchOther(const IteratorContextView &hostView, const IteratorContextView *c, const AttrView &a,
+                               uint32_t numRays, const float *org, const float *dir, const float *tRanges,
+                               const float *times, HitOut *out, uint32_t maxOut, uint32_t *count, hipStream_t stream);
 
 }  // namespace vklhip

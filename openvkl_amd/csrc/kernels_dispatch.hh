@@ -18,34 +18,63 @@ hipError_t withFilter(int filter, const F &f)
   return f.template operator()<Src, kFilterLinear>();
 }
 
-template <typename T, typename F>
+// Source kinds a dispatch instantiates kernels for; the hit kernels split
+// them over translation units that compile in parallel.
+enum : unsigned
+{
+  kDenseKinds    = 1u,
+  kSparseKinds   = 2u,
+  kTemporalKinds = 4u,
+  kAllKinds      = 7u
+};
+
+template <typename T, unsigned kKinds, typename F>
 hipError_t withDense(const GridView &g, int filter, const F &f)
 {
-  const uint64_t nbz = ((uint64_t)g.activeSize[2] + (1u << g.layout.sz) - 1) >> g.layout.sz;
-  if (nbz * g.layout.strideZ >= (1ull << 32)) return withFilter<DenseSource<T, true>>(filter, f);
-  return withFilter<DenseSource<T, false>>(filter, f);
+  if (g.temporalFormat) {
+    if constexpr ((kKinds & kTemporalKinds) != 0)
+      return withFilter<TemporalDenseSource<T>>(filter, f);
+    else
+      return hipErrorInvalidValue;
+  } else {
+    if constexpr ((kKinds & kDenseKinds) != 0) {
+      const uint64_t nbz = ((uint64_t)g.activeSize[2] + (1u << g.layout.sz) - 1) >> g.layout.sz;
+      if (nbz * g.layout.strideZ >= (1ull << 32)) return withFilter<DenseSource<T, true>>(filter, f);
+      return withFilter<DenseSource<T, false>>(filter, f);
+    } else {
+      return hipErrorInvalidValue;
+    }
+  }
 }
 
-template <typename F>
+template <unsigned kKinds = kAllKinds, typename F>
 hipError_t dispatchSource(const GridView &g, const AttrView &a, int filter, const F &f)
 {
   if (g.dense) {
-    switch (a.type) {
-    case kTypeUchar: return withDense<uint8_t>(g, filter, f);
-    case kTypeShort: return withDense<int16_t>(g, filter, f);
-    case kTypeUshort: return withDense<uint16_t>(g, filter, f);
-    case kTypeHalf: return withDense<__half>(g, filter, f);
-    case kTypeFloat: return withDense<float>(g, filter, f);
-    case kTypeDouble: return withDense<double>(g, filter, f);
-    default: return hipErrorInvalidValue;
+    if constexpr ((kKinds & (kDenseKinds | kTemporalKinds)) != 0) {
+      switch (a.type) {
+      case kTypeUchar: return withDense<uint8_t, kKinds>(g, filter, f);
+      case kTypeShort: return withDense<int16_t, kKinds>(g, filter, f);
+      case kTypeUshort: return withDense<uint16_t, kKinds>(g, filter, f);
+      case kTypeHalf: return withDense<__half, kKinds>(g, filter, f);
+      case kTypeFloat: return withDense<float, kKinds>(g, filter, f);
+      case kTypeDouble: return withDense<double, kKinds>(g, filter, f);
+      default: return hipErrorInvalidValue;
+      }
+    } else {
+      return hipErrorInvalidValue;
     }
   }
-  const bool obs = g.leafAccess != nullptr;
-  if (a.type == kTypeHalf)
-    return obs ? withFilter<SparseSource<__half, true>>(filter, f) : withFilter<SparseSource<__half>>(filter, f);
-  if (a.type == kTypeFloat)
-    return obs ? withFilter<SparseSource<float, true>>(filter, f) : withFilter<SparseSource<float>>(filter, f);
-  return hipErrorInvalidValue;
+  if constexpr ((kKinds & kSparseKinds) != 0) {
+    const bool obs = g.leafAccess != nullptr;
+    if (a.type == kTypeHalf)
+      return obs ? withFilter<SparseSource<__half, true>>(filter, f) : withFilter<SparseSource<__half>>(filter, f);
+    if (a.type == kTypeFloat)
+      return obs ? withFilter<SparseSource<float, true>>(filter, f) : withFilter<SparseSource<float>>(filter, f);
+    return hipErrorInvalidValue;
+  } else {
+    return hipErrorInvalidValue;
+  }
 }
 
 }  // namespace vklhip

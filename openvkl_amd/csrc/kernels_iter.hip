@@ -7,12 +7,12 @@
 
 #include "kernels.h"
 #include "kernels_dispatch.hh"
+#include "kernels_hit.hh"
 #include "vkl_device.hh"
 
 namespace vklhip {
 
 constexpr int kRangeBlock = 128;
-constexpr int kIterBlock  = 64;
 
 // ---- majorants ----------------------------------------------------------------
 
@@ -29,6 +29,28 @@ __device__ Range1f nodeRange(const GridView &g, const AttrView &a, uint32_t attr
         for (uint32_t z = rz[0]; z < rz[1]; ++z) {
           const uint32_t px = ox + x, py = oy + y, pz = oz + z;
           if (!(px < g.activeSize[0] && py < g.activeSize[1] && pz < g.activeSize[2])) continue;
+          if (g.temporalFormat) {
+            // every time sample of the voxel (VdbSampler_dense.ih, temporal
+            // value ranges)
+            const uint64_t v = ((uint64_t)pz * g.activeSize[1] + py) * g.activeSize[0] + px;
+            uint64_t b, e;
+            if (g.temporalFormat == 1u) {
+              b = v * g.numTimesteps;
+              e = b + g.numTimesteps;
+            } else if (g.tuIndex64) {
+              b = ((const uint64_t *)g.tuIndices)[v];
+              e = ((const uint64_t *)g.tuIndices)[v + 1];
+            } else {
+              b = ((const uint32_t *)g.tuIndices)[v];
+              e = ((const uint32_t *)g.tuIndices)[v + 1];
+            }
+            for (uint64_t k = b; k < e; ++k) {
+              const float tv = fetchAny(a.data, a.type, k);
+              r.lower        = vmin(r.lower, tv);
+              r.upper        = vmax(r.upper, tv);
+            }
+            continue;
+          }
           const float v = fetchAny(a.data, a.type, denseOffset(g.layout, px, py, pz));
           r.lower            = vmin(r.lower, v);
           r.upper            = vmax(r.upper, v);
@@ -117,28 +139,12 @@ __global__ void kIntervalNext(IteratorContextView c, IntervalState *s, IntervalO
   *result = r ? 1 : 0;
 }
 
-// The hit kernels read their context from device memory and sample through
-// a PointSampler specialised on the volume's source type and filter (chosen
-// on the host from the same context).
 __global__ void kHitInit(const IteratorContextView *__restrict__ cp, HitState *s, const float *ray)
 {
   const IteratorContextView &c = *cp;
   hitInit(c, *s, F3{ray[0], ray[1], ray[2]}, F3{ray[3], ray[4], ray[5]}, Range1f{ray[6], ray[7]}, ray[8]);
 }
 
-template <typename Src, int kFilter>
-__global__ void kHitNext(const IteratorContextView *__restrict__ cp, AttrView a, HitState *s, HitOut *out,
-                         int *result)
-{
-  const IteratorContextView &c = *cp;
-  const PointSampler<Src, kFilter> smp(c.sampler.grid, a, c.attributeIndex);
-  HitState st = *s;
-  HitOut o;
-  const bool r = hitNext(c, smp, st, o);
-  *s           = st;
-  if (r) *out = o;
-  *result = r ? 1 : 0;
-}
 
 hipError_t launchIntervalInit(const IteratorContextView &c, IntervalState *state, const float *ray, hipStream_t st)
 {
@@ -157,38 +163,18 @@ hipError_t launchHitInit(const IteratorContextView *c, HitState *state, const fl
   return hipGetLastError();
 }
 
-namespace {
-struct HitNextLaunch
-{
-  const IteratorContextView *c;
-  AttrView a;
-  HitState *state;
-  HitOut *out;
-  int *result;
-  hipStream_t st;
-  template <typename Src, int kFilter>
-  hipError_t operator()() const
-  {
-    hipLaunchKernelGGL((kHitNext<Src, kFilter>), dim3(1), dim3(1), 0, st, c, a, state, out, result);
-    return hipGetLastError();
-  }
-};
-}  // namespace
 
 hipError_t launchHitNext(const IteratorContextView &hostView, const IteratorContextView *c, const AttrView &a,
                          HitState *state, HitOut *out, int *result, hipStream_t st)
 {
-  return dispatchSource(hostView.sampler.grid, a, hostView.sampler.filter,
-                        HitNextLaunch{c, a, state, out, result, st});
+  if (hostView.sampler.grid.temporalFormat || !hostView.sampler.grid.dense)
+    return launchHitNextOther(hostView, c, a, state, out, result, st);
+  return dispatchSource<kDenseKinds>(hostView.sampler.grid, a, hostView.sampler.filter,
+                                     HitNextLaunch{c, a, state, out, result, st});
 }
 
 // ---- batches ---------------------------------------------------------------------
 
-__device__ inline Range1f rayRange(const float *tRanges, uint32_t r)
-{
-  if (!tRanges) return {0.f, vinf()};
-  return {tRanges[2 * r], tRanges[2 * r + 1]};
-}
 
 __global__ __launch_bounds__(kIterBlock) void kIntervalBatch(IteratorContextView c, uint32_t numRays,
                                                               const float *__restrict__ org,
@@ -209,27 +195,6 @@ __global__ __launch_bounds__(kIterBlock) void kIntervalBatch(IteratorContextView
   count[r] = n;
 }
 
-template <typename Src, int kFilter>
-__global__ __launch_bounds__(kIterBlock) void kHitBatch(const IteratorContextView *__restrict__ cp, AttrView a,
-                                                         uint32_t numRays, const float *__restrict__ org,
-                                                         const float *__restrict__ dir,
-                                                         const float *__restrict__ tRanges,
-                                                         const float *__restrict__ times, HitOut *__restrict__ out,
-                                                         uint32_t maxOut, uint32_t *__restrict__ count)
-{
-  const uint32_t r = blockIdx.x * kIterBlock + threadIdx.x;
-  if (r >= numRays) return;
-  const IteratorContextView &c = *cp;
-  const PointSampler<Src, kFilter> smp(c.sampler.grid, a, c.attributeIndex);
-  HitState st;
-  hitInit(c, st, F3{org[3 * r], org[3 * r + 1], org[3 * r + 2]}, F3{dir[3 * r], dir[3 * r + 1], dir[3 * r + 2]},
-          rayRange(tRanges, r), times ? times[r] : 0.f);
-  uint32_t n = 0;
-  HitOut o;
-  HitOut *dst = out + (uint64_t)r * maxOut;
-  while (n < maxOut && hitNext(c, smp, st, o)) dst[n++] = o;
-  count[r] = n;
-}
 
 hipError_t launchIntervalBatch(const IteratorContextView &c, uint32_t numRays, const float *org, const float *dir,
                                const float *tRanges, IntervalOut *out, uint32_t maxOut, uint32_t *count,
@@ -241,34 +206,16 @@ hipError_t launchIntervalBatch(const IteratorContextView &c, uint32_t numRays, c
   return hipGetLastError();
 }
 
-namespace {
-struct HitBatchLaunch
-{
-  const IteratorContextView *c;
-  AttrView a;
-  uint32_t numRays;
-  const float *org, *dir, *tRanges, *times;
-  HitOut *out;
-  uint32_t maxOut;
-  uint32_t *count;
-  hipStream_t st;
-  template <typename Src, int kFilter>
-  hipError_t operator()() const
-  {
-    hipLaunchKernelGGL((kHitBatch<Src, kFilter>), dim3((numRays + kIterBlock - 1) / kIterBlock), dim3(kIterBlock), 0,
-                       st, c, a, numRays, org, dir, tRanges, times, out, maxOut, count);
-    return hipGetLastError();
-  }
-};
-}  // namespace
 
 hipError_t launchHitBatch(const IteratorContextView &hostView, const IteratorContextView *c, const AttrView &a,
                           uint32_t numRays, const float *org, const float *dir, const float *tRanges,
                           const float *times, HitOut *out, uint32_t maxOut, uint32_t *count, hipStream_t st)
 {
   if (numRays == 0) return hipSuccess;
-  return dispatchSource(hostView.sampler.grid, a, hostView.sampler.filter,
-                        HitBatchLaunch{c, a, numRays, org, dir, tRanges, times, out, maxOut, count, st});
+  if (hostView.sampler.grid.temporalFormat || !hostView.sampler.grid.dense)
+    return launchHitBatchOther(hostView, c, a, numRays, org, dir, tRanges, times, out, maxOut, count, st);
+  return dispatchSource<kDenseKinds>(hostView.sampler.grid, a, hostView.sampler.filter,
+                                     HitBatchLaunch{c, a, numRays, org, dir, tRanges, times, out, maxOut, count, st});
 }
 
 }  // namespace vklhip

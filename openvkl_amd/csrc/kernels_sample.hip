@@ -16,13 +16,14 @@ constexpr int kBlock = 256;
 template <typename Src, int kFilter>
 __global__ __launch_bounds__(kBlock) void kSample(GridView g, AttrView a, uint32_t attr, uint64_t n,
                                                    const float *__restrict__ xyz, float *__restrict__ out, float noff,
-                                                   uint32_t outStride)
+                                                   uint32_t outStride, const float *__restrict__ times)
 {
   const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   const F3 oc = {xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]};
   const F3 p  = xfmPoint(g.o2i, oc);
-  const Src src(g, a, attr);
+  Src src(g, a, attr);
+  if (times) src.setTime(times[i]);
   float r;
   if (kFilter == kFilterNearest)
     r = sampleNearest(src, p, noff);
@@ -37,7 +38,8 @@ __global__ __launch_bounds__(kBlock) void kSample(GridView g, AttrView a, uint32
 // object->index matrix (the transposed inverse of index->object).
 template <typename Src, int kFilter>
 __global__ __launch_bounds__(kBlock) void kGradient(GridView g, AttrView a, uint32_t attr, uint64_t n,
-                                                     const float *__restrict__ xyz, float *__restrict__ out)
+                                                     const float *__restrict__ xyz, float *__restrict__ out,
+                                                     const float *__restrict__ times)
 {
   const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
@@ -45,7 +47,8 @@ __global__ __launch_bounds__(kBlock) void kGradient(GridView g, AttrView a, uint
   if (kFilter != kFilterNearest) {
     const F3 oc = {xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]};
     const F3 p  = xfmPoint(g.o2i, oc);
-    const Src src(g, a, attr);
+    Src src(g, a, attr);
+    if (times) src.setTime(times[i]);
     const F3 gi = (kFilter == kFilterCubic) ? gradientCubic(src, p) : gradientLinear(src, p);
     r           = xfmNormal(g.o2i, gi);
   }
@@ -119,6 +122,7 @@ template <typename T>
 hipError_t marchDense(const SamplerView &s, const AttrView &a, uint32_t attr, const MarchArgs &m, hipStream_t st)
 {
   const GridView &g = s.grid;
+  if (g.temporalFormat) return marchFilter<TemporalDenseSource<T>>(s, a, attr, m, st);
   const uint64_t nbz = ((uint64_t)g.activeSize[2] + (1u << g.layout.sz) - 1) >> g.layout.sz;
   if (nbz * g.layout.strideZ >= (1ull << 32)) return marchFilter<DenseSource<T, true>>(s, a, attr, m, st);
   return marchFilter<DenseSource<T, false>>(s, a, attr, m, st);
@@ -126,7 +130,7 @@ hipError_t marchDense(const SamplerView &s, const AttrView &a, uint32_t attr, co
 
 template <typename Src>
 hipError_t dispatchFilter(const SamplerView &s, const AttrView &a, uint32_t attr, uint64_t n, const float *xyz,
-                          float *out, uint32_t stride, hipStream_t st, bool gradient)
+                          float *out, uint32_t stride, const float *times, hipStream_t st, bool gradient)
 {
   const GridView &g = s.grid;
   const float noff  = g.constantCellData ? 0.f : 0.5f;
@@ -134,74 +138,76 @@ hipError_t dispatchFilter(const SamplerView &s, const AttrView &a, uint32_t attr
   const dim3 grid(blocksFor(n)), block(kBlock);
   if (gradient) {
     if (f == kFilterNearest)
-      hipLaunchKernelGGL((kGradient<Src, kFilterNearest>), grid, block, 0, st, g, a, attr, n, xyz, out);
+      hipLaunchKernelGGL((kGradient<Src, kFilterNearest>), grid, block, 0, st, g, a, attr, n, xyz, out, times);
     else if (f == kFilterCubic)
-      hipLaunchKernelGGL((kGradient<Src, kFilterCubic>), grid, block, 0, st, g, a, attr, n, xyz, out);
+      hipLaunchKernelGGL((kGradient<Src, kFilterCubic>), grid, block, 0, st, g, a, attr, n, xyz, out, times);
     else
-      hipLaunchKernelGGL((kGradient<Src, kFilterLinear>), grid, block, 0, st, g, a, attr, n, xyz, out);
+      hipLaunchKernelGGL((kGradient<Src, kFilterLinear>), grid, block, 0, st, g, a, attr, n, xyz, out, times);
   } else {
     if (f == kFilterNearest)
       hipLaunchKernelGGL((kSample<Src, kFilterNearest>), grid, block, 0, st, g, a, attr, n, xyz, out, noff,
-                         stride);
+                         stride, times);
     else if (f == kFilterCubic)
       hipLaunchKernelGGL((kSample<Src, kFilterCubic>), grid, block, 0, st, g, a, attr, n, xyz, out, noff,
-                         stride);
+                         stride, times);
     else
       hipLaunchKernelGGL((kSample<Src, kFilterLinear>), grid, block, 0, st, g, a, attr, n, xyz, out, noff,
-                         stride);
+                         stride, times);
   }
   return hipGetLastError();
 }
 
 template <typename T>
 hipError_t dispatchDense(const SamplerView &s, const AttrView &a, uint32_t attr, uint64_t n, const float *xyz,
-                         float *out, uint32_t stride, hipStream_t st, bool gradient)
+                         float *out, uint32_t stride, const float *times, hipStream_t st, bool gradient)
 {
   const GridView &g = s.grid;
+  if (g.temporalFormat)
+    return dispatchFilter<TemporalDenseSource<T>>(s, a, attr, n, xyz, out, stride, times, st, gradient);
   const uint64_t nbz = ((uint64_t)g.activeSize[2] + (1u << g.layout.sz) - 1) >> g.layout.sz;
   if (nbz * g.layout.strideZ >= (1ull << 32))
-    return dispatchFilter<DenseSource<T, true>>(s, a, attr, n, xyz, out, stride, st, gradient);
-  return dispatchFilter<DenseSource<T, false>>(s, a, attr, n, xyz, out, stride, st, gradient);
+    return dispatchFilter<DenseSource<T, true>>(s, a, attr, n, xyz, out, stride, times, st, gradient);
+  return dispatchFilter<DenseSource<T, false>>(s, a, attr, n, xyz, out, stride, times, st, gradient);
 }
 
 hipError_t dispatch(const SamplerView &s, uint32_t attr, const AttrView &a, uint64_t n, const float *xyz,
-                    float *out, uint32_t stride, hipStream_t st, bool gradient)
+                    float *out, uint32_t stride, const float *times, hipStream_t st, bool gradient)
 {
   if (s.grid.dense) {
     switch (a.type) {
-    case kTypeUchar: return dispatchDense<uint8_t>(s, a, attr, n, xyz, out, stride, st, gradient);
-    case kTypeShort: return dispatchDense<int16_t>(s, a, attr, n, xyz, out, stride, st, gradient);
-    case kTypeUshort: return dispatchDense<uint16_t>(s, a, attr, n, xyz, out, stride, st, gradient);
-    case kTypeHalf: return dispatchDense<__half>(s, a, attr, n, xyz, out, stride, st, gradient);
-    case kTypeFloat: return dispatchDense<float>(s, a, attr, n, xyz, out, stride, st, gradient);
-    case kTypeDouble: return dispatchDense<double>(s, a, attr, n, xyz, out, stride, st, gradient);
+    case kTypeUchar: return dispatchDense<uint8_t>(s, a, attr, n, xyz, out, stride, times, st, gradient);
+    case kTypeShort: return dispatchDense<int16_t>(s, a, attr, n, xyz, out, stride, times, st, gradient);
+    case kTypeUshort: return dispatchDense<uint16_t>(s, a, attr, n, xyz, out, stride, times, st, gradient);
+    case kTypeHalf: return dispatchDense<__half>(s, a, attr, n, xyz, out, stride, times, st, gradient);
+    case kTypeFloat: return dispatchDense<float>(s, a, attr, n, xyz, out, stride, times, st, gradient);
+    case kTypeDouble: return dispatchDense<double>(s, a, attr, n, xyz, out, stride, times, st, gradient);
     default: return hipErrorInvalidValue;
     }
   }
   const bool obs = s.grid.leafAccess != nullptr;
   if (a.type == kTypeHalf)
-    return obs ? dispatchFilter<SparseSource<__half, true>>(s, a, attr, n, xyz, out, stride, st, gradient)
-               : dispatchFilter<SparseSource<__half>>(s, a, attr, n, xyz, out, stride, st, gradient);
+    return obs ? dispatchFilter<SparseSource<__half, true>>(s, a, attr, n, xyz, out, stride, times, st, gradient)
+               : dispatchFilter<SparseSource<__half>>(s, a, attr, n, xyz, out, stride, times, st, gradient);
   if (a.type == kTypeFloat)
-    return obs ? dispatchFilter<SparseSource<float, true>>(s, a, attr, n, xyz, out, stride, st, gradient)
-               : dispatchFilter<SparseSource<float>>(s, a, attr, n, xyz, out, stride, st, gradient);
+    return obs ? dispatchFilter<SparseSource<float, true>>(s, a, attr, n, xyz, out, stride, times, st, gradient)
+               : dispatchFilter<SparseSource<float>>(s, a, attr, n, xyz, out, stride, times, st, gradient);
   return hipErrorInvalidValue;
 }
 
 }  // namespace
 
 hipError_t launchSample(const SamplerView &s, const AttrView &a, uint32_t attr, uint64_t n, const float *xyz,
-                        float *out, hipStream_t stream)
+                        float *out, hipStream_t stream, const float *times)
 {
   if (n == 0) return hipSuccess;
-  return dispatch(s, attr, a, n, xyz, out, 1, stream, false);
+  return dispatch(s, attr, a, n, xyz, out, 1, times, stream, false);
 }
 
 hipError_t launchGradient(const SamplerView &s, const AttrView &a, uint32_t attr, uint64_t n, const float *xyz,
-                          float *out, hipStream_t stream)
+                          float *out, hipStream_t stream, const float *times)
 {
   if (n == 0) return hipSuccess;
-  return dispatch(s, attr, a, n, xyz, out, 1, stream, true);
+  return dispatch(s, attr, a, n, xyz, out, 1, times, stream, true);
 }
 
 hipError_t launchRayMarch(const SamplerView &s, const AttrView &a, uint32_t attr, uint32_t numRays, const float *org,
@@ -231,10 +237,10 @@ hipError_t launchRayMarch(const SamplerView &s, const AttrView &a, uint32_t attr
 }
 
 hipError_t launchSampleStrided(const SamplerView &s, const AttrView &a, uint32_t attr, uint64_t n, const float *xyz,
-                               float *out, uint32_t outStride, hipStream_t stream)
+                               float *out, uint32_t outStride, hipStream_t stream, const float *times)
 {
   if (n == 0) return hipSuccess;
-  return dispatch(s, attr, a, n, xyz, out, outStride, stream, false);
+  return dispatch(s, attr, a, n, xyz, out, outStride, times, stream, false);
 }
 
 // Re-lay an x-fastest array into the brick layout L (one work-item per

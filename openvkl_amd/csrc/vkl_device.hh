@@ -154,6 +154,8 @@ struct DenseSource
   {
   }
 
+  VKL_DEV void setTime(float) {}  // temporally constant
+
   VKL_DEV Off xTerm(uint32_t x) const { return (((Off)(x >> sx) << S) | (Off)(x & ((1u << sx) - 1))) << pk; }
   VKL_DEV Off yTerm(uint32_t y) const
   {
@@ -285,6 +287,8 @@ struct SparseSource
     return v;
   }
 
+  VKL_DEV void setTime(float) {}  // temporally constant
+
   VKL_DEV uint64_t slotOf(uint64_t v) const { return leafSlot ? leafSlot[v >> 6] : (v >> 6); }
 
   VKL_DEV float errorValue(const Range1f *er, uint32_t eo) const
@@ -369,6 +373,115 @@ struct SparseSource
     const float c = (tag == 3u) ? toFloat(tile[slotOf(v)]) : (tag == 1u) ? errorValue(er, eo) : 0.f;
 #pragma unroll
     for (int i = 0; i < 8; i++) s[i] = c;
+  }
+};
+
+// structuredRegular with temporal data (temporal_data_interpolation.ih:13-152,
+// VdbSampler_dense.ih:360-560): every voxel read interpolates that voxel's
+// time samples at the source's current time (setTime; 0 by default, as for
+// calls without times). Voxels are x-fastest (temporal volumes have no bricks
+// or corner packing).
+template <typename T>
+struct TemporalDenseSource
+{
+  const T *data;
+  const uint32_t *idx32;
+  const uint64_t *idx64;
+  const float *times;
+  uint32_t nx, ny, nz;
+  int32_t rx, ry, rz;
+  uint32_t format, numTimesteps;
+  float bg, time;
+
+  VKL_DEV TemporalDenseSource(const GridView &g, const AttrView &a, uint32_t = 0)
+      : data((const T *)a.data),
+        idx32(g.tuIndex64 ? nullptr : (const uint32_t *)g.tuIndices),
+        idx64(g.tuIndex64 ? (const uint64_t *)g.tuIndices : nullptr),
+        times(g.tuTimes),
+        nx(g.activeSize[0]),
+        ny(g.activeSize[1]),
+        nz(g.activeSize[2]),
+        rx(g.rootOrigin[0]),
+        ry(g.rootOrigin[1]),
+        rz(g.rootOrigin[2]),
+        format(g.temporalFormat),
+        numTimesteps(g.numTimesteps),
+        bg(a.background),
+        time(0.f)
+  {
+  }
+
+  VKL_DEV void setTime(float t) { time = t; }
+
+  // interpolateTemporallyStructured / interpolateTemporallyUnstructured for
+  // the voxel with linear index v.
+  VKL_DEV float voxel(uint64_t v) const
+  {
+    if (format == 1u) {
+      // Time is clamped to [0, 1] (NaN to 0), which keeps both reads among
+      // the voxel's own samples; inside [0, 1] this is the reference's
+      // arithmetic.
+      const float tc      = vmin(vmax(time, 0.f), 1.f);
+      const float tf      = tc * (float)(numTimesteps - 1);
+      const uint32_t ts   = (uint32_t)tf;
+      const float w       = tf - (float)ts;
+      const uint32_t next = (w == 0.f) ? ts : ts + 1;
+      const uint64_t base = (uint64_t)numTimesteps * v;
+      return (1.f - w) * toFloat(data[base + ts]) + w * toFloat(data[base + next]);
+    }
+    const uint64_t i0 = idx64 ? idx64[v] : (uint64_t)idx32[v];
+    const uint64_t i1 = idx64 ? idx64[v + 1] : (uint64_t)idx32[v + 1];
+    const float t0    = times[i0];
+    if (time <= t0) return toFloat(data[i0]);
+    const float t1 = times[i1 - 1];
+    if (time >= t1) return toFloat(data[i1 - 1]);
+    if (isnan(time)) return time;
+    const uint64_t n = i1 - i0;
+    if (n == 2) {
+      const float w = (time - t0) * (1.f / (t1 - t0));
+      return (1.f - w) * toFloat(data[i0]) + w * toFloat(data[i1 - 1]);
+    }
+    // Binary search for [times[lower], times[upper]] around time. Commit
+    // checked that the voxel's times increase strictly, so with
+    // t0 < time < t1 every read stays inside the voxel's range and the
+    // search ends within log2(n) + 1 rounds; the bound and the clamp only
+    // guard the loop.
+    uint64_t lower = 0, upper = n;
+    bool found     = false;
+    for (int it = 0; it < 66 && lower <= upper; ++it) {
+      const uint64_t mid = (lower + upper) / 2;
+      const float tm     = times[i0 + mid];
+      const uint64_t nx1 = i0 + mid + 1 < i1 ? i0 + mid + 1 : i1 - 1;
+      if (time >= tm && time <= times[nx1]) {
+        lower = i0 + mid;
+        upper = lower + 1 < i1 ? lower + 1 : i1 - 1;
+        found = true;
+        break;
+      } else if (time < tm) {
+        upper = mid;
+      } else {
+        lower = mid;
+      }
+    }
+    if (!found) return __int_as_float(0x7fc00000);
+    const float tl = times[lower], th = times[upper];
+    const float w  = (time - tl) * (1.f / (th - tl));
+    return (1.f - w) * toFloat(data[lower]) + w * toFloat(data[upper]);
+  }
+
+  VKL_DEV float at(int32_t x, int32_t y, int32_t z) const
+  {
+    const uint32_t ox = (uint32_t)(x - rx), oy = (uint32_t)(y - ry), oz = (uint32_t)(z - rz);
+    if (!(ox < nx && oy < ny && oz < nz)) return bg;
+    return voxel(((uint64_t)oz * ny + oy) * nx + ox);
+  }
+
+  // A rolled loop: one inlined copy of the time interpolation per stencil
+  // instead of eight.
+  VKL_DEV void corners8(I3 ic, float *s) const
+  {
+#pragma unroll 1
+    for (int i = 0; i < 8; i++) s[i] = at(ic.x + ((i >> 2) & 1), ic.y + ((i >> 1) & 1), ic.z + (i & 1));
   }
 };
 
@@ -468,6 +581,16 @@ VKL_DEV void cubicTaps(const DenseSource<T, kIdx64> &src, I3 ic, float *S)
   for (int i = 0; i < 64; i++) S[i] = src.at(ic.x + (i >> 4) - 1, ic.y + ((i >> 2) & 3) - 1, ic.z + (i & 3) - 1);
 }
 
+// Temporal volumes: the 64 reads are a rolled loop. Each read interpolates in
+// time (a binary search for unstructured data), and 64 inlined copies of that
+// make the kernels that sample them several times larger.
+template <typename T>
+VKL_DEV void cubicTaps(const TemporalDenseSource<T> &src, I3 ic, float *S)
+{
+#pragma unroll 1
+  for (int i = 0; i < 64; i++) S[i] = src.at(ic.x + (i >> 4) - 1, ic.y + ((i >> 2) & 3) - 1, ic.z + (i & 3) - 1);
+}
+
 template <typename Src>
 VKL_DEV float sampleCubic(const Src &src, F3 p)
 {
@@ -551,6 +674,8 @@ struct PointSampler
       : src(g, a, attr), o2i(g.o2i), noff(g.constantCellData ? 0.f : 0.5f)
   {
   }
+
+  VKL_DEV void setTime(float t) { src.setTime(t); }
 
   VKL_DEV float operator()(F3 oc) const
   {

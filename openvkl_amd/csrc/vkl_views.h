@@ -98,6 +98,15 @@ struct GridView
   // pointer store accessStamp at leafAccess[leaf index] (null: unobserved).
   uint32_t *leafAccess;
   uint32_t accessStamp;
+  // structuredRegular temporal data (VKLTemporalFormat): 0 constant; 1
+  // structured, numTimesteps values per voxel (time fastest); 2 unstructured,
+  // voxel v owns positions [tuIndices[v], tuIndices[v + 1]) of the data and of
+  // tuTimes (indices uint32, or uint64 when tuIndex64).
+  uint32_t temporalFormat;
+  uint32_t numTimesteps;
+  const void *tuIndices;
+  uint32_t tuIndex64;
+  const float *tuTimes;
 };
 
 // Sampler = grid + filters.

@@ -157,6 +157,70 @@ std::vector<float> parseBackground(const ManagedObject &o, uint32_t numAttribute
   return bg;
 }
 
+// verifyTemporalData, temporal_data_verification.h:17-181: the number of
+// elements each attribute must hold for this temporal configuration.
+uint64_t verifyTemporalData(Device &device, uint64_t numVoxels, uint32_t format, int numTimesteps, Data *indices,
+                            Data *times)
+{
+  if (format == VKL_TEMPORAL_FORMAT_CONSTANT) {
+    if (numTimesteps != 0)
+      throw VklError("Temporally constant volumes should not have temporallyStructuredNumTimesteps provided");
+    if (indices || times)
+      throw VklError("Temporally constant volumes should not have temporallyUnstructuredIndices or "
+                     "temporallyUnstructuredTimes provided");
+    return numVoxels;
+  }
+  if (format == VKL_TEMPORAL_FORMAT_STRUCTURED) {
+    if (indices || times)
+      throw VklError("Temporally structured volumes should not have temporallyUnstructuredIndices or "
+                     "temporallyUnstructuredTimes provided");
+    if (numTimesteps <= 0)
+      throw VklError("Temporal format is VKL_TEMPORAL_FORMAT_STRUCTURED but temporallyStructuredNumTimesteps is not "
+                     "greater than 0.");
+    return numVoxels * (uint64_t)numTimesteps;
+  }
+  if (numTimesteps != 0)
+    throw VklError("Temporally unstructured volumes should not have temporallyStructuredNumTimesteps provided");
+  if (!indices)
+    throw VklError("temporal format is VKL_TEMPORAL_FORMAT_UNSTRUCTURED but temporallyUnstructuredIndices is not "
+                   "specified.");
+  if (!times)
+    throw VklError("temporal format is VKL_TEMPORAL_FORMAT_UNSTRUCTURED but temporallyUnstructuredTimes is not "
+                   "specified.");
+  if (indices->numItems < numVoxels + 1) throw VklError("temporallyUnstructuredIndices is too small.");
+  const bool need64 = numVoxels >= (uint64_t)std::numeric_limits<uint32_t>::max();
+  if (need64 && indices->dataType != VKL_ULONG)
+    throw VklError("temporallyUnstructuredIndices must be VKL_ULONG due to attribute data size");
+  if (!need64 && indices->dataType == VKL_ULONG)
+    device.log(VKL_LOG_WARNING, "WARNING: temporallyUnstructuredIndices is VKL_ULONG when VKL_UINT is sufficient "
+                                "and may be more performant");
+  if (indices->dataType != VKL_UINT && indices->dataType != VKL_ULONG)
+    throw VklError("temporallyUnstructuredIndices must have type VKL_UINT or VKL_ULONG");
+  if (times->dataType != VKL_FLOAT) throw VklError("temporallyUnstructuredIndices must have type VKL_FLOAT");
+  std::vector<uint64_t> idx(indices->numItems);
+  if (indices->dataType == VKL_UINT) {
+    const std::vector<uint32_t> i32 = indices->hostArray<uint32_t>();
+    std::copy(i32.begin(), i32.end(), idx.begin());
+  } else {
+    idx = indices->hostArray<uint64_t>();
+  }
+  const std::vector<float> t = times->hostArray<float>();
+  uint64_t end = 0;
+  for (uint64_t i = 0; i < numVoxels; ++i) {
+    const uint64_t b = idx[i];
+    end              = idx[i + 1];
+    if (!(b < end)) throw VklError("temporallyUnstructuredIndices must increase strictly monotonically.");
+    if (!(end <= t.size()))
+      throw VklError("Values in temporallyUnstructuredIndices are out of bounds in temporallyUnstructredTimes.");
+    for (uint64_t k = b + 1; k < end; ++k)
+      if (!(t[k - 1] < t[k]))
+        throw VklError("temporallyUnstructuredTimes must increase strictly monotonically for every voxel.");
+    if (t[b] < 0.f || t[b] > 1.f || t[end - 1] < 0.f || t[end - 1] > 1.f)
+      throw VklError("Values in temporallyUnstructuredTimes must be bounded by 0.0 and 1.0 for every voxel.");
+  }
+  return end;
+}
+
 void rejectTemporal(const ManagedObject &o)
 {
   static const char *names[] = {"temporallyStructuredNumTimesteps", "temporallyUnstructuredIndices",
@@ -294,7 +358,6 @@ void DenseVolume::commit()
   buffers.clear();
   committed = false;
   commitCommonParams();
-  rejectTemporal(*this);
 
   int32_t dims[3];
   if (!getVec3i("dimensions", dims)) throw VklError(toString() + ": missing required 'dimensions' parameter");
@@ -322,12 +385,22 @@ void DenseVolume::commit()
       ss << toString() << ": unsupported data element type (attribute " << i << ") for 'data' parameter";
       throw VklError(ss.str());
     }
-    if (attrs[i]->numItems != nvox) {
+  }
+  // Motion blur data (DenseVdbVolume.cpp:148-189): the format follows from
+  // which parameters are present; the attribute size from the format.
+  const int numTimesteps = getInt("temporallyStructuredNumTimesteps", 0);
+  Data *tuIdx            = getData("temporallyUnstructuredIndices");
+  Data *tuTimes          = getData("temporallyUnstructuredTimes");
+  const uint32_t tformat = numTimesteps > 0 ? VKL_TEMPORAL_FORMAT_STRUCTURED
+                           : tuIdx          ? VKL_TEMPORAL_FORMAT_UNSTRUCTURED
+                                            : VKL_TEMPORAL_FORMAT_CONSTANT;
+  const uint64_t expectedItems = verifyTemporalData(*device, nvox, tformat, numTimesteps, tuIdx, tuTimes);
+  for (size_t i = 0; i < attrs.size(); i++)
+    if (attrs[i]->numItems != expectedItems) {
       std::ostringstream ss;
-      ss << "attribute " << i << " has " << attrs[i]->numItems << " elements, but expected " << nvox;
+      ss << "attribute " << i << " has " << attrs[i]->numItems << " elements, but expected " << expectedItems;
       throw VklError(ss.str());
     }
-  }
 
   float col[12] = {spacing[0], 0.f, 0.f, 0.f, spacing[1], 0.f, 0.f, 0.f, spacing[2], origin[0], origin[1], origin[2]};
   getAffine3f("indexToObject", col);
@@ -364,6 +437,8 @@ void DenseVolume::commit()
   const bool packed = getBool("cornerPacked", false);
   if (packed && (shift[0] | shift[1] | shift[2]))
     throw VklError("cornerPacked and brickShift cannot be combined", VKL_INVALID_ARGUMENT);
+  if (tformat != VKL_TEMPORAL_FORMAT_CONSTANT && (packed || (shift[0] | shift[1] | shift[2])))
+    throw VklError("cornerPacked and brickShift are not supported for temporal data", VKL_INVALID_ARGUMENT);
   DenseLayout &L = grid.layout;
   L.sx           = (uint32_t)shift[0];
   L.sy           = (uint32_t)shift[1];
@@ -422,6 +497,17 @@ void DenseVolume::commit()
     attrViews.push_back(av);
   }
   uploadAttrViews(*this, st);
+
+  grid.temporalFormat = tformat;
+  grid.numTimesteps   = tformat == VKL_TEMPORAL_FORMAT_STRUCTURED ? (uint32_t)numTimesteps : 0u;
+  grid.tuIndices      = nullptr;
+  grid.tuIndex64      = 0;
+  grid.tuTimes        = nullptr;
+  if (tformat == VKL_TEMPORAL_FORMAT_UNSTRUCTURED) {
+    grid.tuIndices = deviceData(*this, *tuIdx, st);
+    grid.tuIndex64 = tuIdx->dataType == VKL_ULONG ? 1u : 0u;
+    grid.tuTimes   = static_cast<const float *>(deviceData(*this, *tuTimes, st));
+  }
 
   // One leaf per 8^3 block, x fastest (DenseVdbVolume.cpp:66-119).
   const uint32_t lx = (dims[0] + 7) / 8, ly = (dims[1] + 7) / 8, lz = (dims[2] + 7) / 8;

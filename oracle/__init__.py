@@ -72,6 +72,10 @@ def lib():
         L.oracle_iterate_hits.argtypes = [vp, u32, i32, i32, vp, vp, vp, vp, f32, vp, u32]
         L.oracle_tricubic_matrix.argtypes = [vp]
         L.oracle_set_leaf_access.argtypes = [vp, vp]
+        L.oracle_dense_new_temporal.restype = vp
+        L.oracle_dense_new_temporal.argtypes = [vp, vp, vp, i32, u32, vp, vp, vp, u32, u32, vp, vp, u64]
+        L.oracle_sample_times.argtypes = [vp, u64, vp, vp, vp, u32]
+        L.oracle_gradient_times.argtypes = [vp, u64, vp, vp, vp, u32]
         L.oracle_inner_nodes.restype = u64
         L.oracle_inner_nodes.argtypes = [vp, i32, vp, u64]
         L.oracle_half_to_float.restype = f32
@@ -123,6 +127,31 @@ class Grid:
         bg = None if background is None else np.asarray(background, np.float32).reshape(-1)
         h = lib().oracle_dense_new(_p(dims), _p(io), _p(m), int(cell_centered), len(arrs), _p(dt), ptrs, _p(bg))
         return cls(h, (arrs, dims, io, m, dt, ptrs, bg))
+
+    @classmethod
+    def dense_temporal(cls, dims, data, structured_steps=None, tu_indices=None, tu_times=None, origin=(0, 0, 0),
+                       spacing=(1, 1, 1), i2o=None, cell_centered=False, index_origin=(0, 0, 0), background=None):
+        """structuredRegular with motion-blur data: data is a 1-D array (or a
+        list of them) holding structured_steps values per voxel (x-fastest
+        voxels, time fastest), or tu_indices[voxels] values addressed through
+        tu_indices / tu_times."""
+        arrs = data if isinstance(data, (list, tuple)) else [data]
+        arrs = [np.ascontiguousarray(a).reshape(-1) for a in arrs]
+        d = np.asarray(dims, np.int32)
+        io = np.array(index_origin, np.int32)
+        m = np.asarray(i2o, np.float32) if i2o is not None else affine_from_origin_spacing(origin, spacing)
+        dt = np.array([DTYPE_CODE[a.dtype] for a in arrs], np.uint32)
+        ptrs = (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+        bg = None if background is None else np.asarray(background, np.float32).reshape(-1)
+        if structured_steps:
+            fmt, idx, tms, steps = 1, None, None, int(structured_steps)
+        else:
+            fmt, steps = 2, 0
+            idx = np.ascontiguousarray(tu_indices, np.uint64)
+            tms = np.ascontiguousarray(tu_times, np.float32)
+        h = lib().oracle_dense_new_temporal(_p(d), _p(io), _p(m), int(cell_centered), len(arrs), _p(dt), ptrs, _p(bg),
+                                            fmt, steps, _p(idx), _p(tms), 0 if tms is None else len(tms))
+        return cls(h, (arrs, d, io, m, dt, ptrs, bg, idx, tms))
 
     @classmethod
     def vdb(cls, level, origin, fmt, packed_dense=None, packed_tile=None, leaf_data=None,
@@ -191,10 +220,13 @@ class Grid:
     def set_cubic_double(self, on=True):
         lib().oracle_set_cubic_double(self.h, int(bool(on)))
 
-    def sample(self, xyz, attr=0, threads=1):
+    def sample(self, xyz, attr=0, threads=1, times=None):
         xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
         out = np.empty(xyz.shape[0], np.float32)
-        if threads > 1:
+        if times is not None:
+            tm = np.ascontiguousarray(np.broadcast_to(np.asarray(times, np.float32), (xyz.shape[0],)))
+            lib().oracle_sample_times(self.h, xyz.shape[0], _p(xyz), _p(tm), _p(out), attr)
+        elif threads > 1:
             lib().oracle_sample_mt(self.h, xyz.shape[0], _p(xyz), _p(out), attr, threads)
         else:
             lib().oracle_sample(self.h, xyz.shape[0], _p(xyz), _p(out), attr)
@@ -207,10 +239,14 @@ class Grid:
         lib().oracle_sample_m(self.h, xyz.shape[0], _p(xyz), _p(out), at.shape[0], _p(at))
         return out
 
-    def gradient(self, xyz, attr=0):
+    def gradient(self, xyz, attr=0, times=None):
         xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
         out = np.empty((xyz.shape[0], 3), np.float32)
-        lib().oracle_gradient(self.h, xyz.shape[0], _p(xyz), _p(out), attr)
+        if times is not None:
+            tm = np.ascontiguousarray(np.broadcast_to(np.asarray(times, np.float32), (xyz.shape[0],)))
+            lib().oracle_gradient_times(self.h, xyz.shape[0], _p(xyz), _p(tm), _p(out), attr)
+        else:
+            lib().oracle_gradient(self.h, xyz.shape[0], _p(xyz), _p(out), attr)
         return out
 
     # -- metadata -----------------------------------------------------------------

@@ -266,6 +266,15 @@ struct OracleGrid
   // LeafNodeAccess observation (VdbLeafAccessObserver.ih): when set, every
   // sampling traversal that ends on a leaf pointer sets entry [leaf index].
   uint32_t *leafAccess = nullptr;
+
+  // structuredRegular temporal data: format 0 constant, 1 structured
+  // (numTimesteps values per voxel), 2 unstructured (tuIndices / tuTimes);
+  // curTime is the time of the sample being evaluated.
+  uint32_t temporalFormat = 0;
+  uint32_t numTimesteps   = 0;
+  std::vector<uint64_t> tuIndices;
+  std::vector<float> tuTimes;
+  mutable float curTime = 0.f;
 };
 
 namespace {
@@ -335,12 +344,63 @@ inline bool isInDomain(V3u size, V3u o)
 // --- dense sampling (VdbSampler_traverseAndSample.ih:314-349,
 //     VdbSampler_dense.ih:290-330) ---------------------------------------------
 
+// interpolateTemporallyStructured, temporal_data_interpolation.ih:13-45, with
+// baseIdx = numTimesteps * voxel (VdbSampler_dense.ih:360-446).
+float temporallyStructured(const OracleGrid &g, uint32_t attr, uint64_t voxel, float time)
+{
+  const float timestepFloat = time * ((float)(g.numTimesteps - 1));
+  const uint32_t timestep   = (uint32_t)timestepFloat;
+  const float w             = timestepFloat - ((float)timestep);
+  const uint32_t next       = (w == 0.f) ? timestep : timestep + 1;
+  const uint64_t base       = (uint64_t)g.numTimesteps * voxel;
+  const float v0 = fetch(g.denseData[attr].data(), g.attrTypes[attr], base + timestep);
+  const float v1 = fetch(g.denseData[attr].data(), g.attrTypes[attr], base + next);
+  return (1.f - w) * v0 + w * v1;
+}
+
+// interpolateTemporallyUnstructured, temporal_data_interpolation.ih:82-152.
+// rcp is taken as the correctly rounded 1 / x.
+float temporallyUnstructured(const OracleGrid &g, uint32_t attr, uint64_t voxel, float time)
+{
+  const uint8_t *d  = g.denseData[attr].data();
+  const uint32_t ty = g.attrTypes[attr];
+  const uint64_t dataIdx = g.tuIndices[voxel], dataNextIdx = g.tuIndices[voxel + 1];
+  const float t0 = g.tuTimes[dataIdx];
+  if (time <= t0) return fetch(d, ty, dataIdx);
+  const float t1 = g.tuTimes[dataNextIdx - 1];
+  if (time >= t1) return fetch(d, ty, dataNextIdx - 1);
+  const uint64_t numTimeSamples = dataNextIdx - dataIdx;
+  if (numTimeSamples == 2) {
+    const float w = (time - t0) * (1.f / (t1 - t0));
+    return (1.f - w) * fetch(d, ty, dataIdx) + w * fetch(d, ty, dataNextIdx - 1);
+  }
+  uint64_t lower = 0, upper = numTimeSamples;
+  while (lower <= upper) {
+    const uint64_t mid   = (lower + upper) / 2;
+    const float timesMid = g.tuTimes[dataIdx + mid];
+    if (time >= timesMid && time <= g.tuTimes[dataIdx + mid + 1]) {
+      lower = dataIdx + mid;
+      upper = lower + 1;
+      break;
+    } else if (time < timesMid) {
+      upper = mid;
+    } else {
+      lower = mid;
+    }
+  }
+  const float timesLow = g.tuTimes[lower], timesHigh = g.tuTimes[upper];
+  const float w        = (time - timesLow) * (1.f / (timesHigh - timesLow));
+  return (1.f - w) * fetch(d, ty, lower) + w * fetch(d, ty, upper);
+}
+
 inline float denseCorner(const OracleGrid &g, uint32_t attr, V3i ic)
 {
   const V3u o = toDomainOffset(ic, g.rootOrigin);
   if (!isInDomain(g.activeSize, o)) return g.background[attr];
   const uint64_t idx = (uint64_t)o.z * g.activeSize.y * (uint64_t)g.activeSize.x +
                        (uint64_t)o.y * g.activeSize.x + o.x;
+  if (g.temporalFormat == 1) return temporallyStructured(g, attr, idx, g.curTime);
+  if (g.temporalFormat == 2) return temporallyUnstructured(g, attr, idx, g.curTime);
   return fetch(g.denseData[attr].data(), g.attrTypes[attr], idx);
 }
 
@@ -667,9 +727,20 @@ R1f nodeValueRange(const OracleGrid &g, uint64_t voxel, V3u o, const uint32_t rx
             continue;
           const uint64_t idx = (uint64_t)p.z * g.activeSize.y * (uint64_t)g.activeSize.x +
                                (uint64_t)p.y * g.activeSize.x + p.x;
-          const float v = fetch(g.denseData[attr].data(), g.attrTypes[attr], idx);
-          r.lower = fmin_(r.lower, v);
-          r.upper = fmax_(r.upper, v);
+          // temporal data: every time sample of the voxel
+          uint64_t b = idx, e = idx + 1;
+          if (g.temporalFormat == 1) {
+            b = idx * g.numTimesteps;
+            e = b + g.numTimesteps;
+          } else if (g.temporalFormat == 2) {
+            b = g.tuIndices[idx];
+            e = g.tuIndices[idx + 1];
+          }
+          for (uint64_t k = b; k < e; ++k) {
+            const float v = fetch(g.denseData[attr].data(), g.attrTypes[attr], k);
+            r.lower = fmin_(r.lower, v);
+            r.upper = fmax_(r.upper, v);
+          }
         }
     return r;
   }
@@ -1213,6 +1284,16 @@ OracleGrid *oracle_dense_new(const int32_t dims[3], const int32_t indexOrigin[3]
                              int cellCentered, uint32_t numAttributes, const uint32_t *dtypes,
                              const void *const *data, const float *background)
 {
+  return oracle_dense_new_temporal(dims, indexOrigin, i2o, cellCentered, numAttributes, dtypes, data, background,
+                                   0, 0, nullptr, nullptr, 0);
+}
+
+OracleGrid *oracle_dense_new_temporal(const int32_t dims[3], const int32_t indexOrigin[3], const float i2o[12],
+                                      int cellCentered, uint32_t numAttributes, const uint32_t *dtypes,
+                                      const void *const *data, const float *background, uint32_t temporalFormat,
+                                      uint32_t numTimesteps, const uint64_t *tuIndices, const float *tuTimes,
+                                      uint64_t numTimes)
+{
   try {
     auto *g = new OracleGrid();
     g->dense            = true;
@@ -1222,9 +1303,18 @@ OracleGrid *oracle_dense_new(const int32_t dims[3], const int32_t indexOrigin[3]
     std::memcpy(g->i2o, i2o, sizeof(g->i2o));
     invertAffine(g->i2o, g->o2i);
     const uint64_t nvox = (uint64_t)dims[0] * dims[1] * dims[2];
+    g->temporalFormat   = temporalFormat;
+    g->numTimesteps     = numTimesteps;
+    uint64_t items      = nvox;
+    if (temporalFormat == 1) items = nvox * numTimesteps;
+    if (temporalFormat == 2) {
+      g->tuIndices.assign(tuIndices, tuIndices + nvox + 1);
+      g->tuTimes.assign(tuTimes, tuTimes + numTimes);
+      items = g->tuIndices[nvox];
+    }
     for (uint32_t a = 0; a < numAttributes; a++) {
       g->attrTypes.push_back(dtypes[a]);
-      const size_t bytes = nvox * typeSize(dtypes[a]);
+      const size_t bytes = items * typeSize(dtypes[a]);
       g->denseData.emplace_back((const uint8_t *)data[a], (const uint8_t *)data[a] + bytes);
       g->background.push_back(background ? background[a] : vkl_bg());
     }
@@ -1355,8 +1445,32 @@ void oracle_set_sampler(OracleGrid *g, int filter, int gradientFilter, int maxSa
   g->maxSamplingDepth = (uint32_t)std::min(maxSamplingDepth, 3);
 }
 
+void oracle_sample_times(const OracleGrid *g, uint64_t n, const float *xyz, const float *times, float *out,
+                         uint32_t attr)
+{
+  for (uint64_t i = 0; i < n; i++) {
+    g->curTime = times ? times[i] : 0.f;
+    out[i]     = sampleObject(*g, V3f{xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]}, attr);
+  }
+  g->curTime = 0.f;
+}
+
+void oracle_gradient_times(const OracleGrid *g, uint64_t n, const float *xyz, const float *times, float *out3,
+                           uint32_t attr)
+{
+  for (uint64_t i = 0; i < n; i++) {
+    g->curTime  = times ? times[i] : 0.f;
+    const V3f r = gradientObject(*g, V3f{xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]}, attr);
+    out3[3 * i]     = r.x;
+    out3[3 * i + 1] = r.y;
+    out3[3 * i + 2] = r.z;
+  }
+  g->curTime = 0.f;
+}
+
 void oracle_sample(const OracleGrid *g, uint64_t n, const float *xyz, float *out, uint32_t attr)
 {
+  g->curTime = 0.f;
   for (uint64_t i = 0; i < n; i++)
     out[i] = sampleObject(*g, V3f{xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]}, attr);
 }
@@ -1376,6 +1490,7 @@ void oracle_sample_mt(const OracleGrid *g, uint64_t n, const float *xyz, float *
 void oracle_sample_m(const OracleGrid *g, uint64_t n, const float *xyz, float *out, uint32_t m,
                      const uint32_t *attrs)
 {
+  g->curTime = 0.f;
   for (uint64_t i = 0; i < n; i++)
     for (uint32_t a = 0; a < m; a++)
       out[i * m + a] = sampleObject(*g, V3f{xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]}, attrs[a]);
@@ -1383,6 +1498,7 @@ void oracle_sample_m(const OracleGrid *g, uint64_t n, const float *xyz, float *o
 
 void oracle_gradient(const OracleGrid *g, uint64_t n, const float *xyz, float *out3, uint32_t attr)
 {
+  g->curTime = 0.f;
   for (uint64_t i = 0; i < n; i++) {
     const V3f r = gradientObject(*g, V3f{xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]}, attr);
     out3[3 * i]     = r.x;
@@ -1497,7 +1613,7 @@ uint32_t oracle_iterate_hits(const OracleGrid *g, uint32_t attr, int depth, int 
                              const float org[3], const float dir[3], const float tRange[2], float time,
                              float *out, uint32_t maxOut)
 {
-  (void)time;
+  g->curTime = time;  // temporal volumes sample at the ray time
   if (depth < 0) depth = g->dense ? oracle_map_depth(0.5f) : oracle_map_depth(1.f);
   std::vector<float> vals(values, values + numValues);
   std::vector<float> pr;

@@ -33,6 +33,24 @@ OracleGrid *oracle_dense_new(const int32_t dims[3],
                              const void *const *data,
                              const float *background);
 
+/* structuredRegular with temporal data: temporalFormat 0 constant, 1
+ * structured (data[a] holds numTimesteps values per voxel, time fastest), 2
+ * unstructured (tuIndices: voxels + 1 entries, tuTimes: numTimes floats,
+ * data[a] holds tuIndices[voxels] values). */
+OracleGrid *oracle_dense_new_temporal(const int32_t dims[3],
+                                      const int32_t indexOrigin[3],
+                                      const float i2o[12],
+                                      int cellCentered,
+                                      uint32_t numAttributes,
+                                      const uint32_t *dtypes,
+                                      const void *const *data,
+                                      const float *background,
+                                      uint32_t temporalFormat,
+                                      uint32_t numTimesteps,
+                                      const uint64_t *tuIndices,
+                                      const float *tuTimes,
+                                      uint64_t numTimes);
+
 /* vdb volume. Leaves: level, origin (index space, 3 ints each), format.
  * packed != 0: packedDense[a] holds (#dense leaves)*512 elements and
  * packedTile[a] (#tile leaves) elements (may be NULL when there are no tiles).
@@ -62,6 +80,11 @@ void oracle_set_sampler(OracleGrid *g, int filter, int gradientFilter, int maxSa
 void oracle_set_cubic_double(OracleGrid *g, int on);
 
 void oracle_sample(const OracleGrid *g, uint64_t n, const float *xyz, float *out, uint32_t attr);
+/* Per-point times (NULL: 0) for temporal volumes; single-threaded. */
+void oracle_sample_times(const OracleGrid *g, uint64_t n, const float *xyz, const float *times, float *out,
+                         uint32_t attr);
+void oracle_gradient_times(const OracleGrid *g, uint64_t n, const float *xyz, const float *times, float *out3,
+                           uint32_t attr);
 void oracle_sample_mt(const OracleGrid *g, uint64_t n, const float *xyz, float *out, uint32_t attr, int threads);
 void oracle_sample_m(const OracleGrid *g, uint64_t n, const float *xyz, float *out, uint32_t m, const uint32_t *attrs);
 void oracle_gradient(const OracleGrid *g, uint64_t n, const float *xyz, float *out3, uint32_t attr);

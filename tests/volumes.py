@@ -103,3 +103,55 @@ def dense_vdb_leaves(dims, fns, origin=(0, 0, 0), spacing=(1, 1, 1)):
     y = (o[1] + iy.astype(np.float32) * sp[1]).astype(np.float64)
     z = (o[2] + iz.astype(np.float32) * sp[2]).astype(np.float64)
     return origins, [fn(x, y, z).astype(np.float32) for fn in fns]
+
+
+def temporal_volume(dims, fn, kind, dtype=np.float32, origin=(0, 0, 0), spacing=(1, 1, 1), seed=SEED + 5,
+                    index64=False):
+    """Motion-blur voxel data of fn(x, y, z, t) at the voxel object positions
+    (testing/volume/TestingStructuredVolume.h TemporalConfig):
+    ("structured", T) - T equally spaced times per voxel, time fastest;
+    ("unstructured", times) - the same sorted times for every voxel;
+    ("varying", None) - 1..5 sorted random times per voxel (first 0 and last 1
+    when there are two or more).
+    Returns (data 1-D, kwargs for the volume constructors)."""
+    nx, ny, nz = dims
+    cx = object_coords(nx, origin, spacing)[0]
+    cy = object_coords(ny, origin, spacing)[1]
+    cz = object_coords(nz, origin, spacing)[2]
+    z, y, x = np.meshgrid(cz, cy, cx, indexing="ij")
+    x, y, z = x.reshape(-1), y.reshape(-1), z.reshape(-1)
+    nvox = x.size
+    if kind[0] == "structured":
+        steps = int(kind[1])
+        t = np.linspace(0, 1, steps, dtype=np.float32) if steps > 1 else np.zeros(1, np.float32)
+        vals = fn(x[:, None], y[:, None], z[:, None], t[None, :].astype(np.float64))
+        return vals.reshape(-1).astype(dtype), {"structured_steps": steps}
+    if kind[0] == "unstructured":
+        t = np.asarray(kind[1], np.float32)
+        counts = np.full(nvox, t.size, np.int64)
+        times = np.tile(t, nvox)
+    else:
+        rng = np.random.default_rng(seed)
+        counts = rng.integers(1, 6, nvox)
+        times = []
+        for c in counts:
+            if c == 1:
+                times.append(rng.random(1, dtype=np.float32))
+            else:
+                inner = np.sort(rng.choice(np.arange(1, 1000), c - 2, replace=False)).astype(np.float32) / 1000
+                times.append(np.concatenate([[0.0], inner, [1.0]]).astype(np.float32))
+        times = np.concatenate(times).astype(np.float32)
+    idx = np.zeros(nvox + 1, np.uint64 if index64 else np.uint32)
+    idx[1:] = np.cumsum(counts)
+    rep = np.repeat(np.arange(nvox), counts)
+    vals = fn(x[rep], y[rep], z[rep], times.astype(np.float64))
+    return vals.astype(dtype), {"tu_indices": idx, "tu_times": times}
+
+
+def xyz_time(x, y, z, t):
+    """(1 - t) x y z (procedural_functions.h getXYZValue, time-linear)."""
+    return (1.0 - t) * x * y * z
+
+
+def wavelet_time(x, y, z, t):
+    return (np.sin(3 * x) + np.sin(3 * y) + np.cos(3 * z)) * (1.0 + t)
