@@ -1,5 +1,6 @@
 /* Temporal formats; values fixed by openvkl/include/openvkl/VKLTemporalFormat.h.
- * The HIP device accepts VKL_TEMPORAL_FORMAT_CONSTANT only. */
+ * On the HIP device structuredRegular volumes take all three formats and vdb
+ * volumes VKL_TEMPORAL_FORMAT_CONSTANT only. */
 #pragma once
 #include "types.h"
 
