@@ -1,6 +1,6 @@
 /* Temporal formats; values fixed by openvkl/include/openvkl/VKLTemporalFormat.h.
- * On the HIP device structuredRegular volumes take all three formats and vdb
- * volumes VKL_TEMPORAL_FORMAT_CONSTANT only. */
+ * On the HIP device structuredRegular volumes and vdb nodes given through
+ * node.data take all three formats. */
 #pragma once
 #include "types.h"
 

@@ -262,7 +262,8 @@ class Device:
         return obj
 
     def data_of_data(self, items):
-        arr = (VKLObject * len(items))(*[i.h for i in items])
+        """VKL_DATA array of handles (None entries are null objects)."""
+        arr = (VKLObject * len(items))(*[VKLObject() if i is None else i.h for i in items])
         h = lib().vklNewData(self.h, len(items), VKL_DATA, C.addressof(arr), 0, 0)
         self.check()
         return Handle(self, h)
@@ -325,8 +326,12 @@ class Device:
         return v
 
     def vdb(self, level, origin, fmt, packed_dense=None, packed_tile=None, leaf_data=None, background=None,
-            index_to_object=None, filter=None, corner_packed=None, max_sampling_depth=None):
-        """Sparse vdb volume from node arrays (packed or per-leaf data)."""
+            index_to_object=None, filter=None, corner_packed=None, max_sampling_depth=None, temporal_format=None,
+            steps=None, tu_indices=None, tu_times=None):
+        """Sparse vdb volume from node arrays (packed or per-leaf data).
+        Motion blur (per-leaf data only): temporal_format per node, steps per
+        node (node.temporallyStructuredNumTimesteps), tu_indices / tu_times
+        per node (arrays, None for other nodes)."""
         v = self.new_volume("vdb")
         tmp = []
 
@@ -368,6 +373,24 @@ class Device:
             v.set_bool("cornerPacked", corner_packed)
         if max_sampling_depth is not None:
             v.set_int("maxSamplingDepth", max_sampling_depth)
+        if temporal_format is not None:
+            v.set_data("node.temporalFormat", d(np.ascontiguousarray(temporal_format, np.uint32)))
+        if steps is not None:
+            v.set_data("node.temporallyStructuredNumTimesteps", d(np.ascontiguousarray(steps, np.int32)))
+        for name, arrs in (("node.temporallyUnstructuredIndices", tu_indices),
+                           ("node.temporallyUnstructuredTimes", tu_times)):
+            if arrs is None:
+                continue
+            items = []
+            for a in arrs:
+                if a is None:
+                    items.append(None)
+                else:
+                    a = np.ascontiguousarray(a)
+                    items.append(d(a, VKL_ULONG if a.dtype == np.uint64 else None))
+            dd = self.data_of_data(items)
+            tmp.append(dd)
+            v.set_data(name, dd)
         v.commit()
         for h in tmp:
             h.release()

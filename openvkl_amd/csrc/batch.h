@@ -58,7 +58,7 @@ inline uint32_t sourceKey(const GridView &g, const AttrView &a, int filter)
     const uint64_t nbz = ((uint64_t)g.activeSize[2] + (1u << g.layout.sz) - 1) >> g.layout.sz;
     variant            = nbz * g.layout.strideZ >= (1ull << 32) ? 1u : 0u;
   } else {
-    variant = g.leafAccess ? 1u : 0u;
+    variant = g.temporalFormat ? (1u << 11) : (g.leafAccess ? 1u : 0u);
   }
   return (a.type << 12) | ((uint32_t)filter << 2) | (g.dense ? 2u : 0u) | variant;
 }

@@ -65,6 +65,13 @@ hipError_t dispatchSource(const GridView &g, const AttrView &a, int filter, cons
       return hipErrorInvalidValue;
     }
   }
+  if (g.temporalFormat) {
+    if constexpr ((kKinds & kTemporalKinds) != 0) {
+      if (a.type == kTypeHalf) return withFilter<TemporalSparseSource<__half>>(filter, f);
+      if (a.type == kTypeFloat) return withFilter<TemporalSparseSource<float>>(filter, f);
+    }
+    return hipErrorInvalidValue;
+  }
   if constexpr ((kKinds & kSparseKinds) != 0) {
     const bool obs = g.leafAccess != nullptr;
     if (a.type == kTypeHalf)

@@ -69,6 +69,37 @@ __device__ Range1f nodeRange(const GridView &g, const AttrView &a, uint32_t attr
     if (!inDom) return {a.background, a.background};
     return {0.f, 0.f};
   }
+  if (g.temporalFormat) {
+    // temporal leaves: every time sample of the window's voxels (of the one
+    // voxel of a tile)
+    const uint64_t li   = voxel >> 6;
+    const uint32_t tf   = (uint32_t)(voxel >> 4) & 3u;
+    const uint64_t base = g.leafDataOffset[li];
+    auto extend = [&](uint64_t vi) {
+      uint64_t b = base + vi, e = b + 1;
+      if (tf == 1u) {
+        b = base + vi * g.leafSteps[li];
+        e = b + g.leafSteps[li];
+      } else if (tf == 2u) {
+        const uint64_t *ix = (const uint64_t *)g.tuIndices + g.leafIdxOffset[li];
+        b                  = base + ix[vi];
+        e                  = base + ix[vi + 1];
+      }
+      for (uint64_t k = b; k < e; ++k) {
+        const float v = fetchAny(a.data, a.type, k);
+        r.lower       = vmin(r.lower, v);
+        r.upper       = vmax(r.upper, v);
+      }
+    };
+    if (((voxel >> 2) & 3u) != 1u) {
+      extend(0);
+      return r;
+    }
+    for (uint32_t x = rx[0]; x < rx[1]; ++x)
+      for (uint32_t y = ry[0]; y < ry[1]; ++y)
+        for (uint32_t z = rz[0]; z < rz[1]; ++z) extend((x << 6) | (y << 3) | z);
+    return r;
+  }
   const uint64_t leaf = g.leafSlot ? g.leafSlot[voxel >> 6] : (voxel >> 6);
   if (((voxel >> 2) & 3u) == 0u) {
     const float v = fetchAny(a.tile, a.type, leaf);

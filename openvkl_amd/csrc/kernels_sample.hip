@@ -184,6 +184,13 @@ hipError_t dispatch(const SamplerView &s, uint32_t attr, const AttrView &a, uint
     default: return hipErrorInvalidValue;
     }
   }
+  if (s.grid.temporalFormat) {
+    if (a.type == kTypeHalf)
+      return dispatchFilter<TemporalSparseSource<__half>>(s, a, attr, n, xyz, out, stride, times, st, gradient);
+    if (a.type == kTypeFloat)
+      return dispatchFilter<TemporalSparseSource<float>>(s, a, attr, n, xyz, out, stride, times, st, gradient);
+    return hipErrorInvalidValue;
+  }
   const bool obs = s.grid.leafAccess != nullptr;
   if (a.type == kTypeHalf)
     return obs ? dispatchFilter<SparseSource<__half, true>>(s, a, attr, n, xyz, out, stride, times, st, gradient)
@@ -225,6 +232,11 @@ hipError_t launchRayMarch(const SamplerView &s, const AttrView &a, uint32_t attr
     case kTypeDouble: return marchDense<double>(s, a, attr, m, stream);
     default: return hipErrorInvalidValue;
     }
+  }
+  if (s.grid.temporalFormat) {
+    if (a.type == kTypeHalf) return marchFilter<TemporalSparseSource<__half>>(s, a, attr, m, stream);
+    if (a.type == kTypeFloat) return marchFilter<TemporalSparseSource<float>>(s, a, attr, m, stream);
+    return hipErrorInvalidValue;
   }
   const bool obs = s.grid.leafAccess != nullptr;
   if (a.type == kTypeHalf)

@@ -91,6 +91,7 @@ OPENVKL_INTERFACE VKLNeuralField vklNewNeuralField(const VKLSampler *exactSample
   Sampler *s = static_cast<Sampler *>(exactSampler->host);
   Volume *v  = s->volume;
   if (v->grid.dense) throw VklError("neural fields are built for vdb volumes");
+  if (v->grid.temporalFormat) throw VklError("neural fields are built for temporally constant vdb volumes");
   (void)s->attr(attributeIndex);
   dev_->makeCurrent();
   hipStream_t st = dev_->stream;

@@ -283,7 +283,7 @@ struct SparseSource
   // store the same stamp, so the races are benign.
   VKL_DEV uint64_t found(uint64_t v) const
   {
-    if (kObserve && ((uint32_t)v & 3u) == 3u) leafAccess[v >> 6] = accessStamp;
+    if (kObserve && ((uint32_t)v & 3u) == 3u && leafAccess) leafAccess[v >> 6] = accessStamp;
     return v;
   }
 
@@ -376,6 +376,65 @@ struct SparseSource
   }
 };
 
+// interpolateTemporallyStructured, temporal_data_interpolation.ih:13-45:
+// voxel v owns samples [steps * v, steps * v + steps) of data. Time is
+// clamped to [0, 1] (NaN to 0), which keeps both reads among the voxel's own
+// samples; inside [0, 1] this is the reference's arithmetic.
+template <typename T>
+VKL_DEV float temporallyStructured(const T *data, uint32_t steps, uint64_t v, float time)
+{
+  const float tc      = vmin(vmax(time, 0.f), 1.f);
+  const float tf      = tc * (float)(steps - 1);
+  const uint32_t ts   = (uint32_t)tf;
+  const float w       = tf - (float)ts;
+  const uint32_t next = (w == 0.f) ? ts : ts + 1;
+  const uint64_t base = (uint64_t)steps * v;
+  return (1.f - w) * toFloat(data[base + ts]) + w * toFloat(data[base + next]);
+}
+
+// interpolateTemporallyUnstructured, temporal_data_interpolation.ih:82-152:
+// voxel v owns positions [idx[v], idx[v + 1]) of data and times.
+template <typename T, typename I>
+VKL_DEV float temporallyUnstructured(const T *data, const I *idx, const float *times, uint64_t v, float time)
+{
+  const uint64_t i0 = idx[v], i1 = idx[v + 1];
+  const float t0    = times[i0];
+  if (time <= t0) return toFloat(data[i0]);
+  const float t1 = times[i1 - 1];
+  if (time >= t1) return toFloat(data[i1 - 1]);
+  if (isnan(time)) return time;
+  const uint64_t n = i1 - i0;
+  if (n == 2) {
+    const float w = (time - t0) * (1.f / (t1 - t0));
+    return (1.f - w) * toFloat(data[i0]) + w * toFloat(data[i1 - 1]);
+  }
+  // Binary search for [times[lower], times[upper]] around time. Commit
+  // checked that the voxel's times increase strictly, so with
+  // t0 < time < t1 every read stays inside the voxel's range and the search
+  // ends within log2(n) + 1 rounds; the bound and the clamps only guard it.
+  uint64_t lower = 0, upper = n;
+  bool found     = false;
+  for (int it = 0; it < 66 && lower <= upper; ++it) {
+    const uint64_t mid = (lower + upper) / 2;
+    const float tm     = times[i0 + mid];
+    const uint64_t nx1 = i0 + mid + 1 < i1 ? i0 + mid + 1 : i1 - 1;
+    if (time >= tm && time <= times[nx1]) {
+      lower = i0 + mid;
+      upper = lower + 1 < i1 ? lower + 1 : i1 - 1;
+      found = true;
+      break;
+    } else if (time < tm) {
+      upper = mid;
+    } else {
+      lower = mid;
+    }
+  }
+  if (!found) return __int_as_float(0x7fc00000);
+  const float tl = times[lower], th = times[upper];
+  const float w  = (time - tl) * (1.f / (th - tl));
+  return (1.f - w) * toFloat(data[lower]) + w * toFloat(data[upper]);
+}
+
 // structuredRegular with temporal data (temporal_data_interpolation.ih:13-152,
 // VdbSampler_dense.ih:360-560): every voxel read interpolates that voxel's
 // time samples at the source's current time (setTime; 0 by default, as for
@@ -413,60 +472,12 @@ struct TemporalDenseSource
 
   VKL_DEV void setTime(float t) { time = t; }
 
-  // interpolateTemporallyStructured / interpolateTemporallyUnstructured for
-  // the voxel with linear index v.
+  // The voxel with linear index v at the current time.
   VKL_DEV float voxel(uint64_t v) const
   {
-    if (format == 1u) {
-      // Time is clamped to [0, 1] (NaN to 0), which keeps both reads among
-      // the voxel's own samples; inside [0, 1] this is the reference's
-      // arithmetic.
-      const float tc      = vmin(vmax(time, 0.f), 1.f);
-      const float tf      = tc * (float)(numTimesteps - 1);
-      const uint32_t ts   = (uint32_t)tf;
-      const float w       = tf - (float)ts;
-      const uint32_t next = (w == 0.f) ? ts : ts + 1;
-      const uint64_t base = (uint64_t)numTimesteps * v;
-      return (1.f - w) * toFloat(data[base + ts]) + w * toFloat(data[base + next]);
-    }
-    const uint64_t i0 = idx64 ? idx64[v] : (uint64_t)idx32[v];
-    const uint64_t i1 = idx64 ? idx64[v + 1] : (uint64_t)idx32[v + 1];
-    const float t0    = times[i0];
-    if (time <= t0) return toFloat(data[i0]);
-    const float t1 = times[i1 - 1];
-    if (time >= t1) return toFloat(data[i1 - 1]);
-    if (isnan(time)) return time;
-    const uint64_t n = i1 - i0;
-    if (n == 2) {
-      const float w = (time - t0) * (1.f / (t1 - t0));
-      return (1.f - w) * toFloat(data[i0]) + w * toFloat(data[i1 - 1]);
-    }
-    // Binary search for [times[lower], times[upper]] around time. Commit
-    // checked that the voxel's times increase strictly, so with
-    // t0 < time < t1 every read stays inside the voxel's range and the
-    // search ends within log2(n) + 1 rounds; the bound and the clamp only
-    // guard the loop.
-    uint64_t lower = 0, upper = n;
-    bool found     = false;
-    for (int it = 0; it < 66 && lower <= upper; ++it) {
-      const uint64_t mid = (lower + upper) / 2;
-      const float tm     = times[i0 + mid];
-      const uint64_t nx1 = i0 + mid + 1 < i1 ? i0 + mid + 1 : i1 - 1;
-      if (time >= tm && time <= times[nx1]) {
-        lower = i0 + mid;
-        upper = lower + 1 < i1 ? lower + 1 : i1 - 1;
-        found = true;
-        break;
-      } else if (time < tm) {
-        upper = mid;
-      } else {
-        lower = mid;
-      }
-    }
-    if (!found) return __int_as_float(0x7fc00000);
-    const float tl = times[lower], th = times[upper];
-    const float w  = (time - tl) * (1.f / (th - tl));
-    return (1.f - w) * toFloat(data[lower]) + w * toFloat(data[upper]);
+    if (format == 1u) return temporallyStructured(data, numTimesteps, v, time);
+    if (idx64) return temporallyUnstructured(data, idx64, times, v, time);
+    return temporallyUnstructured(data, idx32, times, v, time);
   }
 
   VKL_DEV float at(int32_t x, int32_t y, int32_t z) const
@@ -484,6 +495,77 @@ struct TemporalDenseSource
     for (int i = 0; i < 8; i++) s[i] = at(ic.x + ((i >> 2) & 1), ic.y + ((i >> 1) & 1), ic.z + (i & 1));
   }
 };
+
+// vdb with temporal leaves (VdbSampler_denseZYX.ih / VdbSampler_tile.ih,
+// structured and unstructured sampling): the node table is traversed as for
+// constant leaves; a leaf pointer's temporal format selects how its voxel is
+// read at the source's current time. The traversal records leaf accesses
+// when the sampler is observed.
+template <typename T>
+struct TemporalSparseSource
+{
+  SparseSource<T, true> nodes;
+  const T *data;
+  const uint64_t *dataOff, *idxOff, *timesOff, *idx;
+  const uint32_t *steps;
+  const float *times;
+  float time;
+
+  VKL_DEV TemporalSparseSource(const GridView &g, const AttrView &a, uint32_t attributeIndex)
+      : nodes(g, a, attributeIndex),
+        data((const T *)a.data),
+        dataOff(g.leafDataOffset),
+        idxOff(g.leafIdxOffset),
+        timesOff(g.leafTimesOffset),
+        idx((const uint64_t *)g.tuIndices),
+        steps(g.leafSteps),
+        times(g.tuTimes),
+        time(0.f)
+  {
+    nodes.cells = nullptr;
+  }
+
+  VKL_DEV void setTime(float t) { time = t; }
+
+  // Leaf voxel (ox, oy, oz) of the leaf pointer v (tiles: their one voxel).
+  VKL_DEV float leafValue(uint64_t v, uint32_t ox, uint32_t oy, uint32_t oz) const
+  {
+    const uint64_t leaf = v >> 6;
+    const uint32_t tf   = (uint32_t)(v >> 4) & 3u;
+    const uint64_t vi   = (((v >> 2) & 3u) == 1u) ? (((ox & 7u) << 6) | ((oy & 7u) << 3) | (oz & 7u)) : 0u;
+    const T *d          = data + dataOff[leaf];
+    if (tf == 1u) return temporallyStructured(d, steps[leaf], vi, time);
+    if (tf == 2u) return temporallyUnstructured(d, idx + idxOff[leaf], times + timesOff[leaf], vi, time);
+    return toFloat(d[vi]);
+  }
+
+  VKL_DEV float at(int32_t x, int32_t y, int32_t z) const
+  {
+    const uint32_t ox = (uint32_t)(x - nodes.root[0]), oy = (uint32_t)(y - nodes.root[1]),
+                   oz = (uint32_t)(z - nodes.root[2]);
+    if (!nodes.inDomain(ox, oy, oz)) return nodes.bg;
+    const Range1f *er = nullptr;
+    uint32_t eo       = 0;
+    const uint64_t v  = nodes.traverse(ox, oy, oz, er, eo);
+    const uint32_t tag = (uint32_t)v & 3u;
+    if (tag == 3u) return leafValue(v, ox, oy, oz);
+    if (tag == 1u) return nodes.errorValue(er, eo);
+    return 0.f;
+  }
+
+  VKL_DEV void corners8(I3 ic, float *s) const
+  {
+#pragma unroll 1
+    for (int i = 0; i < 8; i++) s[i] = at(ic.x + ((i >> 2) & 1), ic.y + ((i >> 1) & 1), ic.z + (i & 1));
+  }
+};
+
+template <typename T>
+VKL_DEV void cubicTaps(const TemporalSparseSource<T> &src, I3 ic, float *S)
+{
+#pragma unroll 1
+  for (int i = 0; i < 64; i++) S[i] = src.at(ic.x + (i >> 4) - 1, ic.y + ((i >> 2) & 3) - 1, ic.z + (i & 3) - 1);
+}
 
 // Whether dense offsets (including brick padding) can exceed 32 bits.
 VKL_DEV bool denseNeeds64(const GridView &g)

@@ -107,6 +107,15 @@ struct GridView
   const void *tuIndices;
   uint32_t tuIndex64;
   const float *tuTimes;
+  // vdb with temporal leaves (temporalFormat != 0 on a sparse grid): every
+  // leaf's elements in one array per attribute, leaf i at leafDataOffset[i];
+  // the leaf's temporal format is in its voxel code; structured leaves have
+  // leafSteps[i] samples per voxel; unstructured leaves index tuIndices
+  // (uint64) from leafIdxOffset[i] and tuTimes from leafTimesOffset[i].
+  const uint64_t *leafDataOffset;
+  const uint32_t *leafSteps;
+  const uint64_t *leafIdxOffset;
+  const uint64_t *leafTimesOffset;
 };
 
 // Sampler = grid + filters.

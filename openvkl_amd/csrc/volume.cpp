@@ -30,7 +30,8 @@ uint64_t linearIndex(uint32_t l, uint32_t x, uint32_t y, uint32_t z)
 }
 
 inline uint64_t childCode(uint64_t idx) { return (idx << 2) | 2u; }
-inline uint64_t leafCode(uint64_t idx, uint32_t fmt) { return (idx << 6) + (((uint64_t)fmt & 3u) << 2) + 3u; }
+// fmt: data format in bits 0-1, temporal format in bits 2-3.
+inline uint64_t leafCode(uint64_t idx, uint32_t fmt) { return (idx << 6) + (((uint64_t)fmt & 15u) << 2) + 3u; }
 
 struct NodeTable
 {
@@ -219,16 +220,6 @@ uint64_t verifyTemporalData(Device &device, uint64_t numVoxels, uint32_t format,
       throw VklError("Values in temporallyUnstructuredTimes must be bounded by 0.0 and 1.0 for every voxel.");
   }
   return end;
-}
-
-void rejectTemporal(const ManagedObject &o)
-{
-  static const char *names[] = {"temporallyStructuredNumTimesteps", "temporallyUnstructuredIndices",
-                                "temporallyUnstructuredTimes", "node.temporallyStructuredNumTimesteps",
-                                "node.temporallyUnstructuredIndices", "node.temporallyUnstructuredTimes"};
-  for (const char *n : names)
-    if (o.hasParam(n) && !(std::string(n) == "temporallyStructuredNumTimesteps" && o.getInt(n, 0) == 0))
-      throw VklError(std::string("temporal data (") + n + ") is not supported by the hip device");
 }
 
 }  // namespace
@@ -535,7 +526,6 @@ void VdbVolume::commit()
   buffers.clear();
   committed = false;
   commitCommonParams();
-  rejectTemporal(*this);
 
   Data *nodeData = getData("node.data");
   Data *packedD  = getData("nodesPackedDense");
@@ -556,8 +546,26 @@ void VdbVolume::commit()
     std::memcpy(origin.data(), b.data(), 12 * n);
   }
   std::vector<uint32_t> format = formatD->hostArray<uint32_t>();
-  if (formatD->numItems != n || (nodeData && nodeData->numItems != n))
+  // Motion blur per node (VdbVolume.cpp:441-457, 710-747): node.temporalFormat
+  // (default constant) with per-node numbers of time steps or unstructured
+  // index / time arrays; node.data only.
+  Data *tfD    = getData("node.temporalFormat");
+  Data *stepsD = getData("node.temporallyStructuredNumTimesteps");
+  Data *tuIdxD = getData("node.temporallyUnstructuredIndices");
+  Data *tuTmD  = getData("node.temporallyUnstructuredTimes");
+  std::vector<uint32_t> tformat = tfD ? tfD->hostArray<uint32_t>() : std::vector<uint32_t>(n, 0u);
+  if (formatD->numItems != n || (nodeData && nodeData->numItems != n) || tformat.size() != n)
     throw VklError("node.level, node.origin, node.format, node.temporalFormat, and node.data must all have the same size");
+  if (stepsD && stepsD->numItems != n)
+    throw VklError("If node.temporallyStructuredNumTimesteps is set, it must have the same size as node.data");
+  if (tuIdxD && tuIdxD->numItems != n)
+    throw VklError("If node.temporallyUnstructuredIndices is set, it must have the same size as node.data");
+  if (tuTmD && tuTmD->numItems != n)
+    throw VklError("If node.temporallyUnstructuredTimes is set, it must have the same size as node.data");
+  const std::vector<int32_t> steps = stepsD ? stepsD->hostArray<int32_t>() : std::vector<int32_t>(n, 0);
+  bool anyTemporal = false;
+  for (uint32_t f : tformat) anyTemporal |= f != VKL_TEMPORAL_FORMAT_CONSTANT;
+  if (anyTemporal && packedD) throw VklError("only temporally constant data is supported for packed dense / tile data");
 
   // verifyLevel / verifyNodeDataFormat (VdbVolume.cpp:536-563).
   uint64_t numDense = 0, numTile = 0;
@@ -605,17 +613,46 @@ void VdbVolume::commit()
   } else {
     // node.data: per-leaf arrays (or per-leaf VKL_DATA of per-attribute
     // arrays), gathered into packed device arrays plus a leaf -> slot table.
+    // With temporal leaves every leaf's elements go, in input order, into one
+    // array per attribute (leaf i at leafDataOffset[i]), and the unstructured
+    // index / time arrays into two flat arrays.
     Data *first          = nodeData->item(0);
     const bool multi     = first->dataType == VKL_DATA;
     const uint32_t na    = multi ? (uint32_t)first->numItems : 1;
     for (uint32_t a = 0; a < na; a++) types.push_back(multi ? first->item(a)->dataType : first->dataType);
     std::vector<std::vector<uint8_t>> dense(na), tile(na);
+    std::vector<uint64_t> dataOff(n), idxOff(n, 0), timesOff(n, 0), flatIdx;
+    std::vector<uint32_t> leafSteps(n, 0);
+    std::vector<float> flatTimes;
+    uint64_t dataTotal = 0;
     for (uint64_t i = 0; i < n; i++) {
       Data *ld = nodeData->item(i);
       if ((ld->dataType == VKL_DATA) != multi) throw VklError("All nodes must have the same VKLDataType in vdb volumes.");
+      const uint64_t voxels = format[i] == VKL_FORMAT_TILE ? 1 : 512;
+      uint64_t expected     = voxels;
+      if (anyTemporal) {
+        Data *ti = tuIdxD ? tuIdxD->item(i) : nullptr;
+        Data *tt = tuTmD ? tuTmD->item(i) : nullptr;
+        expected = verifyTemporalData(*device, voxels, tformat[i], steps[i], ti, tt);
+        dataOff[i] = dataTotal;
+        dataTotal += expected;
+        if (tformat[i] == VKL_TEMPORAL_FORMAT_STRUCTURED) leafSteps[i] = (uint32_t)steps[i];
+        if (tformat[i] == VKL_TEMPORAL_FORMAT_UNSTRUCTURED) {
+          idxOff[i]   = flatIdx.size();
+          timesOff[i] = flatTimes.size();
+          if (ti->dataType == VKL_UINT) {
+            const std::vector<uint32_t> v = ti->hostArray<uint32_t>();
+            flatIdx.insert(flatIdx.end(), v.begin(), v.end());
+          } else {
+            const std::vector<uint64_t> v = ti->hostArray<uint64_t>();
+            flatIdx.insert(flatIdx.end(), v.begin(), v.end());
+          }
+          const std::vector<float> tv = tt->hostArray<float>();
+          flatTimes.insert(flatTimes.end(), tv.begin(), tv.end());
+        }
+      }
       for (uint32_t a = 0; a < na; a++) {
         Data *ad = multi ? ld->item(a) : ld;
-        const uint64_t expected = format[i] == VKL_FORMAT_TILE ? 1 : 512;
         if (ad->numItems < expected) {
           std::ostringstream ss;
           ss << "Node data too small: found " << ad->numItems << " elements, but expected " << expected;
@@ -632,9 +669,24 @@ void VdbVolume::commit()
           throw VklError(ss.str());
         }
         std::vector<uint8_t> h = ad->hostCopy();
-        auto &dst = format[i] == VKL_FORMAT_TILE ? tile[a] : dense[a];
+        auto &dst = (format[i] == VKL_FORMAT_TILE && !anyTemporal) ? tile[a] : dense[a];
         dst.insert(dst.end(), h.begin(), h.end());
       }
+    }
+    grid.temporalFormat = anyTemporal ? 1u : 0u;
+    if (anyTemporal) {
+      auto up = [&](const void *src, size_t bytes) {
+        DeviceBuffer *b = newBuffer(std::max<size_t>(bytes, 8));
+        if (bytes) b->upload(src, bytes, st);
+        return b->ptr;
+      };
+      grid.leafDataOffset  = static_cast<const uint64_t *>(up(dataOff.data(), n * 8));
+      grid.leafSteps       = static_cast<const uint32_t *>(up(leafSteps.data(), n * 4));
+      grid.leafIdxOffset   = static_cast<const uint64_t *>(up(idxOff.data(), n * 8));
+      grid.leafTimesOffset = static_cast<const uint64_t *>(up(timesOff.data(), n * 8));
+      grid.tuIndices       = up(flatIdx.data(), flatIdx.size() * 8);
+      grid.tuIndex64       = 1;
+      grid.tuTimes         = static_cast<const float *>(up(flatTimes.data(), flatTimes.size() * 4));
     }
     for (uint32_t a = 0; a < na; a++) {
       AttrView av{};
@@ -704,13 +756,18 @@ void VdbVolume::commit()
   // Leaf codes carry the packed index for packed input, else the input index.
   std::vector<uint64_t> leafId(n);
   for (uint64_t i = 0; i < n; i++) leafId[i] = packedD ? slot[i] : i;
-  finishGrid(*this, level, format, offs, leafId, st);
+  // Voxel codes carry the data format and the temporal format.
+  std::vector<uint32_t> codeFormat(n);
+  for (uint64_t i = 0; i < n; i++) codeFormat[i] = (format[i] & 3u) | ((tformat[i] & 3u) << 2);
+  finishGrid(*this, level, codeFormat, offs, leafId, st);
 
   // "cornerPacked" (bool): a leaf index grid over the domain plus, per dense
   // leaf, 512 cell records of the 8 corner values the sampler reads (8 floats
   // per cell; baked for this volume's maxSamplingDepth).
   grid.leafIndex   = nullptr;
   grid.packedDepth = 0xffffffffu;
+  if (getBool("cornerPacked", false) && anyTemporal)
+    throw VklError("cornerPacked is not supported for temporal data", VKL_INVALID_ARGUMENT);
   if (getBool("cornerPacked", false)) {
     for (int c = 0; c < 3; c++) grid.leafDims[c] = (grid.activeSize[c] + 7) / 8;
     const uint64_t cellsOfGrid = (uint64_t)grid.leafDims[0] * grid.leafDims[1] * grid.leafDims[2];

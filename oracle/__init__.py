@@ -75,6 +75,9 @@ def lib():
         L.oracle_dense_new_temporal.restype = vp
         L.oracle_dense_new_temporal.argtypes = [vp, vp, vp, i32, u32, vp, vp, vp, u32, u32, vp, vp, u64]
         L.oracle_sample_times.argtypes = [vp, u64, vp, vp, vp, u32]
+        L.oracle_vdb_new_temporal.restype = vp
+        L.oracle_vdb_new_temporal.argtypes = [u64, vp, vp, vp, u32, vp, i32, vp, vp, vp, vp, vp, vp,
+                                              vp, vp, vp, vp, vp]
         L.oracle_gradient_times.argtypes = [vp, u64, vp, vp, vp, u32]
         L.oracle_inner_nodes.restype = u64
         L.oracle_inner_nodes.argtypes = [vp, i32, vp, u64]
@@ -152,6 +155,36 @@ class Grid:
         h = lib().oracle_dense_new_temporal(_p(d), _p(io), _p(m), int(cell_centered), len(arrs), _p(dt), ptrs, _p(bg),
                                             fmt, steps, _p(idx), _p(tms), 0 if tms is None else len(tms))
         return cls(h, (arrs, d, io, m, dt, ptrs, bg, idx, tms))
+
+    @classmethod
+    def vdb_temporal(cls, level, origin, fmt, leaf_data, temporal_format, steps=None, tu_indices=None,
+                     tu_times=None, i2o=None, background=None):
+        """vdb with per-node motion blur (node.data form). temporal_format: per
+        node 0 / 1 / 2; steps: per node time steps (structured nodes);
+        tu_indices / tu_times: per node arrays (None for other nodes)."""
+        level = np.ascontiguousarray(level, np.uint32)
+        origin = np.ascontiguousarray(origin, np.int32).reshape(-1, 3)
+        fmt = np.ascontiguousarray(fmt, np.uint32)
+        n = level.shape[0]
+        m = np.asarray(i2o, np.float32) if i2o is not None else affine_from_origin_spacing((0, 0, 0), (1, 1, 1))
+        ld = [[np.ascontiguousarray(a) for a in leaf] for leaf in leaf_data]
+        na = len(ld[0])
+        dt = np.array([DTYPE_CODE[a.dtype] for a in ld[0]], np.uint32)
+        ldp = (C.c_void_p * (n * na))(*[a.ctypes.data for leaf in ld for a in leaf])
+        tf = np.ascontiguousarray(temporal_format, np.uint32)
+        st = np.ascontiguousarray(steps if steps is not None else np.zeros(n), np.uint32)
+        ix = [None if a is None else np.ascontiguousarray(a, np.uint64) for a in (tu_indices or [None] * n)]
+        tm = [None if a is None else np.ascontiguousarray(a, np.float32) for a in (tu_times or [None] * n)]
+        ixp = (C.c_void_p * n)(*[None if a is None else a.ctypes.data for a in ix])
+        tmp = (C.c_void_p * n)(*[None if a is None else a.ctypes.data for a in tm])
+        cnt = np.array([0 if a is None else a.size for a in tm], np.uint64)
+        bg = None if background is None else np.asarray(background, np.float32).reshape(-1)
+        keep = [level, origin, fmt, m, ld, ldp, dt, tf, st, ix, tm, ixp, tmp, cnt, bg]
+        h = lib().oracle_vdb_new_temporal(n, _p(level), _p(origin), _p(fmt), na, _p(dt), 0, None, None, ldp, _p(m),
+                                          _p(bg), None, _p(tf), _p(st), ixp, tmp, _p(cnt))
+        g = cls(h, keep)
+        g.num_leaves = n
+        return g
 
     @classmethod
     def vdb(cls, level, origin, fmt, packed_dense=None, packed_tile=None, leaf_data=None,

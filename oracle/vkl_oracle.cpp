@@ -275,6 +275,12 @@ struct OracleGrid
   std::vector<uint64_t> tuIndices;
   std::vector<float> tuTimes;
   mutable float curTime = 0.f;
+
+  // vdb temporal leaves (node.data only): per input node its temporal
+  // format, structured time steps and unstructured index / time arrays.
+  std::vector<uint32_t> leafTF, leafSteps;
+  std::vector<std::vector<uint64_t>> leafIdx;
+  std::vector<std::vector<float>> leafTimes;
 };
 
 namespace {
@@ -344,30 +350,29 @@ inline bool isInDomain(V3u size, V3u o)
 // --- dense sampling (VdbSampler_traverseAndSample.ih:314-349,
 //     VdbSampler_dense.ih:290-330) ---------------------------------------------
 
-// interpolateTemporallyStructured, temporal_data_interpolation.ih:13-45, with
-// baseIdx = numTimesteps * voxel (VdbSampler_dense.ih:360-446).
-float temporallyStructured(const OracleGrid &g, uint32_t attr, uint64_t voxel, float time)
+// interpolateTemporallyStructured, temporal_data_interpolation.ih:13-45:
+// the voxel's samples start at baseIdx of data (element type ty).
+float interpStructured(const uint8_t *d, uint32_t ty, uint32_t numTimesteps, uint64_t baseIdx, float time)
 {
-  const float timestepFloat = time * ((float)(g.numTimesteps - 1));
+  const float timestepFloat = time * ((float)(numTimesteps - 1));
   const uint32_t timestep   = (uint32_t)timestepFloat;
   const float w             = timestepFloat - ((float)timestep);
   const uint32_t next       = (w == 0.f) ? timestep : timestep + 1;
-  const uint64_t base       = (uint64_t)g.numTimesteps * voxel;
-  const float v0 = fetch(g.denseData[attr].data(), g.attrTypes[attr], base + timestep);
-  const float v1 = fetch(g.denseData[attr].data(), g.attrTypes[attr], base + next);
+  const float v0 = fetch(d, ty, baseIdx + timestep);
+  const float v1 = fetch(d, ty, baseIdx + next);
   return (1.f - w) * v0 + w * v1;
 }
 
-// interpolateTemporallyUnstructured, temporal_data_interpolation.ih:82-152.
+// interpolateTemporallyUnstructured, temporal_data_interpolation.ih:82-152:
+// voxel v owns positions [indices[v], indices[v + 1]) of data and times.
 // rcp is taken as the correctly rounded 1 / x.
-float temporallyUnstructured(const OracleGrid &g, uint32_t attr, uint64_t voxel, float time)
+float interpUnstructured(const uint8_t *d, uint32_t ty, const uint64_t *indices, const float *times, uint64_t voxel,
+                         float time)
 {
-  const uint8_t *d  = g.denseData[attr].data();
-  const uint32_t ty = g.attrTypes[attr];
-  const uint64_t dataIdx = g.tuIndices[voxel], dataNextIdx = g.tuIndices[voxel + 1];
-  const float t0 = g.tuTimes[dataIdx];
+  const uint64_t dataIdx = indices[voxel], dataNextIdx = indices[voxel + 1];
+  const float t0 = times[dataIdx];
   if (time <= t0) return fetch(d, ty, dataIdx);
-  const float t1 = g.tuTimes[dataNextIdx - 1];
+  const float t1 = times[dataNextIdx - 1];
   if (time >= t1) return fetch(d, ty, dataNextIdx - 1);
   const uint64_t numTimeSamples = dataNextIdx - dataIdx;
   if (numTimeSamples == 2) {
@@ -377,8 +382,8 @@ float temporallyUnstructured(const OracleGrid &g, uint32_t attr, uint64_t voxel,
   uint64_t lower = 0, upper = numTimeSamples;
   while (lower <= upper) {
     const uint64_t mid   = (lower + upper) / 2;
-    const float timesMid = g.tuTimes[dataIdx + mid];
-    if (time >= timesMid && time <= g.tuTimes[dataIdx + mid + 1]) {
+    const float timesMid = times[dataIdx + mid];
+    if (time >= timesMid && time <= times[dataIdx + mid + 1]) {
       lower = dataIdx + mid;
       upper = lower + 1;
       break;
@@ -388,9 +393,22 @@ float temporallyUnstructured(const OracleGrid &g, uint32_t attr, uint64_t voxel,
       lower = mid;
     }
   }
-  const float timesLow = g.tuTimes[lower], timesHigh = g.tuTimes[upper];
+  const float timesLow = times[lower], timesHigh = times[upper];
   const float w        = (time - timesLow) * (1.f / (timesHigh - timesLow));
   return (1.f - w) * fetch(d, ty, lower) + w * fetch(d, ty, upper);
+}
+
+// VdbSampler_dense.ih:360-446: structured base index numTimesteps * voxel.
+float temporallyStructured(const OracleGrid &g, uint32_t attr, uint64_t voxel, float time)
+{
+  return interpStructured(g.denseData[attr].data(), g.attrTypes[attr], g.numTimesteps,
+                          (uint64_t)g.numTimesteps * voxel, time);
+}
+
+float temporallyUnstructured(const OracleGrid &g, uint32_t attr, uint64_t voxel, float time)
+{
+  return interpUnstructured(g.denseData[attr].data(), g.attrTypes[attr], g.tuIndices.data(), g.tuTimes.data(),
+                            voxel, time);
 }
 
 inline float denseCorner(const OracleGrid &g, uint32_t attr, V3i ic)
@@ -438,8 +456,38 @@ float leafValue(const OracleGrid &g, uint64_t voxel, V3u o, uint32_t attr)
     return fetch(g.packedTile[attr].data(), t, leaf);
   }
   // VdbSampler_denseZYX.ih:124-183, VdbSampler_tile.ih:82-120
-  const auto &d = g.leafData[leaf * g.numAttributes + attr];
-  return fetch(d.data(), t, fmt == kFormatDense ? lin : 0);
+  const auto &d      = g.leafData[leaf * g.numAttributes + attr];
+  const uint64_t vi  = fmt == kFormatDense ? lin : 0;
+  const uint32_t tfm = (uint32_t)(voxel >> 4) & 3u;  // leaf temporal format
+  if (tfm == 1)  // VdbSampler_denseZYX.ih:191-260: base numTimesteps * voxel
+    return interpStructured(d.data(), t, g.leafSteps[leaf], (uint64_t)g.leafSteps[leaf] * vi, g.curTime);
+  if (tfm == 2)
+    return interpUnstructured(d.data(), t, g.leafIdx[leaf].data(), g.leafTimes[leaf].data(), vi, g.curTime);
+  return fetch(d.data(), t, vi);
+}
+
+// Value range of one voxel of a temporal leaf: every time sample
+// (VdbSampler_denseZYX.ih:45-118).
+R1f temporalVoxelRange(const OracleGrid &g, uint64_t voxel, uint64_t vi, uint32_t attr)
+{
+  const uint64_t leaf = leafIndexOf(voxel);
+  const uint32_t tfm  = (uint32_t)(voxel >> 4) & 3u;
+  const auto &d       = g.leafData[leaf * g.numAttributes + attr];
+  uint64_t b = vi, e = vi + 1;
+  if (tfm == 1) {
+    b = vi * g.leafSteps[leaf];
+    e = b + g.leafSteps[leaf];
+  } else if (tfm == 2) {
+    b = g.leafIdx[leaf][vi];
+    e = g.leafIdx[leaf][vi + 1];
+  }
+  R1f r = emptyRange();
+  for (uint64_t k = b; k < e; ++k) {
+    const float v = fetch(d.data(), g.attrTypes[attr], k);
+    r.lower = fmin_(r.lower, v);
+    r.upper = fmax_(r.upper, v);
+  }
+  return r;
 }
 
 // VdbSampler_sample_inner, VdbSampler_traverseAndSample.ih:144-197.
@@ -700,7 +748,8 @@ void buildLevels(OracleGrid &g, const LeafInput &in, const std::vector<V3u> &off
             g.levels[l].voxels[v] = makeChild(child);  // re-index: addNode may reallocate
             nodeIndex = child;
           } else {
-            g.levels[l].voxels[v] = makeLeaf(g.packed ? packedIndex[idx] : idx, in.format[idx], 0);
+            g.levels[l].voxels[v] = makeLeaf(g.packed ? packedIndex[idx] : idx, in.format[idx],
+                                             g.leafTF.empty() ? 0 : g.leafTF[idx]);
           }
         } else {
           nodeIndex = voxel >> 2;
@@ -747,6 +796,17 @@ R1f nodeValueRange(const OracleGrid &g, uint64_t voxel, V3u o, const uint32_t rx
   if (!voxelIsLeaf(voxel)) {
     if (!isInDomain(g.activeSize, o)) return {bgOut, bgOut};
     return {0.f, 0.f};
+  }
+  if (!g.packed && ((voxel >> 4) & 3u) != 0) {
+    if (leafFormatOf(voxel) == kFormatTile) return temporalVoxelRange(g, voxel, 0, attr);
+    for (uint32_t x = rx[0]; x < rx[1]; ++x)
+      for (uint32_t y = ry[0]; y < ry[1]; ++y)
+        for (uint32_t z = rz[0]; z < rz[1]; ++z) {
+          const R1f v = temporalVoxelRange(g, voxel, ((uint64_t)x << 6) + (y << 3) + z, attr);
+          r.lower = fmin_(r.lower, v.lower);
+          r.upper = fmax_(r.upper, v.upper);
+        }
+    return r;
   }
   if (leafFormatOf(voxel) == kFormatTile) {
     const float v = leafValue(g, voxel, V3u{0, 0, 0}, attr);
@@ -1357,6 +1417,19 @@ OracleGrid *oracle_vdb_new(uint64_t numLeaves, const uint32_t *level, const int3
                            const void *const *leafData, const float i2o[12], const float *background,
                            const int32_t *clip)
 {
+  return oracle_vdb_new_temporal(numLeaves, level, origin, format, numAttributes, dtypes, packed, packedDense,
+                                 packedTile, leafData, i2o, background, clip, nullptr, nullptr, nullptr, nullptr,
+                                 nullptr);
+}
+
+OracleGrid *oracle_vdb_new_temporal(uint64_t numLeaves, const uint32_t *level, const int32_t *origin,
+                                    const uint32_t *format, uint32_t numAttributes, const uint32_t *dtypes,
+                                    int packed, const void *const *packedDense, const void *const *packedTile,
+                                    const void *const *leafData, const float i2o[12], const float *background,
+                                    const int32_t *clip, const uint32_t *leafTF, const uint32_t *leafSteps,
+                                    const uint64_t *const *leafIdx, const float *const *leafTimes,
+                                    const uint64_t *leafTimeCounts)
+{
   try {
     if (numLeaves == 0) throw std::runtime_error("Vdb volumes must have at least one leaf node.");
     auto *g = new OracleGrid();
@@ -1396,10 +1469,28 @@ OracleGrid *oracle_vdb_new(uint64_t numLeaves, const uint32_t *level, const int3
           g->packedTile.emplace_back();
       }
     }
+    if (leafTF) {
+      g->leafTF.assign(leafTF, leafTF + numLeaves);
+      g->leafSteps.assign(numLeaves, 0);
+      g->leafIdx.resize(numLeaves);
+      g->leafTimes.resize(numLeaves);
+      for (uint64_t i = 0; i < numLeaves; i++) {
+        const uint64_t voxels = format[i] == kFormatDense ? 512 : 1;
+        if (leafTF[i] == 1) g->leafSteps[i] = leafSteps[i];
+        if (leafTF[i] == 2) {
+          g->leafIdx[i].assign(leafIdx[i], leafIdx[i] + voxels + 1);
+          g->leafTimes[i].assign(leafTimes[i], leafTimes[i] + leafTimeCounts[i]);
+        }
+      }
+    }
     if (!g->packed) {
       for (uint64_t i = 0; i < numLeaves; i++)
         for (uint32_t a = 0; a < numAttributes; a++) {
-          const size_t n = (format[i] == kFormatDense ? 512 : 1) * typeSize(dtypes[a]);
+          const uint64_t voxels = format[i] == kFormatDense ? 512 : 1;
+          uint64_t items        = voxels;
+          if (leafTF && leafTF[i] == 1) items = voxels * leafSteps[i];
+          if (leafTF && leafTF[i] == 2) items = g->leafIdx[i][voxels];
+          const size_t n = items * typeSize(dtypes[a]);
           const uint8_t *p = (const uint8_t *)leafData[i * numAttributes + a];
           g->leafData.emplace_back(p, p + n);
         }

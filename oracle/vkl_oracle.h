@@ -71,6 +71,29 @@ OracleGrid *oracle_vdb_new(uint64_t numLeaves,
                            const float *background,
                            const int32_t *clip);
 
+/* vdb with temporal leaves (leafData only): per leaf its temporal format
+ * (0 constant, 1 structured, 2 unstructured), structured time steps,
+ * unstructured index array (voxels + 1 uint64) and time array (leafTimeCounts
+ * floats); leafData[leaf * numAttributes + a] holds the leaf's samples. */
+OracleGrid *oracle_vdb_new_temporal(uint64_t numLeaves,
+                                    const uint32_t *level,
+                                    const int32_t *origin,
+                                    const uint32_t *format,
+                                    uint32_t numAttributes,
+                                    const uint32_t *dtypes,
+                                    int packed,
+                                    const void *const *packedDense,
+                                    const void *const *packedTile,
+                                    const void *const *leafData,
+                                    const float i2o[12],
+                                    const float *background,
+                                    const int32_t *clip,
+                                    const uint32_t *leafTF,
+                                    const uint32_t *leafSteps,
+                                    const uint64_t *const *leafIdx,
+                                    const float *const *leafTimes,
+                                    const uint64_t *leafTimeCounts);
+
 void oracle_free(OracleGrid *g);
 const char *oracle_last_error(void);
 

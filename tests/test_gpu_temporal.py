@@ -181,3 +181,80 @@ def test_commit_validation(device):
                  "attribute 0 has 23 elements, but expected 24")
     expect_error(device, lambda: sr(np.zeros(16, np.float32), dims=dims, structured_steps=2, corner_packed=True),
                  "cornerPacked and brickShift are not supported for temporal data")
+
+
+# ---- vdb with temporal leaves --------------------------------------------------------------
+
+
+def vdb_pair(device, fn=V.wavelet_time, dtype=np.float32, seed=V.SEED + 6):
+    nd = V.temporal_vdb_nodes(fn, seed=seed, dtype=dtype)
+    vol = device.vdb(nd["level"], nd["origin"], nd["fmt"], leaf_data=nd["leaf_data"],
+                     temporal_format=nd["temporal_format"], steps=nd["steps"], tu_indices=nd["tu_indices"],
+                     tu_times=nd["tu_times"])
+    grid = O.Grid.vdb_temporal(nd["level"], nd["origin"], nd["fmt"], nd["leaf_data"], nd["temporal_format"],
+                               nd["steps"], nd["tu_indices"], nd["tu_times"])
+    return vol, grid, nd
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float16])
+@pytest.mark.parametrize("filt", [vkl.VKL_FILTER_NEAREST, vkl.VKL_FILTER_LINEAR])
+def test_vdb_temporal_sample_gradient_bitexact(device, dtype, filt):
+    vol, grid, _ = vdb_pair(device, dtype=dtype)
+    grid.set_sampler(filt)
+    pts = V.uniform_points(40000, (-2, -2, -2), (258, 258, 132), seed=41)
+    t = np.random.default_rng(42).random(40000, dtype=np.float32)
+    s = vol.sampler(filter=filt)
+    assert_same(s.sample(torch.from_numpy(pts).cuda(), times=torch.from_numpy(t).cuda()).cpu().numpy(),
+                grid.sample(pts, times=t))
+    if filt == vkl.VKL_FILTER_LINEAR:
+        assert_same(s.gradient(pts[:8000], times=t[:8000]), grid.gradient(pts[:8000], times=t[:8000]))
+
+
+def test_vdb_temporal_node_table_intervals_hits(device):
+    vol, grid, _ = vdb_pair(device, seed=V.SEED + 7)
+    for l in range(3):
+        n, vox, rng_ = vol.level(l)
+        _, ovox, orng = grid.level(l)
+        np.testing.assert_array_equal(vox, ovox)
+        np.testing.assert_array_equal(rng_.view(np.uint32), orng.view(np.uint32))
+    org, d = rays_through(grid.bounding_box(), 64, seed=43)
+    out, cnt = vol.sampler().interval_context().batch(org, d, max_per_ray=256)
+    for r in range(64):
+        want = grid.intervals(org[r], d[r], depth=O.map_depth(0.5), max_out=256)
+        np.testing.assert_array_equal(out[r, :cnt[r]].view(np.uint32), want.view(np.uint32))
+    times = np.random.default_rng(44).random(64, dtype=np.float32)
+    out, cnt = vol.sampler().hit_context([0.5, 1.5]).batch(org, d, times=times, max_per_ray=16)
+    total = 0
+    for r in range(64):
+        want = grid.hits(org[r], d[r], [0.5, 1.5], time=float(times[r]), max_out=16)
+        assert_same(out[r, :cnt[r]], want)
+        total += len(want)
+    assert total > 10
+
+
+def test_vdb_temporal_observer_and_errors(device):
+    vol, grid, nd = vdb_pair(device, seed=V.SEED + 8)
+    s = vol.sampler()
+    obs = s.observer("LeafNodeAccess")
+    acc = grid.observe_leaf_access()
+    pts = V.uniform_points(5000, (0, 0, 0), (200, 200, 128), seed=45)
+    t = np.random.default_rng(46).random(5000, dtype=np.float32)
+    assert_same(s.sample(pts, times=t), grid.sample(pts, times=t))
+    np.testing.assert_array_equal(obs.map(), acc)
+    obs.release()
+    grid.observe_leaf_access(False)
+    n = len(nd["level"])
+    dense = np.concatenate([a[0] for a, f in zip(nd["leaf_data"], nd["temporal_format"]) if f == 0])
+    expect_error(device, lambda: device.vdb(nd["level"], nd["origin"], nd["fmt"], packed_dense=[dense],
+                                            temporal_format=nd["temporal_format"]),
+                 "only temporally constant data is supported for packed dense / tile data")
+    expect_error(device, lambda: device.vdb(nd["level"], nd["origin"], nd["fmt"], leaf_data=nd["leaf_data"],
+                                            temporal_format=nd["temporal_format"], steps=nd["steps"][:-1],
+                                            tu_indices=nd["tu_indices"], tu_times=nd["tu_times"]),
+                 "If node.temporallyStructuredNumTimesteps is set, it must have the same size as node.data")
+    expect_error(device, lambda: device.vdb(nd["level"], nd["origin"], nd["fmt"], leaf_data=nd["leaf_data"],
+                                            temporal_format=nd["temporal_format"], steps=nd["steps"],
+                                            tu_indices=nd["tu_indices"], tu_times=nd["tu_times"],
+                                            corner_packed=True),
+                 "cornerPacked is not supported for temporal data")
+    assert n > 100

@@ -110,3 +110,76 @@ def test_value_range_covers_every_time_sample(kind):
     data, kw = V.temporal_volume((20, 13, 9), V.wavelet_time, kind, seed=11)
     g = O.Grid.dense_temporal((20, 13, 9), data, **kw)
     np.testing.assert_array_equal(g.value_range(), [data.min(), data.max()])
+
+
+# ---- vdb with temporal leaves ------------------------------------------------------------
+
+
+def vdb_pair(fn, **kw):
+    nd = V.temporal_vdb_nodes(fn, **kw)
+    g = O.Grid.vdb_temporal(nd["level"], nd["origin"], nd["fmt"], nd["leaf_data"], nd["temporal_format"],
+                            nd["steps"], nd["tu_indices"], nd["tu_times"])
+    return nd, g
+
+
+def leaf_voxel_points(nd, count, seed):
+    """Voxel positions inside the dense leaves (integer index coordinates)."""
+    rng = np.random.default_rng(seed)
+    dense = np.nonzero(nd["fmt"] == 1)[0]
+    pick = rng.choice(dense, count)
+    return (nd["origin"][pick] + rng.integers(0, 8, (count, 3))).astype(np.float32)
+
+
+def expected_time_linear(nd, pts, t):
+    """Expected sample of a time-linear field at leaf voxel positions: the
+    field at time t, except constant leaves (their values are the field at
+    t = 0) and unstructured voxels with a single time sample (that sample)."""
+    key = {tuple((o // 8).tolist()): i for i, o in enumerate(nd["origin"]) if nd["fmt"][i] == 1}
+    p64 = pts.astype(np.float64)
+    want = V.xyz_time(p64[:, 0], p64[:, 1], p64[:, 2], t)
+    for k, p in enumerate(pts.astype(np.int64)):
+        i = key[tuple((p // 8).tolist())]
+        vi = ((p[0] & 7) << 6) | ((p[1] & 7) << 3) | (p[2] & 7)
+        if nd["temporal_format"][i] == 0:
+            want[k] = V.xyz_time(p64[k, 0], p64[k, 1], p64[k, 2], 0.0)
+        elif nd["temporal_format"][i] == 2:
+            ix = nd["tu_indices"][i]
+            if ix[vi + 1] - ix[vi] == 1:
+                want[k] = nd["leaf_data"][i][0][ix[vi]]
+    return want
+
+
+def test_vdb_time_linear_leaves_on_vertices():
+    """Nearest samples at voxel positions of a time-linear field equal the
+    field at the sample time, whatever each leaf's temporal format."""
+    nd, g = vdb_pair(V.xyz_time)
+    g.set_sampler(O.NEAREST)
+    pts = leaf_voxel_points(nd, 3000, 21)
+    for t in (0.0, 0.25, 0.5, 0.75, 1.0):
+        got = g.sample(pts, times=np.full(len(pts), t, np.float32))
+        np.testing.assert_allclose(got, expected_time_linear(nd, pts, t), rtol=2e-6, atol=1e-3)
+
+
+def test_vdb_end_times_and_ranges():
+    nd, g = vdb_pair(V.wavelet_time, seed=31)
+    g.set_sampler(O.NEAREST)
+    first, last, pts = [], [], []
+    for i in np.nonzero(nd["fmt"] == 1)[0][:40]:
+        d, tf = nd["leaf_data"][i][0], nd["temporal_format"][i]
+        for vi in (0, 77, 511):
+            x, y, z = vi >> 6, (vi >> 3) & 7, vi & 7
+            pts.append(nd["origin"][i] + [x, y, z])
+            if tf == 0:
+                first.append(d[vi]), last.append(d[vi])
+            elif tf == 1:
+                T = nd["steps"][i]
+                first.append(d[vi * T]), last.append(d[vi * T + T - 1])
+            else:
+                ix = nd["tu_indices"][i]
+                first.append(d[ix[vi]]), last.append(d[ix[vi + 1] - 1])
+    pts = np.array(pts, np.float32)
+    np.testing.assert_array_equal(g.sample(pts, times=np.zeros(len(pts), np.float32)), np.array(first, np.float32))
+    np.testing.assert_array_equal(g.sample(pts, times=np.ones(len(pts), np.float32)), np.array(last, np.float32))
+    allv = np.concatenate([a[0] for a in nd["leaf_data"]])
+    lo, hi = g.value_range()
+    assert lo <= allv.min() and hi >= allv.max()

@@ -155,3 +155,59 @@ def xyz_time(x, y, z, t):
 
 def wavelet_time(x, y, z, t):
     return (np.sin(3 * x) + np.sin(3 * y) + np.cos(3 * z)) * (1.0 + t)
+
+
+def temporal_vdb_nodes(fn, seed=SEED + 6, dtype=np.float32, n_leaves=120):
+    """vdb nodes with per-node motion blur (node.data form): dense 8^3 leaves
+    in [0, 128)^3 and two level-2 tiles at x >= 128, each node temporally
+    constant, structured (2..4 steps) or unstructured (per voxel 1..4 sorted
+    times, 0 and 1 at the ends when there are two or more). Values are
+    fn(x, y, z, t) at the voxel index positions. Returns a dict of the node
+    arrays for Device.vdb / Grid.vdb_temporal."""
+    rng = np.random.default_rng(seed)
+    cells = rng.choice(16 ** 3, n_leaves, replace=False)
+    leaf_org = np.stack([cells // 256, (cells // 16) % 16, cells % 16], 1) * 8
+    origins = np.concatenate([leaf_org, [[128, 0, 0], [128, 128, 0]]]).astype(np.int32)
+    n = len(origins)
+    level = np.array([3] * n_leaves + [2, 2], np.uint32)
+    fmt = np.array([1] * n_leaves + [0, 0], np.uint32)
+    tformat = rng.integers(0, 3, n).astype(np.uint32)
+    steps = np.where(tformat == 1, rng.integers(2, 5, n), 0).astype(np.int32)
+    leaf_data, tu_indices, tu_times = [], [], []
+    l = np.arange(8)
+    lx, ly, lz = np.meshgrid(l, l, l, indexing="ij")
+    for i in range(n):
+        o = origins[i]
+        if fmt[i] == 1:
+            px, py, pz = (o[0] + lx.reshape(-1)).astype(np.float64), (o[1] + ly.reshape(-1)).astype(np.float64), \
+                (o[2] + lz.reshape(-1)).astype(np.float64)
+        else:
+            px, py, pz = np.array([o[0] + 64.0]), np.array([o[1] + 64.0]), np.array([o[2] + 64.0])
+        nv = px.size
+        if tformat[i] == 0:
+            leaf_data.append([fn(px, py, pz, 0.0).astype(dtype)])
+            tu_indices.append(None)
+            tu_times.append(None)
+        elif tformat[i] == 1:
+            t = np.linspace(0, 1, steps[i], dtype=np.float32).astype(np.float64)
+            leaf_data.append([fn(px[:, None], py[:, None], pz[:, None], t[None, :]).reshape(-1).astype(dtype)])
+            tu_indices.append(None)
+            tu_times.append(None)
+        else:
+            counts = rng.integers(1, 5, nv)
+            times = []
+            for c in counts:
+                if c == 1:
+                    times.append(rng.random(1, dtype=np.float32))
+                else:
+                    inner = np.sort(rng.choice(np.arange(1, 1000), c - 2, replace=False)).astype(np.float32) / 1000
+                    times.append(np.concatenate([[0.0], inner, [1.0]]).astype(np.float32))
+            times = np.concatenate(times).astype(np.float32)
+            idx = np.zeros(nv + 1, np.uint32)
+            idx[1:] = np.cumsum(counts)
+            rep = np.repeat(np.arange(nv), counts)
+            leaf_data.append([fn(px[rep], py[rep], pz[rep], times.astype(np.float64)).astype(dtype)])
+            tu_indices.append(idx)
+            tu_times.append(times)
+    return dict(level=level, origin=origins, fmt=fmt, leaf_data=leaf_data, temporal_format=tformat, steps=steps,
+                tu_indices=tu_indices, tu_times=tu_times)
