@@ -71,8 +71,22 @@ hipError_t launchIntervalBatch(const IteratorContextView &c, uint32_t numRays, c
 hipError_t launchHitBatch(const IteratorContextView &hostView, const IteratorContextView *c, const AttrView &a,
                           uint32_t numRays, const float *org, const float *dir, const float *tRanges,
                           const float *times, HitOut *out, uint32_t maxOut, uint32_t *count, hipStream_t stream);
-// The same for temporal structuredRegular and for vdb volumes (compiled in
-// kernels_hit_other.hip); launchHitNext / launchHitBatch forward to them.
+// The same for structuredRegular volumes of half / float / double voxels
+// (kernels_hit_dense.hip), for temporal structuredRegular and vdb volumes
+// (kernels_hit_other.hip) and for observed vdb samplers
+// (kernels_hit_observed.hip); launchHitNext / launchHitBatch forward to them.
+hipError_t launchHitNextObserved(const IteratorContextView &hostView, const IteratorContextView *c,
+                                 const AttrView &a, HitState *state, HitOut *out, int *result, hipStream_t stream);
+hipError_t launchHitBatchObserved(const IteratorContextView &hostView, const IteratorContextView *c,
+                                  const AttrView &a, uint32_t numRays, const float *org, const float *dir,
+                                  const float *tRanges, const float *times, HitOut *out, uint32_t maxOut,
+                                  uint32_t *count, hipStream_t stream);
+hipError_t launchHitNextDenseFloat(const IteratorContextView &hostView, const IteratorContextView *c,
+                                   const AttrView &a, HitState *state, HitOut *out, int *result, hipStream_t stream);
+hipError_t launchHitBatchDenseFloat(const IteratorContextView &hostView, const IteratorContextView *c,
+                                    const AttrView &a, uint32_t numRays, const float *org, const float *dir,
+                                    const float *tRanges, const float *times, HitOut *out, uint32_t maxOut,
+                                    uint32_t *count, hipStream_t stream);
 hipError_t launchHitNextOther(const IteratorContextView &hostView, const IteratorContextView *c, const AttrView &a,
                               HitState *state, HitOut *out, int *result, hipStream_t stream);
 hipError_t launchHitBatchOther(const IteratorContextView &hostView, const IteratorContextView *c, const AttrView &a,

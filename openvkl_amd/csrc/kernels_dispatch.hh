@@ -22,11 +22,20 @@ hipError_t withFilter(int filter, const F &f)
 // them over translation units that compile in parallel.
 enum : unsigned
 {
-  kDenseKinds    = 1u,
-  kSparseKinds   = 2u,
-  kTemporalKinds = 4u,
-  kAllKinds      = 7u
+  kDenseIntKinds       = 1u,  // uint8, int16, uint16 voxels
+  kSparseKinds         = 2u,  // vdb, unobserved samplers
+  kTemporalKinds       = 4u,
+  kDenseFloatKinds     = 8u,  // half, float, double voxels
+  kSparseObservedKinds = 16u,
+  kDenseKinds          = kDenseIntKinds | kDenseFloatKinds,
+  kAllKinds            = 31u
 };
+
+template <typename T>
+constexpr unsigned denseKindOf()
+{
+  return std::is_integral<T>::value ? kDenseIntKinds : kDenseFloatKinds;
+}
 
 template <typename T, unsigned kKinds, typename F>
 hipError_t withDense(const GridView &g, int filter, const F &f)
@@ -37,7 +46,7 @@ hipError_t withDense(const GridView &g, int filter, const F &f)
     else
       return hipErrorInvalidValue;
   } else {
-    if constexpr ((kKinds & kDenseKinds) != 0) {
+    if constexpr ((kKinds & denseKindOf<T>()) != 0) {
       const uint64_t nbz = ((uint64_t)g.activeSize[2] + (1u << g.layout.sz) - 1) >> g.layout.sz;
       if (nbz * g.layout.strideZ >= (1ull << 32)) return withFilter<DenseSource<T, true>>(filter, f);
       return withFilter<DenseSource<T, false>>(filter, f);
@@ -72,16 +81,18 @@ hipError_t dispatchSource(const GridView &g, const AttrView &a, int filter, cons
     }
     return hipErrorInvalidValue;
   }
-  if constexpr ((kKinds & kSparseKinds) != 0) {
-    const bool obs = g.leafAccess != nullptr;
-    if (a.type == kTypeHalf)
-      return obs ? withFilter<SparseSource<__half, true>>(filter, f) : withFilter<SparseSource<__half>>(filter, f);
-    if (a.type == kTypeFloat)
-      return obs ? withFilter<SparseSource<float, true>>(filter, f) : withFilter<SparseSource<float>>(filter, f);
-    return hipErrorInvalidValue;
-  } else {
+  if (g.leafAccess) {
+    if constexpr ((kKinds & kSparseObservedKinds) != 0) {
+      if (a.type == kTypeHalf) return withFilter<SparseSource<__half, true>>(filter, f);
+      if (a.type == kTypeFloat) return withFilter<SparseSource<float, true>>(filter, f);
+    }
     return hipErrorInvalidValue;
   }
+  if constexpr ((kKinds & kSparseKinds) != 0) {
+    if (a.type == kTypeHalf) return withFilter<SparseSource<__half>>(filter, f);
+    if (a.type == kTypeFloat) return withFilter<SparseSource<float>>(filter, f);
+  }
+  return hipErrorInvalidValue;
 }
 
 }  // namespace vklhip

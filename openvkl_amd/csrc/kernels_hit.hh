@@ -1,7 +1,9 @@
 // Hit-iteration kernels (one specialisation per voxel source type and
-// filter) and the functors that launch them; shared by kernels_iter.hip
-// (constant-in-time structuredRegular sources) and kernels_hit_other.hip
-// (temporal and vdb sources), so that the two sets compile in parallel.
+// filter) and the functors that launch them; instantiated by kernels_iter.hip
+// (integer structuredRegular voxels), kernels_hit_dense.hip (half / float /
+// double), kernels_hit_other.hip (temporal and vdb sources) and
+// kernels_hit_observed.hip (observed vdb samplers), so that the four sets
+// compile in parallel.
 #pragma once
 
 #include "kernels_dispatch.hh"

@@ -1,0 +1,29 @@
+// Hit iteration over vdb volumes whose sampler has a LeafNodeAccess observer:
+// the kernels of kernels_hit.hh instantiated for the observing sources, in a
+// translation unit of their own so that they compile in parallel with the
+// other hit kernels.
+
+#include "kernels.h"
+#include "kernels_hit.hh"
+
+namespace vklhip {
+
+hipError_t launchHitNextObserved(const IteratorContextView &hostView, const IteratorContextView *c,
+                                 const AttrView &a, HitState *state, HitOut *out, int *result, hipStream_t st)
+{
+  return dispatchSource<kSparseObservedKinds>(hostView.sampler.grid, a, hostView.sampler.filter,
+                                              HitNextLaunch{c, a, state, out, result, st});
+}
+
+hipError_t launchHitBatchObserved(const IteratorContextView &hostView, const IteratorContextView *c,
+                                  const AttrView &a, uint32_t numRays, const float *org, const float *dir,
+                                  const float *tRanges, const float *times, HitOut *out, uint32_t maxOut,
+                                  uint32_t *count, hipStream_t st)
+{
+  if (numRays == 0) return hipSuccess;
+  return dispatchSource<kSparseObservedKinds>(hostView.sampler.grid, a, hostView.sampler.filter,
+                                              HitBatchLaunch{c, a, numRays, org, dir, tRanges, times, out, maxOut,
+                                                             count, st});
+}
+
+}  // namespace vklhip

@@ -198,10 +198,14 @@ hipError_t launchHitInit(const IteratorContextView *c, HitState *state, const fl
 hipError_t launchHitNext(const IteratorContextView &hostView, const IteratorContextView *c, const AttrView &a,
                          HitState *state, HitOut *out, int *result, hipStream_t st)
 {
-  if (hostView.sampler.grid.temporalFormat || !hostView.sampler.grid.dense)
-    return launchHitNextOther(hostView, c, a, state, out, result, st);
-  return dispatchSource<kDenseKinds>(hostView.sampler.grid, a, hostView.sampler.filter,
-                                     HitNextLaunch{c, a, state, out, result, st});
+  const GridView &g = hostView.sampler.grid;
+  if (!g.dense && !g.temporalFormat && g.leafAccess)
+    return launchHitNextObserved(hostView, c, a, state, out, result, st);
+  if (g.temporalFormat || !g.dense) return launchHitNextOther(hostView, c, a, state, out, result, st);
+  if (a.type == kTypeHalf || a.type == kTypeFloat || a.type == kTypeDouble)
+    return launchHitNextDenseFloat(hostView, c, a, state, out, result, st);
+  return dispatchSource<kDenseIntKinds>(hostView.sampler.grid, a, hostView.sampler.filter,
+                                        HitNextLaunch{c, a, state, out, result, st});
 }
 
 // ---- batches ---------------------------------------------------------------------
@@ -243,10 +247,16 @@ hipError_t launchHitBatch(const IteratorContextView &hostView, const IteratorCon
                           const float *times, HitOut *out, uint32_t maxOut, uint32_t *count, hipStream_t st)
 {
   if (numRays == 0) return hipSuccess;
-  if (hostView.sampler.grid.temporalFormat || !hostView.sampler.grid.dense)
+  const GridView &g = hostView.sampler.grid;
+  if (!g.dense && !g.temporalFormat && g.leafAccess)
+    return launchHitBatchObserved(hostView, c, a, numRays, org, dir, tRanges, times, out, maxOut, count, st);
+  if (g.temporalFormat || !g.dense)
     return launchHitBatchOther(hostView, c, a, numRays, org, dir, tRanges, times, out, maxOut, count, st);
-  return dispatchSource<kDenseKinds>(hostView.sampler.grid, a, hostView.sampler.filter,
-                                     HitBatchLaunch{c, a, numRays, org, dir, tRanges, times, out, maxOut, count, st});
+  if (a.type == kTypeHalf || a.type == kTypeFloat || a.type == kTypeDouble)
+    return launchHitBatchDenseFloat(hostView, c, a, numRays, org, dir, tRanges, times, out, maxOut, count, st);
+  return dispatchSource<kDenseIntKinds>(hostView.sampler.grid, a, hostView.sampler.filter,
+                                        HitBatchLaunch{c, a, numRays, org, dir, tRanges, times, out, maxOut, count,
+                                                       st});
 }
 
 }  // namespace vklhip

@@ -146,6 +146,11 @@ struct IntervalState
   float tMax;
   float nominalDeltaT;
   DdaState dda;
+  // vdb node-table cache: the level-2 node that the last query reached for
+  // the 128^3 block ck (ck[0] == INT32_MIN: empty). A query in the same block
+  // reaches the same node through the same level-0 / level-1 voxels.
+  int32_t ck[3];
+  uint32_t cnode;
 };
 
 struct HitState
