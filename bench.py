@@ -282,6 +282,39 @@ def variant_raymarch(torch, vkl, vol, n, steps, warmup, width=1920, height=1080,
             "note": "coherent gathers hit L1/L2; bound by vector ALU and L1 issue, not HBM"}
 
 
+def variant_woodcock(torch, vkl, vol, n, steps, warmup, width=1920, height=1080, sigma=0.5):
+    """Config-2 free-flight sampling: 1920x1080 rays along +z through the
+    512^3 volume, Woodcock tracking with majorant `sigma` per voxel length and
+    an opacity ramp 0 -> 0.05 over the volume's value range
+    (vklComputeWoodcockN; TEA states continue across steps)."""
+    s = vol.sampler(filter=vkl.VKL_FILTER_LINEAR)
+    ys, xs = torch.meshgrid(torch.arange(height, device="cuda", dtype=torch.float32),
+                            torch.arange(width, device="cuda", dtype=torch.float32), indexing="ij")
+    r = width * height
+    org = torch.stack([(xs.reshape(-1) + 0.5) * ((n - 1) / width), (ys.reshape(-1) + 0.5) * ((n - 1) / height),
+                       torch.zeros(r, device="cuda")], 1).contiguous()
+    d = torch.zeros((r, 3), device="cuda")
+    d[:, 2] = 1.0
+    tr = torch.zeros((r, 2), device="cuda")
+    tr[:, 1] = float(n - 1)
+    rng = torch.zeros((r, 2), dtype=torch.int32, device="cuda")
+    rng[:, 1] = torch.arange(r, dtype=torch.int32, device="cuda")
+    lo, hi = vol.value_range()
+    tf = torch.zeros((16, 4), device="cuda")
+    tf[:, 3] = torch.linspace(0.0, 0.05, 16, device="cuda")
+    res = {}
+
+    def run():
+        res["ev"], _, res["cnt"] = s.woodcock(org, d, tr, rng, sigma, tf, (lo, hi))
+
+    ms = timed(torch, run, steps, warmup)
+    coll = float(res["cnt"].double().sum())
+    return {"rays": r, "sigma_max": sigma, "ms": round(ms, 4), "mrays_s": round(r / ms / 1e3, 2),
+            "collisions_per_ray": round(coll / r, 2), "gcollisions_s": round(coll / ms / 1e6, 2),
+            "escaped": round(float(torch.isinf(res["ev"]).float().mean()), 4),
+            "note": "each tentative collision: two 8-round TEA draws + one trilinear sample; VALU-bound"}
+
+
 def boundary_rays(torch, lo, hi, count, seed):
     """Origins uniform on the faces of the box [lo, hi], directions towards a
     uniform point inside it (unit length)."""
@@ -379,8 +412,9 @@ def main():
                     help="device voxel layout: the caller's x-fastest array (with --brick) or corner-packed cells")
     ap.add_argument("--binned", choices=("auto", "on", "off"), default="auto",
                     help="spatially binned sampling (auto: on for large volumes and batches)")
-    ap.add_argument("--variants", default="layouts,config2,raymarch,batched,neural,iterators",
-                    help="comma list of extra measurements: layouts, config2, raymarch, batched, neural, iterators "
+    ap.add_argument("--variants", default="layouts,config2,raymarch,woodcock,batched,neural,iterators",
+                    help="comma list of extra measurements: layouts, config2, raymarch, woodcock, batched, neural, "
+                         "iterators "
                          "('' = none)")
     ap.add_argument("--nf-res", type=int, default=1024, help="index-domain edge of the neural-field vdb")
     ap.add_argument("--vdb-res", type=int, default=4096, help="index-domain edge of the iterator vdb")
@@ -482,6 +516,14 @@ def main():
         elif v == "raymarch":
             rv = dev.structured_regular(vol_t)
             extra["raymarch"] = variant_raymarch(torch, vkl, rv, n, args.steps, args.warmup)
+            rv.release()
+        elif v == "woodcock":
+            rv = dev.structured_regular(vol_t, corner_packed=True)
+            extra["woodcock"] = variant_woodcock(torch, vkl, rv, n, args.steps, args.warmup)
+            extra["woodcock"]["layout"] = "corner-packed"
+            rv.release()
+            rv = dev.structured_regular(vol_t, corner_packed=False)
+            extra["woodcock_x_fastest"] = variant_woodcock(torch, vkl, rv, n, args.steps, args.warmup)
             rv.release()
         elif v == "config2":
             extra["config2"] = variant_config2(torch, vkl, dev, vol_t, pts, args.steps, args.warmup, U)

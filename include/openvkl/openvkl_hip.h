@@ -35,6 +35,34 @@ OPENVKL_INTERFACE void vklComputeRayMarchN(const VKLSampler *sampler, unsigned i
                                            const vkl_range1f *tRanges, unsigned int numSteps, float *integrals,
                                            unsigned int attributeIndex);
 
+/* Woodcock (delta) tracking fused with sampling: the free-flight sampler of
+ * the reference density path tracer (examples/interactive/renderer/
+ * density_path_tracer/DensityPathTracer.ispc:61-110, sampleWoodcock) for
+ * numRays rays in one launch. Ray r starts at t = tRanges[r].lower with TEA
+ * state (rngStates[2r], rngStates[2r+1]) (the reference seeds it with
+ * (frame, pixel index), Random.ih:32-38). Each tentative collision draws two
+ * TEA pairs (u0, u1), (u2, -), advances t by -ln(1 - u0) / sigmaMax, samples
+ * attribute attributeIndex at origins[r] + t directions[r] at time
+ * clamp(time + (u2 - 0.5) shutter, 0, 1), and is accepted when
+ * u1 < sigmaMax * opacity(sample) / sigmaMax, opacity being the w channel of
+ * the transfer function colorsAndOpacities (numTfValues (r, g, b, a)
+ * entries over tfValueRange, TransferFunction.ih:17-47).
+ * Writes eventT[r] = t of the accepted collision, +inf when t passes
+ * tRanges[r].upper first (transmittance 1) and NaN when maxCollisions
+ * tentative collisions were all rejected; eventSamples[r] = the sample at the
+ * accepted collision (0 otherwise); numCollisions[r] (may be NULL) = samples
+ * taken; rngStates updated in place. sigmaMax <= 0 gives +inf without drawing.
+ * Binary32 without contraction; ln is the library's own atanh-series
+ * logarithm, so the event sequence is reproducible bit for bit. Pointers may
+ * be host or device memory. */
+OPENVKL_INTERFACE void vklComputeWoodcockN(const VKLSampler *sampler, unsigned int numRays,
+                                           const vkl_vec3f *origins, const vkl_vec3f *directions,
+                                           const vkl_range1f *tRanges, unsigned int *rngStates, float sigmaMax,
+                                           const float *colorsAndOpacities, unsigned int numTfValues,
+                                           vkl_range1f tfValueRange, float time, float shutter,
+                                           unsigned int maxCollisions, unsigned int attributeIndex, float *eventT,
+                                           float *eventSamples, unsigned int *numCollisions);
+
 /* Iterate the intervals of numRays rays in one launch. Ray r writes up to
  * maxIntervalsPerRay intervals to intervals[r * maxIntervalsPerRay + k] and
  * their count to numIntervals[r]; the sequence equals calling

@@ -148,6 +148,8 @@ def lib():
         "vklInitHitIterator": (vp, [C.POINTER(O), C.POINTER(vec3f), C.POINTER(vec3f), C.POINTER(range1f), f32, vp]),
         "vklIterateHit": (i32, [vp, C.POINTER(VKLHit)]),
         "vklComputeRayMarchN": (None, [C.POINTER(O), C.c_uint, vp, vp, vp, C.c_uint, vp, C.c_uint]),
+        "vklComputeWoodcockN": (None, [C.POINTER(O), C.c_uint, vp, vp, vp, vp, f32, vp, C.c_uint, range1f, f32, f32,
+                                       C.c_uint, C.c_uint, vp, vp, vp]),
         "vklIterateIntervalsBatch": (None, [C.POINTER(O), C.c_uint, vp, vp, vp, vp, vp, C.c_uint, vp]),
         "vklIterateHitsBatch": (None, [C.POINTER(O), C.c_uint, vp, vp, vp, vp, vp, C.c_uint, vp]),
         "vklNewSampleBatchPlan": (vp, [C.c_uint, vp, vp, vp, vp, vp, i32]),
@@ -562,6 +564,31 @@ class Sampler(Handle):
         lib().vklComputeRayMarchN(C.byref(self.h), r, _ptr(org), _ptr(dir), _ptr(t_ranges), steps, _ptr(out), attr)
         self.device.check()
         return out
+
+    def woodcock(self, org, dir, t_ranges, rng, sigma_max, tf, tf_range, time=0.0, shutter=0.0,
+                 max_collisions=1 << 20, attr=0, counts=True):
+        """vklComputeWoodcockN: one free-flight (delta tracking) event per ray.
+        rng (numRays x 2 uint32, TEA state) is updated in place; tf holds
+        (r, g, b, opacity) rows over tf_range. Returns (eventT, samples,
+        collisions or None)."""
+        r = _n_points(org)
+        ev = self._out(org, (r,))
+        sm = self._out(org, (r,))
+        cnt = None
+        if counts:
+            if isinstance(org, np.ndarray):
+                cnt = np.empty(r, np.uint32)
+            else:
+                import torch
+
+                cnt = torch.empty(r, dtype=torch.int32, device=org.device)
+        tfa = tf.contiguous() if hasattr(tf, "contiguous") else np.ascontiguousarray(tf, np.float32).reshape(-1, 4)
+        lib().vklComputeWoodcockN(C.byref(self.h), r, _ptr(org), _ptr(dir), _ptr(t_ranges), _ptr(rng),
+                                  float(sigma_max), _ptr(tfa), len(tfa), range1f(*[float(x) for x in tf_range]),
+                                  float(time), float(shutter), int(max_collisions), attr, _ptr(ev), _ptr(sm),
+                                  _ptr(cnt))
+        self.device.check()
+        return ev, sm, cnt
 
     def sample_m(self, xyz, attrs, times=None):
         n = _n_points(xyz)

@@ -1287,6 +1287,53 @@ OPENVKL_INTERFACE void vklComputeRayMarchN(const VKLSampler *s, unsigned int num
   VKL_CATCH()
 }
 
+OPENVKL_INTERFACE void vklComputeWoodcockN(const VKLSampler *s, unsigned int numRays, const vkl_vec3f *origins,
+                                           const vkl_vec3f *directions, const vkl_range1f *tRanges,
+                                           unsigned int *rngStates, float sigmaMax, const float *colorsAndOpacities,
+                                           unsigned int numTfValues, vkl_range1f tfValueRange, float time,
+                                           float shutter, unsigned int maxCollisions, unsigned int a, float *eventT,
+                                           float *eventSamples, unsigned int *numCollisions)
+{
+  VKL_TRY(samplerOf(s)->device)
+  Sampler *sp = samplerOf(s);
+  dev_->makeCurrent();
+  const AttrView &av = sp->attr(a);
+  if (numRays == 0) return;
+  if (!origins || !directions || !tRanges || !rngStates || !eventT || !eventSamples)
+    throw VklError("null ray array", VKL_INVALID_ARGUMENT);
+  if (numTfValues && !colorsAndOpacities) throw VklError("null transfer function", VKL_INVALID_ARGUMENT);
+  hipStream_t st = dev_->stream;
+  Staged o(origins, numRays * 12ull, st, true), d(directions, numRays * 12ull, st, true);
+  Staged t(tRanges, numRays * 8ull, st, true), tf(colorsAndOpacities, numTfValues * 16ull, st, true);
+  Staged rng(rngStates, numRays * 8ull, st, true);
+  Staged outT(eventT, numRays * 4ull, st, false), outS(eventSamples, numRays * 4ull, st, false);
+  Staged cnt(numCollisions, numCollisions ? numRays * 4ull : 0, st, false);
+  WoodcockArgs w{};
+  w.numRays       = numRays;
+  w.maxCollisions = maxCollisions;
+  w.tfSize        = numTfValues;
+  w.sigmaMax      = sigmaMax;
+  w.time          = time;
+  w.shutter       = shutter;
+  w.tfLo          = tfValueRange.lower;
+  w.tfHi          = tfValueRange.upper;
+  w.org           = o.as<float>();
+  w.dir           = d.as<float>();
+  w.tRange        = t.as<float>();
+  w.tf            = tf.as<float>();
+  w.rng           = rng.as<uint32_t>();
+  w.eventT        = outT.as<float>();
+  w.samples       = outS.as<float>();
+  w.count         = cnt.as<uint32_t>();
+  checkHip(launchWoodcock(sp->view, av, a, w, st), "woodcock kernel");
+  rng.copyOut();
+  outT.copyOut();
+  outS.copyOut();
+  cnt.copyOut();
+  syncDevice(dev_);
+  VKL_CATCH()
+}
+
 OPENVKL_INTERFACE void vklComputeGradientNAsync(const VKLSampler *s, unsigned int N, const vkl_vec3f *oc,
                                                 vkl_vec3f *out, unsigned int a, void *stream)
 {

@@ -26,6 +26,22 @@ hipError_t launchSampleStrided(const SamplerView &s, const AttrView &a, uint32_t
 hipError_t launchRayMarch(const SamplerView &s, const AttrView &a, uint32_t attr, uint32_t numRays, const float *org,
                           const float *dir, const float *tRange, uint32_t steps, float *out, hipStream_t stream);
 
+// Woodcock (delta) tracking fused with sampling (kernels_woodcock.hip). All
+// pointers are device memory; org / dir 3 floats, tRange 2 floats and rng 2
+// uint32 (TEA state, updated) per ray; tf holds tfSize (r, g, b, opacity)
+// entries over the value range [tfLo, tfHi]; count may be null.
+struct WoodcockArgs
+{
+  uint32_t numRays, maxCollisions, tfSize;
+  float sigmaMax, time, shutter, tfLo, tfHi;
+  const float *org, *dir, *tRange, *tf;
+  uint32_t *rng;
+  float *eventT, *samples;
+  uint32_t *count;
+};
+hipError_t launchWoodcock(const SamplerView &s, const AttrView &a, uint32_t attr, const WoodcockArgs &w,
+                          hipStream_t stream);
+
 // Copy an x-fastest dims[0] x dims[1] x dims[2] array of elemSize-byte
 // elements into brick layout L (total = padded element count).
 hipError_t launchSwizzle(const void *src, void *dst, uint32_t elemSize, const uint32_t dims[3],

@@ -81,6 +81,10 @@ def lib():
         L.oracle_gradient_times.argtypes = [vp, u64, vp, vp, vp, u32]
         L.oracle_inner_nodes.restype = u64
         L.oracle_inner_nodes.argtypes = [vp, i32, vp, u64]
+        L.oracle_woodcock.argtypes = [vp, u64, vp, vp, vp, vp, f32, vp, u32, f32, f32, f32, f32, u32, u32, vp, vp, vp]
+        L.oracle_tea.argtypes = [vp, vp, u32]
+        L.oracle_flight_log.restype = f32
+        L.oracle_flight_log.argtypes = [f32]
         L.oracle_half_to_float.restype = f32
         L.oracle_half_to_float.argtypes = [C.c_uint16]
         _lib = L
@@ -265,6 +269,22 @@ class Grid:
             lib().oracle_sample(self.h, xyz.shape[0], _p(xyz), _p(out), attr)
         return out
 
+    def woodcock(self, org, dir, t_ranges, rng, sigma_max, tf, tf_range, time=0.0, shutter=0.0,
+                 max_collisions=1 << 20, attr=0):
+        """Woodcock tracking per ray; rng (n x 2 uint32) is updated in place.
+        Returns (eventT, samples, collisions)."""
+        org = np.ascontiguousarray(org, np.float32).reshape(-1, 3)
+        dir = np.ascontiguousarray(dir, np.float32).reshape(-1, 3)
+        tr = np.ascontiguousarray(t_ranges, np.float32).reshape(-1, 2)
+        assert rng.dtype == np.uint32 and rng.flags["C_CONTIGUOUS"] and rng.size == 2 * len(org)
+        tf = np.ascontiguousarray(tf, np.float32).reshape(-1, 4)
+        n = len(org)
+        ev, sm, cnt = np.empty(n, np.float32), np.empty(n, np.float32), np.empty(n, np.uint32)
+        lib().oracle_woodcock(self.h, n, _p(org), _p(dir), _p(tr), _p(rng), float(sigma_max), _p(tf), len(tf),
+                              float(tf_range[0]), float(tf_range[1]), float(time), float(shutter),
+                              int(max_collisions), attr, _p(ev), _p(sm), _p(cnt))
+        return ev, sm, cnt
+
     def sample_m(self, xyz, attrs):
         xyz = np.ascontiguousarray(xyz, np.float32).reshape(-1, 3)
         at = np.ascontiguousarray(attrs, np.uint32)
@@ -346,3 +366,15 @@ def tricubic_matrix() -> np.ndarray:
 
 def half_to_float(h: int) -> float:
     return float(lib().oracle_half_to_float(h))
+
+
+def tea(v, key, rounds):
+    """TEA over (v0, v1) with key (4 words) and `rounds` rounds."""
+    st = np.array(v, np.uint32)
+    k = np.array(key, np.uint32)
+    lib().oracle_tea(_p(st), _p(k), rounds)
+    return int(st[0]), int(st[1])
+
+
+def flight_log(x):
+    return np.float32(lib().oracle_flight_log(float(x)))

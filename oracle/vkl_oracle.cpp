@@ -1764,4 +1764,129 @@ uint32_t oracle_iterate_hits(const OracleGrid *g, uint32_t attr, int depth, int 
   return n;
 }
 
+
+// ---- Woodcock (delta) tracking --------------------------------------------
+// Free-flight sampling of the density path tracer
+// (examples/interactive/renderer/density_path_tracer/DensityPathTracer.ispc:
+// 61-110), with the renderer's TEA generator (Random.ih:13-46) and transfer
+// function (TransferFunction.ih:17-47).
+
+// TEA with `rounds` rounds and key k (Wheeler & Needham 1994); the renderer
+// uses 8 rounds and its own key words.
+void oracle_tea(uint32_t *v, const uint32_t *k, uint32_t rounds)
+{
+  uint32_t a = v[0], b = v[1], sum = 0;
+  for (uint32_t i = 0; i < rounds; ++i) {
+    sum += 0x9e3779b9u;
+    a += ((b << 4) + k[0]) ^ (b + sum) ^ ((b >> 5) + k[1]);
+    b += ((a << 4) + k[2]) ^ (a + sum) ^ ((a >> 5) + k[3]);
+  }
+  v[0] = a;
+  v[1] = b;
+}
+
+// The device library's logarithm: x = (1 + f) 2^e, 1 + f in [sqrt(1/2),
+// sqrt(2)); s = f / (2 + f); ln(1 + f) = 2s + s R with R the atanh series
+// 2s^2/3 + ... + 2s^8/9 (Abramowitz & Stegun 4.1.27), rearranged with
+// 2s = f - s f to f - (f^2/2 - s (f^2/2 + R)); ln 2 = 0x3f317180 + 9.0580013515e-6 (ln 2 minus it).
+// Binary32 in this operation order.
+float oracle_flight_log(float x)
+{
+  if (!(x > 0.f)) return x == 0.f ? -kInf : std::numeric_limits<float>::quiet_NaN();
+  if (x == kInf) return x;
+  int e = 0;
+  if (x < kFltMin) {
+    x = x * 8388608.f;
+    e = -23;
+  }
+  uint32_t b;
+  std::memcpy(&b, &x, 4);
+  e += (int)(b >> 23) - 127;
+  const uint32_t mb = (b & 0x7fffffu) | 0x3f800000u;
+  float m;
+  std::memcpy(&m, &mb, 4);
+  if (m > 1.41421354f) {
+    m = m * 0.5f;
+    e += 1;
+  }
+  const float f = m - 1.f;
+  const float s = f / (2.f + f);
+  const float z = s * s;
+  float series  = 0.222222222222f;
+  series        = series * z + 0.285714285714f;
+  series        = series * z + 0.4f;
+  series        = series * z + 0.666666666667f;
+  series        = series * z;
+  const float halfF2 = 0.5f * f * f;
+  const float ke     = (float)e;
+  return ke * 0.693138123f - ((halfF2 - (s * (halfF2 + series) + ke * 9.0580013515e-6f)) - f);
+}
+
+static float tfOpacity(const float *tf, uint32_t n, float lo, float hi, float v)
+{
+  if (std::isnan(v) || n == 0) return 0.f;
+  if (v <= lo) return tf[3];
+  if (v >= hi) return tf[4 * (n - 1) + 3];
+  v               = (v - lo) / (hi - lo) * ((float)n - 1.f);
+  // floor(v) kept inside [0, n - 1], so a non-finite range cannot index out
+  // of the table (for finite ranges floor(v) is already inside).
+  float fi = std::floor(v);
+  if (!(fi >= 0.f)) fi = 0.f;
+  if (fi > (float)(n - 1)) fi = (float)(n - 1);
+  const int i     = (int)fi;
+  const float rem = v - fi;
+  const int j     = std::min(i + 1, (int)n - 1);
+  return (1.f - rem) * tf[4 * i + 3] + rem * tf[4 * j + 3];
+}
+
+void oracle_woodcock(const OracleGrid *g, uint64_t numRays, const float *org, const float *dir, const float *tRange,
+                     uint32_t *rng, float sigmaMax, const float *tf, uint32_t tfSize, float tfLo, float tfHi,
+                     float time, float shutter, uint32_t maxCollisions, uint32_t attr, float *eventT,
+                     float *samples, uint32_t *count)
+{
+  static const uint32_t key[4] = {0xa341316cu, 0xc8013ea4u, 0xad90777du, 0x7e95761eu};
+  const float toFloat = 2.3283064365386962890625e-10f;
+  for (uint64_t r = 0; r < numRays; ++r) {
+    const V3f o{org[3 * r], org[3 * r + 1], org[3 * r + 2]};
+    const V3f d{dir[3 * r], dir[3 * r + 1], dir[3 * r + 2]};
+    float t = tRange[2 * r];
+    const float tfar = tRange[2 * r + 1];
+    uint32_t *st = rng + 2 * r;
+    float sample = 0.f, hitT = kInf;
+    uint32_t n = 0;
+    bool hit = false;
+    if (sigmaMax > 0.f) {
+      hitT = std::numeric_limits<float>::quiet_NaN();
+      while (n < maxCollisions) {
+        oracle_tea(st, key, 8);
+        const float u0 = (float)st[0] * toFloat, u1 = (float)st[1] * toFloat;
+        oracle_tea(st, key, 8);
+        const float u2 = (float)st[0] * toFloat;
+        t = t + -oracle_flight_log(1.f - u0) / sigmaMax;
+        if (t > tfar) {
+          hitT = kInf;
+          break;
+        }
+        ++n;
+        float tm = time;
+        if (shutter != 0.f) tm = tm + (u2 - 0.5f) * shutter;
+        tm = tm < 0.f ? 0.f : tm;
+        tm = tm > 1.f ? 1.f : tm;
+        g->curTime = tm;
+        sample = sampleObject(*g, V3f{o.x + t * d.x, o.y + t * d.y, o.z + t * d.z}, attr);
+        const float sigmaT = sigmaMax * tfOpacity(tf, tfSize, tfLo, tfHi, sample);
+        if (u1 < sigmaT / sigmaMax) {
+          hitT = t;
+          hit  = true;
+          break;
+        }
+      }
+    }
+    eventT[r]  = hitT;
+    samples[r] = hit ? sample : 0.f;
+    if (count) count[r] = n;
+  }
+  g->curTime = 0.f;
+}
+
 }  // extern "C"

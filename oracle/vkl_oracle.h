@@ -154,6 +154,21 @@ void oracle_tricubic_matrix(int32_t *out64x64);
 
 float oracle_half_to_float(uint16_t h);
 
+/* Woodcock (delta) tracking, density path tracer sampleWoodcock
+ * (DensityPathTracer.ispc:61-110): per ray, TEA state rng[2r..2r+1] (updated),
+ * tf = tfSize (r, g, b, opacity) entries over [tfLo, tfHi]. eventT: accepted
+ * collision t, +inf past tRange upper, NaN after maxCollisions rejections;
+ * samples: value at the accepted collision (0 otherwise); count (may be
+ * NULL): samples taken. */
+void oracle_woodcock(const OracleGrid *g, uint64_t numRays, const float *org, const float *dir, const float *tRange,
+                     uint32_t *rng, float sigmaMax, const float *tf, uint32_t tfSize, float tfLo, float tfHi,
+                     float time, float shutter, uint32_t maxCollisions, uint32_t attr, float *eventT,
+                     float *samples, uint32_t *count);
+/* TEA block cipher, `rounds` rounds over v[0..1] with key k[0..3]. */
+void oracle_tea(uint32_t *v, const uint32_t *k, uint32_t rounds);
+/* The device library's binary32 natural logarithm (atanh series). */
+float oracle_flight_log(float x);
+
 #ifdef __cplusplus
 }
 #endif
