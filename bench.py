@@ -312,7 +312,8 @@ def variant_woodcock(torch, vkl, vol, n, steps, warmup, width=1920, height=1080,
     return {"rays": r, "sigma_max": sigma, "ms": round(ms, 4), "mrays_s": round(r / ms / 1e3, 2),
             "collisions_per_ray": round(coll / r, 2), "gcollisions_s": round(coll / ms / 1e6, 2),
             "escaped": round(float(torch.isinf(res["ev"]).float().mean()), 4),
-            "note": "each tentative collision: two 8-round TEA draws + one trilinear sample; VALU-bound"}
+            "note": "each tentative collision: two 8-round TEA draws + one trilinear gather; the lanes of a "
+                    "wavefront (adjacent pixels) advance through similar depths, so their gathers share cache lines"}
 
 
 def boundary_rays(torch, lo, hi, count, seed):
