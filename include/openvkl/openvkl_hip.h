@@ -63,6 +63,27 @@ OPENVKL_INTERFACE void vklComputeWoodcockN(const VKLSampler *sampler, unsigned i
                                            unsigned int maxCollisions, unsigned int attributeIndex, float *eventT,
                                            float *eventSamples, unsigned int *numCollisions);
 
+/* Interval-driven ray marching with transfer-function compositing, the
+ * reference's ray-march iterator renderer (examples/interactive/renderer/
+ * ray_march_iterator/RayMarchIterator.ispc:39-137) for numRays rays in one
+ * launch: the intervals of the context (as vklIterateInterval returns them)
+ * are split into sub-intervals of nominalDeltaT / samplingRate; each is
+ * sampled at its midpoint (at `time`), mapped through the transfer function
+ * colorsAndOpacities (numTfValues (r, g, b, a) entries over tfValueRange,
+ * TransferFunction.ih:17-47), and composited front to back with opacity
+ * clamp(a * dt, 0, 1) until the accumulated opacity reaches 0.99.
+ * rgba[4r..4r+3] = colour and accumulated opacity of ray r (the renderer
+ * writes opacity 1); numSamples[r] (may be NULL) = samples taken, at most
+ * maxSamplesPerRay (which bounds rays whose sub-intervals are tiny).
+ * tRanges == NULL means [0, inf). Binary32 without contraction, in the
+ * renderer's operation order. Pointers may be host or device memory. */
+OPENVKL_INTERFACE void vklComputeRayMarchIntervalsN(const VKLIntervalIteratorContext *context, unsigned int numRays,
+                                                    const vkl_vec3f *origins, const vkl_vec3f *directions,
+                                                    const vkl_range1f *tRanges, float time, float samplingRate,
+                                                    const float *colorsAndOpacities, unsigned int numTfValues,
+                                                    vkl_range1f tfValueRange, unsigned int maxSamplesPerRay,
+                                                    float *rgba, unsigned int *numSamples);
+
 /* Iterate the intervals of numRays rays in one launch. Ray r writes up to
  * maxIntervalsPerRay intervals to intervals[r * maxIntervalsPerRay + k] and
  * their count to numIntervals[r]; the sequence equals calling

@@ -150,6 +150,8 @@ def lib():
         "vklComputeRayMarchN": (None, [C.POINTER(O), C.c_uint, vp, vp, vp, C.c_uint, vp, C.c_uint]),
         "vklComputeWoodcockN": (None, [C.POINTER(O), C.c_uint, vp, vp, vp, vp, f32, vp, C.c_uint, range1f, f32, f32,
                                        C.c_uint, C.c_uint, vp, vp, vp]),
+        "vklComputeRayMarchIntervalsN": (None, [C.POINTER(O), C.c_uint, vp, vp, vp, f32, f32, vp, C.c_uint, range1f,
+                                                C.c_uint, vp, vp]),
         "vklIterateIntervalsBatch": (None, [C.POINTER(O), C.c_uint, vp, vp, vp, vp, vp, C.c_uint, vp]),
         "vklIterateHitsBatch": (None, [C.POINTER(O), C.c_uint, vp, vp, vp, vp, vp, C.c_uint, vp]),
         "vklNewSampleBatchPlan": (vp, [C.c_uint, vp, vp, vp, vp, vp, i32]),
@@ -699,6 +701,27 @@ class IteratorContext(Handle):
             cnt = torch.empty(r, dtype=torch.int32, device=org.device)
         fn = lib().vklIterateHitsBatch if self.hit else lib().vklIterateIntervalsBatch
         fn(C.byref(self.h), r, _ptr(org), _ptr(dir), _ptr(t_ranges), _ptr(times), _ptr(out), max_per_ray, _ptr(cnt))
+        self.device.check()
+        return out, cnt
+
+    def ray_march(self, org, dir, tf, tf_range, t_ranges=None, time=0.0, sampling_rate=1.0,
+                  max_samples=1 << 20, counts=True):
+        """vklComputeRayMarchIntervalsN (interval contexts): per-ray composited
+        (r, g, b, opacity) and samples taken. tf: (r, g, b, opacity) rows."""
+        r = _n_points(org)
+        if isinstance(org, np.ndarray):
+            out = np.empty((r, 4), np.float32)
+            cnt = np.empty(r, np.uint32) if counts else None
+        else:
+            import torch
+
+            out = torch.empty((r, 4), dtype=torch.float32, device=org.device)
+            cnt = torch.empty(r, dtype=torch.int32, device=org.device) if counts else None
+        tfa = tf.contiguous() if hasattr(tf, "contiguous") else np.ascontiguousarray(tf, np.float32).reshape(-1, 4)
+        lib().vklComputeRayMarchIntervalsN(C.byref(self.h), r, _ptr(org), _ptr(dir), _ptr(t_ranges), float(time),
+                                           float(sampling_rate), _ptr(tfa), len(tfa),
+                                           range1f(*[float(x) for x in tf_range]), int(max_samples), _ptr(out),
+                                           _ptr(cnt))
         self.device.check()
         return out, cnt
 

@@ -42,6 +42,24 @@ struct WoodcockArgs
 hipError_t launchWoodcock(const SamplerView &s, const AttrView &a, uint32_t attr, const WoodcockArgs &w,
                           hipStream_t stream);
 
+// Interval-driven ray marching with transfer-function compositing
+// (kernels_rmi_*.hip). org / dir 3 floats and tRanges 2 floats (or null:
+// [0, inf)) per ray; tf holds tfSize (r, g, b, opacity) entries over
+// [tfLo, tfHi]; rgba 4 floats per ray; count (may be null) = samples taken;
+// at most maxSamples samples per ray. Device pointers.
+struct RayMarchIntervalsArgs
+{
+  uint32_t numRays, tfSize, maxSamples;
+  float time, samplingRate, tfLo, tfHi;
+  const float *org, *dir, *tRanges, *tf;
+  float *rgba;
+  uint32_t *count;
+};
+hipError_t launchRayMarchIntervals(const IteratorContextView &hostView, const IteratorContextView *c,
+                                   const AttrView &a, const RayMarchIntervalsArgs &m, hipStream_t stream);
+hipError_t launchRayMarchIntervalsSparse(const IteratorContextView &hostView, const IteratorContextView *c,
+                                         const AttrView &a, const RayMarchIntervalsArgs &m, hipStream_t stream);
+
 // Copy an x-fastest dims[0] x dims[1] x dims[2] array of elemSize-byte
 // elements into brick layout L (total = padded element count).
 hipError_t launchSwizzle(const void *src, void *dst, uint32_t elemSize, const uint32_t dims[3],

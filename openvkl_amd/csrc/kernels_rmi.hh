@@ -1,0 +1,103 @@
+// Interval-driven ray marching with transfer-function compositing, fused
+// with the interval iterator and the sampler: the reference's ray-march
+// iterator renderer (examples/interactive/renderer/ray_march_iterator/
+// RayMarchIterator.ispc:39-137), one ray per work-item. Instantiated by
+// kernels_rmi_dense.hip (structuredRegular sources) and
+// kernels_rmi_sparse.hip (vdb and temporal sources), which compile in
+// parallel.
+#pragma once
+
+#include "kernels.h"
+#include "kernels_hit.hh"
+
+namespace vklhip {
+
+// TransferFunction_sample (examples/interactive/renderer/
+// TransferFunction.ih:17-47): tf holds n (r, g, b, opacity) entries over
+// [lo, hi]; floor(v) is kept inside [0, n - 1] so a non-finite range cannot
+// index out of the table.
+VKL_DEV void tfSampleRgba(const float *__restrict__ tf, uint32_t n, float lo, float hi, float v, float c[4])
+{
+  if (__builtin_isnan(v) || n == 0) {
+    c[0] = c[1] = c[2] = c[3] = 0.f;
+    return;
+  }
+  if (v <= lo || v >= hi) {
+    const float *e = v <= lo ? tf : tf + 4 * (n - 1);
+    c[0] = e[0], c[1] = e[1], c[2] = e[2], c[3] = e[3];
+    return;
+  }
+  v        = (v - lo) / (hi - lo) * ((float)n - 1.f);
+  float fi = floorf(v);
+  if (!(fi >= 0.f)) fi = 0.f;
+  if (fi > (float)(n - 1)) fi = (float)(n - 1);
+  const int i     = (int)fi;
+  const int j     = i + 1 < (int)n - 1 ? i + 1 : (int)n - 1;
+  const float rem = v - fi;
+#pragma unroll
+  for (int k = 0; k < 4; k++) c[k] = (1.f - rem) * tf[4 * i + k] + rem * tf[4 * j + k];
+}
+
+VKL_DEV float minf_(float a, float b) { return a < b ? a : b; }
+
+// rgba[4r..4r+3] = composited colour and accumulated opacity of ray r.
+template <typename Src, int kFilter>
+__global__ __launch_bounds__(kIterBlock) void kRayMarchIntervals(const IteratorContextView *__restrict__ cp,
+                                                                  AttrView a, RayMarchIntervalsArgs m)
+{
+  const uint32_t r = blockIdx.x * kIterBlock + threadIdx.x;
+  if (r >= m.numRays) return;
+  const IteratorContextView &c = *cp;
+  PointSampler<Src, kFilter> smp(c.sampler.grid, a, c.attributeIndex);
+  smp.setTime(m.time);
+  const F3 o = {m.org[3 * r], m.org[3 * r + 1], m.org[3 * r + 2]};
+  const F3 d = {m.dir[3 * r], m.dir[3 * r + 1], m.dir[3 * r + 2]};
+  IntervalState st;
+  intervalInit(c, st, o, d, rayRange(m.tRanges, r));
+  float cr = 0.f, cg = 0.f, cb = 0.f, alpha = 0.f;
+  uint32_t taken = 0;
+  IntervalOut iv;
+  while (alpha < 0.99f && taken < m.maxSamples && intervalNext(c, st, iv)) {
+    const float dtNominal = iv.nominalDeltaT / m.samplingRate;
+    float lo = iv.tLower, hi = minf_(iv.tLower + dtNominal, iv.tUpper);
+    while (hi - lo > 0.f && alpha < 0.99f && taken < m.maxSamples) {
+      const float t  = 0.5f * (lo + hi);
+      const float dt = hi - lo;
+      const float s  = smp(F3{o.x + t * d.x, o.y + t * d.y, o.z + t * d.z});
+      taken++;
+      float rgba[4];
+      tfSampleRgba(m.tf, m.tfSize, m.tfLo, m.tfHi, s, rgba);
+      float op = rgba[3] * dt;
+      op       = op > 0.f ? op : 0.f;
+      op       = op < 1.f ? op : 1.f;
+      cr       = cr + (1.f - alpha) * (rgba[0] * op);
+      cg       = cg + (1.f - alpha) * (rgba[1] * op);
+      cb       = cb + (1.f - alpha) * (rgba[2] * op);
+      alpha    = alpha + (1.f - alpha) * op;
+      lo       = hi;
+      hi       = minf_(lo + dtNominal, iv.tUpper);
+    }
+  }
+  m.rgba[4 * r]     = cr;
+  m.rgba[4 * r + 1] = cg;
+  m.rgba[4 * r + 2] = cb;
+  m.rgba[4 * r + 3] = alpha;
+  if (m.count) m.count[r] = taken;
+}
+
+struct RayMarchIntervalsLaunch
+{
+  const IteratorContextView *c;
+  AttrView a;
+  RayMarchIntervalsArgs m;
+  hipStream_t st;
+  template <typename Src, int kFilter>
+  hipError_t operator()() const
+  {
+    hipLaunchKernelGGL((kRayMarchIntervals<Src, kFilter>), dim3((m.numRays + kIterBlock - 1) / kIterBlock),
+                       dim3(kIterBlock), 0, st, c, a, m);
+    return hipGetLastError();
+  }
+};
+
+}  // namespace vklhip

@@ -1889,4 +1889,76 @@ void oracle_woodcock(const OracleGrid *g, uint64_t numRays, const float *org, co
   g->curTime = 0.f;
 }
 
+// ---- interval ray marching with compositing --------------------------------
+// The ray-march iterator renderer (examples/interactive/renderer/
+// ray_march_iterator/RayMarchIterator.ispc:39-137) with the renderer's
+// transfer function (TransferFunction.ih:17-47).
+
+static void tfRgba(const float *tf, uint32_t n, float lo, float hi, float v, float c[4])
+{
+  if (std::isnan(v) || n == 0) {
+    c[0] = c[1] = c[2] = c[3] = 0.f;
+    return;
+  }
+  if (v <= lo || v >= hi) {
+    const float *e = v <= lo ? tf : tf + 4 * (n - 1);
+    for (int k = 0; k < 4; ++k) c[k] = e[k];
+    return;
+  }
+  v        = (v - lo) / (hi - lo) * ((float)n - 1.f);
+  float fi = std::floor(v);
+  if (!(fi >= 0.f)) fi = 0.f;
+  if (fi > (float)(n - 1)) fi = (float)(n - 1);
+  const int i     = (int)fi;
+  const int j     = std::min(i + 1, (int)n - 1);
+  const float rem = v - fi;
+  for (int k = 0; k < 4; ++k) c[k] = (1.f - rem) * tf[4 * i + k] + rem * tf[4 * j + k];
+}
+
+void oracle_ray_march_intervals(const OracleGrid *g, uint32_t attr, int depth, int numRanges, const float *ranges,
+                                uint64_t numRays, const float *org, const float *dir, const float *tRanges,
+                                float time, float samplingRate, const float *tf, uint32_t tfSize, float tfLo,
+                                float tfHi, uint32_t maxSamples, float *rgba, uint32_t *count)
+{
+  const Ranges q = makeRanges(numRanges, ranges);
+  g->curTime     = time;
+  for (uint64_t r = 0; r < numRays; ++r) {
+    const V3f o{org[3 * r], org[3 * r + 1], org[3 * r + 2]};
+    const V3f d{dir[3 * r], dir[3 * r + 1], dir[3 * r + 2]};
+    const R1f tr = tRanges ? R1f{tRanges[2 * r], tRanges[2 * r + 1]} : R1f{0.f, kInf};
+    IntervalIt it;
+    intervalInit(it, *g, attr, depth, o, d, tr);
+    float color[3] = {0.f, 0.f, 0.f}, alpha = 0.f;
+    uint32_t taken = 0;
+    R1f tOut, vOut;
+    float ndt;
+    while (alpha < 0.99f && taken < maxSamples && intervalNext(it, q, tOut, vOut, ndt)) {
+      const float step = ndt / samplingRate;
+      float lo = tOut.lower;
+      float hi = (tOut.lower + step < tOut.upper) ? tOut.lower + step : tOut.upper;
+      while (hi - lo > 0.f && alpha < 0.99f && taken < maxSamples) {
+        const float t  = 0.5f * (lo + hi);
+        const float dt = hi - lo;
+        const float s  = sampleObject(*g, V3f{o.x + t * d.x, o.y + t * d.y, o.z + t * d.z}, attr);
+        ++taken;
+        float c[4];
+        tfRgba(tf, tfSize, tfLo, tfHi, s, c);
+        float op = c[3] * dt;
+        op       = op > 0.f ? op : 0.f;
+        op       = op < 1.f ? op : 1.f;
+        for (int k = 0; k < 3; ++k) color[k] = color[k] + (1.f - alpha) * (c[k] * op);
+        alpha = alpha + (1.f - alpha) * op;
+        lo    = hi;
+        hi    = (lo + step < tOut.upper) ? lo + step : tOut.upper;
+      }
+    }
+    rgba[4 * r]     = color[0];
+    rgba[4 * r + 1] = color[1];
+    rgba[4 * r + 2] = color[2];
+    rgba[4 * r + 3] = alpha;
+    if (count) count[r] = taken;
+  }
+  g->curTime = 0.f;
+}
+
 }  // extern "C"

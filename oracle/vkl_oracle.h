@@ -169,6 +169,16 @@ void oracle_tea(uint32_t *v, const uint32_t *k, uint32_t rounds);
 /* The device library's binary32 natural logarithm (atanh series). */
 float oracle_flight_log(float x);
 
+/* Interval ray marching with transfer-function compositing, the ray-march
+ * iterator renderer (RayMarchIterator.ispc:39-137): intervals as
+ * oracle_iterate_intervals (depth, value ranges), sub-intervals of
+ * nominalDeltaT / samplingRate sampled at their midpoints at `time`, front to
+ * back until opacity 0.99; rgba 4 floats per ray; count (may be NULL). */
+void oracle_ray_march_intervals(const OracleGrid *g, uint32_t attr, int depth, int numRanges, const float *ranges,
+                                uint64_t numRays, const float *org, const float *dir, const float *tRanges,
+                                float time, float samplingRate, const float *tf, uint32_t tfSize, float tfLo,
+                                float tfHi, uint32_t maxSamples, float *rgba, uint32_t *count);
+
 #ifdef __cplusplus
 }
 #endif
