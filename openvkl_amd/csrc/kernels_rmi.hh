@@ -45,7 +45,7 @@ template <typename Src, int kFilter>
 __global__ __launch_bounds__(kIterBlock) void kRayMarchIntervals(const IteratorContextView *__restrict__ cp,
                                                                   AttrView a, RayMarchIntervalsArgs m)
 {
-  const uint32_t r = blockIdx.x * kIterBlock + threadIdx.x;
+  const uint32_t r = xcdBandBlock(blockIdx.x, gridDim.x) * kIterBlock + threadIdx.x;
   if (r >= m.numRays) return;
   const IteratorContextView &c = *cp;
   PointSampler<Src, kFilter> smp(c.sampler.grid, a, c.attributeIndex);

@@ -67,7 +67,7 @@ __global__ __launch_bounds__(kBlock) void kRayMarch(GridView g, AttrView a, uint
                                                      const float *__restrict__ tRange, uint32_t steps, float noff,
                                                      float *__restrict__ out)
 {
-  const uint32_t r = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t r = xcdBandBlock(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (r >= numRays) return;
   const F3 o = {org[3 * r], org[3 * r + 1], org[3 * r + 2]};
   const F3 d = {dir[3 * r], dir[3 * r + 1], dir[3 * r + 2]};

@@ -92,7 +92,7 @@ template <typename Src, int kFilter>
 __global__ __launch_bounds__(kWoodBlock) void kWoodcock(GridView g, AttrView a, uint32_t attr, WoodcockArgs w,
                                                        float noff)
 {
-  const uint32_t r = blockIdx.x * kWoodBlock + threadIdx.x;
+  const uint32_t r = xcdBandBlock(blockIdx.x, gridDim.x) * kWoodBlock + threadIdx.x;
   if (r >= w.numRays) return;
   const F3 o = {w.org[3 * r], w.org[3 * r + 1], w.org[3 * r + 2]};
   const F3 d = {w.dir[3 * r], w.dir[3 * r + 1], w.dir[3 * r + 2]};

@@ -37,6 +37,17 @@ VKL_DEV int32_t floorToInt(float f)
   return (fl >= -2147483648.f && fl < 2147483648.f) ? (int32_t)fl : INT32_MIN;
 }
 
+// Workgroup index remapped so each of the 8 XCDs (which take workgroups
+// round-robin, b -> XCD b mod 8, each with its own L2) walks one contiguous
+// band of the launch: neighbouring rays, which share voxel cache lines, then
+// meet in one L2 instead of eight. A bijection on [0, 8 floor(n / 8)); the
+// tail keeps its index.
+VKL_DEV uint32_t xcdBandBlock(uint32_t b, uint32_t n)
+{
+  const uint32_t per = n / 8u;
+  return b < per * 8u ? (b % 8u) * per + b / 8u : b;
+}
+
 // rkcommon divide_safe: 1 / x with |x| < FLT_MIN clamped to +-FLT_MIN.
 VKL_DEV float divideSafe(float x)
 {
