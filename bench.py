@@ -297,6 +297,18 @@ def variant_raymarch(torch, vkl, vol, n, steps, warmup, width=1920, height=1080,
         "ms": round(ms_i, 4), "mrays_s": round(r / ms_i / 1e3, 2), "samples_per_ray": round(taken / r, 1),
         "msamples_s": round(taken / ms_i / 1e3, 1),
         "opaque_rays": round(float((got["rgba"][:, 3] >= 0.99).float().mean()), 4)}
+    # The hit-iterator renderer: hits of one iso value, gradient normals, two
+    # shadow hit queries per hit (vklComputeHitRenderN).
+    iso = float(lo + 0.6 * (hi - lo))
+    hctx = s.hit_context([iso])
+
+    def run_h():
+        got["rgba"], got["cnt"] = hctx.render_hits(org, d, tf, (lo, hi), t_ranges=tr)
+
+    ms_h = timed(torch, run_h, steps, warmup)
+    hits = float(got["cnt"].double().sum())
+    res["hit_render"] = {"iso": round(iso, 4), "ms": round(ms_h, 4), "mrays_s": round(r / ms_h / 1e3, 2),
+                         "hits_per_ray": round(hits / r, 3)}
     return res
 
 

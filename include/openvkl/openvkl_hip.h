@@ -84,6 +84,26 @@ OPENVKL_INTERFACE void vklComputeRayMarchIntervalsN(const VKLIntervalIteratorCon
                                                     vkl_range1f tfValueRange, unsigned int maxSamplesPerRay,
                                                     float *rgba, unsigned int *numSamples);
 
+/* The reference's hit-iterator renderer (examples/interactive/renderer/
+ * hit_iterator_renderer/HitIteratorRenderer.ispc:51-150) for numRays rays in
+ * one launch. The hits of the context's iso values (as vklIterateHit returns
+ * them) are shaded with the gradient normal (sampler's gradient filter, at
+ * `time`): ambient 0.1 plus two lights, normalize(1, 1, 1) with emission 1
+ * and normalize(1, 1, -1) with 0.8, each Lambertian when it lies on the
+ * viewer's side of the surface and a shadow hit query from the hit point over
+ * [epsilon, inf) finds nothing (unshaded 1 without a normal). The albedo is
+ * the transfer function's colour at the hit sample; hits are composited with
+ * surface opacity 0.6 until the opacity reaches 0.99. rgba[4r..4r+3] =
+ * colour and accumulated opacity (the renderer writes opacity 1);
+ * numHits[r] (may be NULL) = hits shaded, at most maxHitsPerRay.
+ * normalize(v) = v (1 / sqrt(v . v)). tRanges == NULL means [0, inf).
+ * Pointers may be host or device memory. */
+OPENVKL_INTERFACE void vklComputeHitRenderN(const VKLHitIteratorContext *context, unsigned int numRays,
+                                            const vkl_vec3f *origins, const vkl_vec3f *directions,
+                                            const vkl_range1f *tRanges, float time, const float *colorsAndOpacities,
+                                            unsigned int numTfValues, vkl_range1f tfValueRange,
+                                            unsigned int maxHitsPerRay, float *rgba, unsigned int *numHits);
+
 /* Iterate the intervals of numRays rays in one launch. Ray r writes up to
  * maxIntervalsPerRay intervals to intervals[r * maxIntervalsPerRay + k] and
  * their count to numIntervals[r]; the sequence equals calling

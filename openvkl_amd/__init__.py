@@ -152,6 +152,8 @@ def lib():
                                        C.c_uint, C.c_uint, vp, vp, vp]),
         "vklComputeRayMarchIntervalsN": (None, [C.POINTER(O), C.c_uint, vp, vp, vp, f32, f32, vp, C.c_uint, range1f,
                                                 C.c_uint, vp, vp]),
+        "vklComputeHitRenderN": (None, [C.POINTER(O), C.c_uint, vp, vp, vp, f32, vp, C.c_uint, range1f, C.c_uint, vp,
+                                        vp]),
         "vklIterateIntervalsBatch": (None, [C.POINTER(O), C.c_uint, vp, vp, vp, vp, vp, C.c_uint, vp]),
         "vklIterateHitsBatch": (None, [C.POINTER(O), C.c_uint, vp, vp, vp, vp, vp, C.c_uint, vp]),
         "vklNewSampleBatchPlan": (vp, [C.c_uint, vp, vp, vp, vp, vp, i32]),
@@ -701,6 +703,24 @@ class IteratorContext(Handle):
             cnt = torch.empty(r, dtype=torch.int32, device=org.device)
         fn = lib().vklIterateHitsBatch if self.hit else lib().vklIterateIntervalsBatch
         fn(C.byref(self.h), r, _ptr(org), _ptr(dir), _ptr(t_ranges), _ptr(times), _ptr(out), max_per_ray, _ptr(cnt))
+        self.device.check()
+        return out, cnt
+
+    def render_hits(self, org, dir, tf, tf_range, t_ranges=None, time=0.0, max_hits=1 << 16):
+        """vklComputeHitRenderN (hit contexts): per-ray shaded (r, g, b,
+        opacity) and hits shaded."""
+        r = _n_points(org)
+        if isinstance(org, np.ndarray):
+            out, cnt = np.empty((r, 4), np.float32), np.empty(r, np.uint32)
+        else:
+            import torch
+
+            out = torch.empty((r, 4), dtype=torch.float32, device=org.device)
+            cnt = torch.empty(r, dtype=torch.int32, device=org.device)
+        tfa = tf.contiguous() if hasattr(tf, "contiguous") else np.ascontiguousarray(tf, np.float32).reshape(-1, 4)
+        lib().vklComputeHitRenderN(C.byref(self.h), r, _ptr(org), _ptr(dir), _ptr(t_ranges), float(time), _ptr(tfa),
+                                   len(tfa), range1f(*[float(x) for x in tf_range]), int(max_hits), _ptr(out),
+                                   _ptr(cnt))
         self.device.check()
         return out, cnt
 

@@ -1437,6 +1437,46 @@ OPENVKL_INTERFACE void vklComputeRayMarchIntervalsN(const VKLIntervalIteratorCon
   VKL_CATCH()
 }
 
+OPENVKL_INTERFACE void vklComputeHitRenderN(const VKLHitIteratorContext *c, unsigned int numRays,
+                                            const vkl_vec3f *origins, const vkl_vec3f *directions,
+                                            const vkl_range1f *tRanges, float time, const float *colorsAndOpacities,
+                                            unsigned int numTfValues, vkl_range1f tfValueRange,
+                                            unsigned int maxHitsPerRay, float *rgba, unsigned int *numHits)
+{
+  VKL_TRY(contextOf(c, true)->device)
+  IteratorContext *ctx = contextOf(c, true);
+  dev_->makeCurrent();
+  if (numRays == 0) return;
+  if (!origins || !directions || !rgba) throw VklError("null ray array", VKL_INVALID_ARGUMENT);
+  if (numTfValues && !colorsAndOpacities) throw VklError("null transfer function", VKL_INVALID_ARGUMENT);
+  hipStream_t st = dev_->stream;
+  Staged o(origins, numRays * 12ull, st, true), d(directions, numRays * 12ull, st, true);
+  Staged t(tRanges, tRanges ? numRays * 8ull : 0, st, true);
+  Staged tf(colorsAndOpacities, numTfValues * 16ull, st, true);
+  Staged out(rgba, numRays * 16ull, st, false);
+  Staged cnt(numHits, numHits ? numRays * 4ull : 0, st, false);
+  HitRenderArgs m{};
+  m.numRays = numRays;
+  m.tfSize  = numTfValues;
+  m.maxHits = maxHitsPerRay;
+  m.time    = time;
+  m.tfLo    = tfValueRange.lower;
+  m.tfHi    = tfValueRange.upper;
+  m.org     = o.as<float>();
+  m.dir     = d.as<float>();
+  m.tRanges = t.as<float>();
+  m.tf      = tf.as<float>();
+  m.rgba    = out.as<float>();
+  m.count   = cnt.as<uint32_t>();
+  checkHip(launchHitRender(ctx->view, ctx->deviceView.as<IteratorContextView>(),
+                           ctx->sampler->attr(ctx->view.attributeIndex), m, st),
+           "hit render");
+  out.copyOut();
+  cnt.copyOut();
+  syncDevice(dev_);
+  VKL_CATCH()
+}
+
 OPENVKL_INTERFACE int vklHipGetVolumeLevel(VKLVolume v, unsigned int level, uint64_t *numNodes,
                                            const uint64_t **voxels, const vkl_range1f **valueRanges)
 {

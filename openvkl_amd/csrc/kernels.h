@@ -60,6 +60,23 @@ hipError_t launchRayMarchIntervals(const IteratorContextView &hostView, const It
 hipError_t launchRayMarchIntervalsSparse(const IteratorContextView &hostView, const IteratorContextView *c,
                                          const AttrView &a, const RayMarchIntervalsArgs &m, hipStream_t stream);
 
+// The hit-iterator renderer (kernels_hitrender_*.hip): per ray the hits of the
+// context's iso values, shaded (gradient normal, ambient + two lights with
+// shadow hit queries, transfer-function albedo) and composited with surface
+// opacity 0.6 until 0.99; at most maxHits hits. Layout as RayMarchIntervalsArgs.
+struct HitRenderArgs
+{
+  uint32_t numRays, tfSize, maxHits;
+  float time, tfLo, tfHi;
+  const float *org, *dir, *tRanges, *tf;
+  float *rgba;
+  uint32_t *count;
+};
+hipError_t launchHitRender(const IteratorContextView &hostView, const IteratorContextView *c, const AttrView &a,
+                           const HitRenderArgs &m, hipStream_t stream);
+hipError_t launchHitRenderSparse(const IteratorContextView &hostView, const IteratorContextView *c,
+                                 const AttrView &a, const HitRenderArgs &m, hipStream_t stream);
+
 // Copy an x-fastest dims[0] x dims[1] x dims[2] array of elemSize-byte
 // elements into brick layout L (total = padded element count).
 hipError_t launchSwizzle(const void *src, void *dst, uint32_t elemSize, const uint32_t dims[3],

@@ -84,6 +84,7 @@ def lib():
         L.oracle_woodcock.argtypes = [vp, u64, vp, vp, vp, vp, f32, vp, u32, f32, f32, f32, f32, u32, u32, vp, vp, vp]
         L.oracle_ray_march_intervals.argtypes = [vp, u32, i32, i32, vp, u64, vp, vp, vp, f32, f32, vp, u32, f32, f32,
                                                  u32, vp, vp]
+        L.oracle_hit_render.argtypes = [vp, u32, i32, vp, u64, vp, vp, vp, f32, vp, u32, f32, f32, u32, vp, vp]
         L.oracle_tea.argtypes = [vp, vp, u32]
         L.oracle_flight_log.restype = f32
         L.oracle_flight_log.argtypes = [f32]
@@ -360,6 +361,19 @@ class Grid:
         lib().oracle_ray_march_intervals(self.h, attr, depth, nr, _p(vr), n, _p(org), _p(dir), _p(tr), float(time),
                                          float(sampling_rate), _p(tf), len(tf), float(tf_range[0]),
                                          float(tf_range[1]), int(max_samples), _p(out), _p(cnt))
+        return out, cnt
+
+    def hit_render(self, org, dir, values, tf, tf_range, t_ranges=None, time=0.0, attr=0, max_hits=1 << 16):
+        """Hit-iterator renderer per ray: (r, g, b, opacity) and hits shaded."""
+        v = np.ascontiguousarray(values, np.float32)
+        org = np.ascontiguousarray(org, np.float32).reshape(-1, 3)
+        dir = np.ascontiguousarray(dir, np.float32).reshape(-1, 3)
+        tr = None if t_ranges is None else np.ascontiguousarray(t_ranges, np.float32).reshape(-1, 2)
+        tf = np.ascontiguousarray(tf, np.float32).reshape(-1, 4)
+        n = len(org)
+        out, cnt = np.empty((n, 4), np.float32), np.empty(n, np.uint32)
+        lib().oracle_hit_render(self.h, attr, len(v), _p(v), n, _p(org), _p(dir), _p(tr), float(time), _p(tf),
+                                len(tf), float(tf_range[0]), float(tf_range[1]), int(max_hits), _p(out), _p(cnt))
         return out, cnt
 
     def hits(self, org, dir, values, t_range=(0.0, np.inf), attr=0, depth=-1, time=0.0, max_out=1024):

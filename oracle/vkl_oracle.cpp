@@ -1700,70 +1700,83 @@ uint32_t oracle_iterate_intervals(const OracleGrid *g, uint32_t attr, int depth,
   return n;
 }
 
+// vklInitHitIterator: the interval iterator plus resetInterval
+// (Interval.ih:15-22).
+void hitStart(HitState &h, const OracleGrid &g, uint32_t attr, int depth, V3f org, V3f dir, R1f tRange)
+{
+  h.origin    = org;
+  h.direction = dir;
+  intervalInit(h.iv, g, attr, depth, h.origin, h.direction, tRange);
+  h.cur            = R1f{1.f, -1.f};
+  h.ndt            = 0.f;
+  h.lastHitOffsetT = -kInf;
+}
+
+// One DefaultHitIterator_iterateHit call (DefaultIterator.ispc:168-258).
+bool hitStep(const OracleGrid &g, const Ranges &q, uint32_t attr, const std::vector<float> &vals, HitState &h,
+             float &ht, float &hv, float &he)
+{
+  if (vals.empty() || h.cur.lower == kInf) return false;
+  for (;;) {
+    const bool need = h.cur.upper <= h.cur.lower;
+    bool have       = !need;
+    if (need) {
+      R1f t, v;
+      float ndt;
+      have = intervalNext(h.iv, q, t, v, ndt);
+      if (have) {
+        h.cur = t;
+        h.ndt = ndt;
+      }
+    }
+    if (need && have) {
+      h.cur.lower = fmax_(h.cur.lower, h.lastHitOffsetT);
+      if (h.cur.upper <= h.cur.lower) continue;
+    }
+    if (!have) {
+      h.cur.lower = kInf;
+      return false;
+    }
+    if (intersectBisection(g, h.origin, h.direction, h.cur, attr, h.ndt, vals, ht, hv, he)) {
+      h.cur.lower      = ht + he;
+      h.lastHitOffsetT = ht + he;
+      return true;
+    }
+    h.cur.lower = kInf;
+  }
+}
+
+Ranges isoRanges(const std::vector<float> &vals)
+{
+  std::vector<float> pr;
+  for (float v : vals) {
+    pr.push_back(v);
+    pr.push_back(v);
+  }
+  return makeRanges((int)vals.size(), pr.data());
+}
+
 uint32_t oracle_iterate_hits(const OracleGrid *g, uint32_t attr, int depth, int numValues, const float *values,
                              const float org[3], const float dir[3], const float tRange[2], float time,
                              float *out, uint32_t maxOut)
 {
   g->curTime = time;  // temporal volumes sample at the ray time
   if (depth < 0) depth = g->dense ? oracle_map_depth(0.5f) : oracle_map_depth(1.f);
-  std::vector<float> vals(values, values + numValues);
-  std::vector<float> pr;
-  for (float v : vals) {
-    pr.push_back(v);
-    pr.push_back(v);
-  }
-  const Ranges q = makeRanges(numValues, pr.data());
+  const std::vector<float> vals(values, values + numValues);
+  const Ranges q = isoRanges(vals);
   HitState h;
-  h.origin    = V3f{org[0], org[1], org[2]};
-  h.direction = V3f{dir[0], dir[1], dir[2]};
-  intervalInit(h.iv, *g, attr, depth, h.origin, h.direction, R1f{tRange[0], tRange[1]});
-  h.cur            = R1f{1.f, -1.f};  // resetInterval, Interval.ih:15-22
-  h.ndt            = 0.f;
-  h.lastHitOffsetT = -kInf;
-  uint32_t n       = 0;
-  if (numValues == 0) return 0;
-  while (n < maxOut) {
-    // One DefaultHitIterator_iterateHit call (DefaultIterator.ispc:168-258).
-    if (h.cur.lower == kInf) break;
-    bool found = false;
-    for (;;) {
-      const bool need = h.cur.upper <= h.cur.lower;
-      bool have       = !need;
-      if (need) {
-        R1f t, v;
-        float ndt;
-        have = intervalNext(h.iv, q, t, v, ndt);
-        if (have) {
-          h.cur = t;
-          h.ndt = ndt;
-        }
-      }
-      if (need && have) {
-        h.cur.lower = fmax_(h.cur.lower, h.lastHitOffsetT);
-        if (h.cur.upper <= h.cur.lower) continue;
-      }
-      if (!have) {
-        h.cur.lower = kInf;
-        break;
-      }
-      float ht, hv, he;
-      if (intersectBisection(*g, h.origin, h.direction, h.cur, attr, h.ndt, vals, ht, hv, he)) {
-        out[3 * n]     = ht;
-        out[3 * n + 1] = hv;
-        out[3 * n + 2] = he;
-        n++;
-        h.cur.lower      = ht + he;
-        h.lastHitOffsetT = ht + he;
-        found            = true;
-        break;
-      }
-      h.cur.lower = kInf;
-    }
-    if (!found) break;
+  hitStart(h, *g, attr, depth, V3f{org[0], org[1], org[2]}, V3f{dir[0], dir[1], dir[2]},
+           R1f{tRange[0], tRange[1]});
+  uint32_t n = 0;
+  float ht, hv, he;
+  while (n < maxOut && hitStep(*g, q, attr, vals, h, ht, hv, he)) {
+    out[3 * n]     = ht;
+    out[3 * n + 1] = hv;
+    out[3 * n + 2] = he;
+    n++;
   }
   return n;
 }
-
 
 // ---- Woodcock (delta) tracking --------------------------------------------
 // Free-flight sampling of the density path tracer
@@ -1957,6 +1970,74 @@ void oracle_ray_march_intervals(const OracleGrid *g, uint32_t attr, int depth, i
     rgba[4 * r + 2] = color[2];
     rgba[4 * r + 3] = alpha;
     if (count) count[r] = taken;
+  }
+  g->curTime = 0.f;
+}
+
+// ---- hit-iterator renderer ---------------------------------------------------
+// examples/interactive/renderer/hit_iterator_renderer/HitIteratorRenderer.ispc:
+// 51-150: per hit a gradient normal, ambient 0.1 plus two Lambertian lights
+// with one shadow hit query each, transfer-function albedo, surface opacity
+// 0.6, until opacity 0.99. normalize(v) = v (1 / sqrt(v . v)).
+
+static V3f normalize3(V3f v)
+{
+  const float inv = 1.f / std::sqrt(v.x * v.x + v.y * v.y + v.z * v.z);
+  return V3f{v.x * inv, v.y * inv, v.z * inv};
+}
+static float dot3(V3f a, V3f b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+
+void oracle_hit_render(const OracleGrid *g, uint32_t attr, int numValues, const float *values, uint64_t numRays,
+                       const float *org, const float *dir, const float *tRanges, float time, const float *tf,
+                       uint32_t tfSize, float tfLo, float tfHi, uint32_t maxHits, float *rgba, uint32_t *count)
+{
+  g->curTime      = time;
+  const int depth = g->dense ? oracle_map_depth(0.5f) : oracle_map_depth(1.f);
+  const std::vector<float> vals(values, values + numValues);
+  const Ranges q         = isoRanges(vals);
+  const V3f light[2]     = {normalize3(V3f{1.f, 1.f, 1.f}), normalize3(V3f{1.f, 1.f, -1.f})};
+  const float emission[2] = {1.f, 0.8f};
+  for (uint64_t r = 0; r < numRays; ++r) {
+    const V3f o{org[3 * r], org[3 * r + 1], org[3 * r + 2]};
+    const V3f d{dir[3 * r], dir[3 * r + 1], dir[3 * r + 2]};
+    const R1f tr = tRanges ? R1f{tRanges[2 * r], tRanges[2 * r + 1]} : R1f{0.f, kInf};
+    HitState h;
+    hitStart(h, *g, attr, depth, o, d, tr);
+    const V3f wi   = normalize3(V3f{-1.f * d.x, -1.f * d.y, -1.f * d.z});
+    float color[3] = {0.f, 0.f, 0.f}, alpha = 0.f;
+    uint32_t n     = 0;
+    float ht, hv, he;
+    while (alpha < 0.99f && n < maxHits && hitStep(*g, q, attr, vals, h, ht, hv, he)) {
+      ++n;
+      const V3f c = {o.x + ht * d.x, o.y + ht * d.y, o.z + ht * d.z};
+      V3f N       = normalize3(gradientObject(*g, c, attr));
+      if (std::isnan(N.x) || std::isnan(N.y) || std::isnan(N.z)) N = V3f{0.f, 0.f, 0.f};
+      const float ci = dot3(N, wi);
+      float illum    = 0.f;
+      if (std::sqrt(dot3(N, N)) > 0.f) {
+        illum = 0.1f;
+        for (int i = 0; i < 2; ++i) {
+          const float co = dot3(N, light[i]);
+          if ((co > 0.f) == (ci > 0.f)) {
+            HitState sh;
+            hitStart(sh, *g, attr, depth, c, light[i], R1f{he, kInf});
+            float st, sv, se;
+            if (!hitStep(*g, q, attr, vals, sh, st, sv, se)) illum += std::fabs(co) * emission[i];
+          }
+        }
+      } else {
+        illum = 1.f;
+      }
+      float albedo[4];
+      tfRgba(tf, tfSize, tfLo, tfHi, hv, albedo);
+      for (int k = 0; k < 3; ++k) color[k] = color[k] + (1.f - alpha) * illum * albedo[k];
+      alpha = alpha + (1.f - alpha) * 0.6f;
+    }
+    rgba[4 * r]     = color[0];
+    rgba[4 * r + 1] = color[1];
+    rgba[4 * r + 2] = color[2];
+    rgba[4 * r + 3] = alpha;
+    if (count) count[r] = n;
   }
   g->curTime = 0.f;
 }

@@ -179,6 +179,15 @@ void oracle_ray_march_intervals(const OracleGrid *g, uint32_t attr, int depth, i
                                 float time, float samplingRate, const float *tf, uint32_t tfSize, float tfLo,
                                 float tfHi, uint32_t maxSamples, float *rgba, uint32_t *count);
 
+/* Hit-iterator renderer (HitIteratorRenderer.ispc:51-150): hits of the iso
+ * values (as oracle_iterate_hits with the default depth), gradient normal,
+ * ambient 0.1 + two lights with shadow hit queries, transfer-function albedo,
+ * surface opacity 0.6 until opacity 0.99, at most maxHits hits; rgba 4 floats
+ * per ray; count (may be NULL) = hits shaded. */
+void oracle_hit_render(const OracleGrid *g, uint32_t attr, int numValues, const float *values, uint64_t numRays,
+                       const float *org, const float *dir, const float *tRanges, float time, const float *tf,
+                       uint32_t tfSize, float tfLo, float tfHi, uint32_t maxHits, float *rgba, uint32_t *count);
+
 #ifdef __cplusplus
 }
 #endif
