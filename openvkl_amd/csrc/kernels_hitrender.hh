@@ -5,7 +5,8 @@
 // kernels_hitrender_sparse.hip (vdb and temporal sources).
 //
 // The primary ray and its shadow rays share one hit-iterator state and one
-// hitNext call site (the primary state is parked while a shadow ray runs),
+// hitNext call site (the primary state is parked while a hit's shadow rays
+// run, and only when the hit casts one),
 // so the large hit-iteration code is inlined once per specialisation.
 #pragma once
 
@@ -41,7 +42,7 @@ __global__ __launch_bounds__(kIterBlock) void kHitRender(const IteratorContextVi
   hitInit(c, st, o, d, rayRange(m.tRanges, r), m.time);
   float cr = 0.f, cg = 0.f, cb = 0.f, alpha = 0.f;
   uint32_t hits = 0;
-  bool shadowRay = false;
+  bool shadowRay = false, isParked = false;
   int light      = 0;  // next light of the hit being shaded
   F3 p = {0.f, 0.f, 0.f}, N = {0.f, 0.f, 0.f};
   float ci = 0.f, co = 0.f, illum = 0.f, hitSample = 0.f, hitEps = 0.f;
@@ -71,7 +72,6 @@ __global__ __launch_bounds__(kIterBlock) void kHitRender(const IteratorContextVi
         illum = 1.f;  // no normal: unshaded
         light = 2;
       }
-      parked = st;
     } else {
       if (!got) illum += fabsf(co) * (light == 0 ? 1.f : 0.8f);
       light++;
@@ -84,6 +84,10 @@ __global__ __launch_bounds__(kIterBlock) void kHitRender(const IteratorContextVi
       light++;
     }
     if (light < 2) {
+      if (!isParked) {  // first shadow ray of this hit: park the primary ray
+        parked   = st;
+        isParked = true;
+      }
       hitInit(c, st, p, light == 0 ? light0 : light1, Range1f{hitEps, vinf()}, m.time);
       shadowRay = true;
       continue;
@@ -94,7 +98,10 @@ __global__ __launch_bounds__(kIterBlock) void kHitRender(const IteratorContextVi
     cg    = cg + (1.f - alpha) * illum * albedo[1];
     cb    = cb + (1.f - alpha) * illum * albedo[2];
     alpha = alpha + (1.f - alpha) * 0.6f;
-    st    = parked;
+    if (isParked) {
+      st       = parked;
+      isParked = false;
+    }
   }
   m.rgba[4 * r]     = cr;
   m.rgba[4 * r + 1] = cg;
