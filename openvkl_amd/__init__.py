@@ -205,6 +205,20 @@ def _ptr(a):
     return a.data_ptr()
 
 
+def _tf_rows(tf):
+    """A transfer function as contiguous float32 rows (r, g, b, opacity):
+    numpy or torch, [n, 4] or flat [4n]. Other element types are converted;
+    a size that is not a multiple of 4 raises."""
+    size = int(tf.size if isinstance(tf, np.ndarray) else tf.numel())
+    if size == 0 or size % 4:
+        raise ValueError(f"a transfer function holds (r, g, b, opacity) rows; got {size} values")
+    if isinstance(tf, np.ndarray):
+        return np.ascontiguousarray(tf, np.float32).reshape(-1, 4)
+    import torch
+
+    return tf.to(torch.float32).reshape(-1, 4).contiguous()
+
+
 def _n_points(a):
     return int(a.shape[0])
 
@@ -304,6 +318,9 @@ class Device:
         v.set_vec3f("gridOrigin", *origin)
         v.set_vec3f("gridSpacing", *spacing)
         datas = [self.data(a) if isinstance(a, np.ndarray) else self.data(a.reshape(-1)) for a in arrs]
+        # Device tensors are shared (VKL_DATA_SHARED_BUFFER): the volume reads
+        # them in place, so it keeps them alive.
+        v._shared = [d._keep for d in datas if d._keep is not None]
         if len(datas) == 1:
             v.set_data("data", datas[0])
         else:
@@ -351,6 +368,7 @@ class Device:
         v.set_data("node.format", d(np.ascontiguousarray(fmt, np.uint32)))
         if packed_dense is not None:
             pd = self.data_of_data([d(a) if isinstance(a, np.ndarray) else d(a.reshape(-1)) for a in packed_dense])
+            v._shared = [h._keep for h in tmp if h._keep is not None]
             tmp.append(pd)
             v.set_data("nodesPackedDense", pd)
             if packed_tile is not None:
@@ -586,9 +604,9 @@ class Sampler(Handle):
                 import torch
 
                 cnt = torch.empty(r, dtype=torch.int32, device=org.device)
-        tfa = tf.contiguous() if hasattr(tf, "contiguous") else np.ascontiguousarray(tf, np.float32).reshape(-1, 4)
+        tfa = _tf_rows(tf)
         lib().vklComputeWoodcockN(C.byref(self.h), r, _ptr(org), _ptr(dir), _ptr(t_ranges), _ptr(rng),
-                                  float(sigma_max), _ptr(tfa), len(tfa), range1f(*[float(x) for x in tf_range]),
+                                  float(sigma_max), _ptr(tfa), int(tfa.shape[0]), range1f(*[float(x) for x in tf_range]),
                                   float(time), float(shutter), int(max_collisions), attr, _ptr(ev), _ptr(sm),
                                   _ptr(cnt))
         self.device.check()
@@ -717,9 +735,9 @@ class IteratorContext(Handle):
 
             out = torch.empty((r, 4), dtype=torch.float32, device=org.device)
             cnt = torch.empty(r, dtype=torch.int32, device=org.device)
-        tfa = tf.contiguous() if hasattr(tf, "contiguous") else np.ascontiguousarray(tf, np.float32).reshape(-1, 4)
+        tfa = _tf_rows(tf)
         lib().vklComputeHitRenderN(C.byref(self.h), r, _ptr(org), _ptr(dir), _ptr(t_ranges), float(time), _ptr(tfa),
-                                   len(tfa), range1f(*[float(x) for x in tf_range]), int(max_hits), _ptr(out),
+                                   int(tfa.shape[0]), range1f(*[float(x) for x in tf_range]), int(max_hits), _ptr(out),
                                    _ptr(cnt))
         self.device.check()
         return out, cnt
@@ -737,9 +755,9 @@ class IteratorContext(Handle):
 
             out = torch.empty((r, 4), dtype=torch.float32, device=org.device)
             cnt = torch.empty(r, dtype=torch.int32, device=org.device) if counts else None
-        tfa = tf.contiguous() if hasattr(tf, "contiguous") else np.ascontiguousarray(tf, np.float32).reshape(-1, 4)
+        tfa = _tf_rows(tf)
         lib().vklComputeRayMarchIntervalsN(C.byref(self.h), r, _ptr(org), _ptr(dir), _ptr(t_ranges), float(time),
-                                           float(sampling_rate), _ptr(tfa), len(tfa),
+                                           float(sampling_rate), _ptr(tfa), int(tfa.shape[0]),
                                            range1f(*[float(x) for x in tf_range]), int(max_samples), _ptr(out),
                                            _ptr(cnt))
         self.device.check()

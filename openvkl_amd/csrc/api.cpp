@@ -54,6 +54,7 @@ void Sampler::commit()
 
 void Sampler::applyObservation()
 {
+  std::lock_guard<std::mutex> lock(viewMutex);
   const bool observed    = liveObservers > 0 && accessBuf.ptr;
   view.grid.leafAccess   = observed ? accessBuf.as<uint32_t>() : nullptr;
   view.grid.accessStamp  = accessEpoch;
@@ -80,13 +81,17 @@ IteratorContext::IteratorContext(Device *d, Sampler *s, bool hit) : ManagedObjec
 
 IteratorContext::~IteratorContext() { sampler->release(); }
 
-void IteratorContext::refreshSampler()
+IteratorContextView IteratorContext::refreshSampler()
 {
-  if (samplerVersion == sampler->viewVersion) return;
-  view.sampler   = sampler->view;
-  samplerVersion = sampler->viewVersion;
-  device->makeCurrent();
-  deviceView.upload(&view, sizeof(IteratorContextView), device->stream);
+  std::lock_guard<std::mutex> lock(viewMutex);
+  std::lock_guard<std::mutex> slock(sampler->viewMutex);
+  if (samplerVersion != sampler->viewVersion) {
+    view.sampler   = sampler->view;
+    samplerVersion = sampler->viewVersion;
+    device->makeCurrent();
+    deviceView.upload(&view, sizeof(IteratorContextView), device->stream);
+  }
+  return view;
 }
 
 namespace {
@@ -304,42 +309,66 @@ BinOp sampleOp(int filter)
   return filter == kFilterNearest ? kBinSampleNearest : filter == kFilterCubic ? kBinSampleCubic : kBinSampleLinear;
 }
 
-// Scratch of a binned call (sampler-owned, grown on demand).
-BinScratch binnedScratch(Sampler *s, const BinGeometry &geo, uint64_t n)
+// Byte layout of a binned call's scratch at base: records, segment counters,
+// overflow list, overflow count. Returns the bytes it needs (base may be null
+// to only size it).
+uint64_t binScratchLayout(const BinGeometry &geo, uint64_t n, uint8_t *base, BinScratch &sc)
 {
   const uint64_t segs = (uint64_t)geo.numBins * 8;
   const uint64_t cap  = (n + segs - 1) / segs * 5 / 4 + 64;
   auto align          = [](uint64_t x) { return (x + 255) & ~255ull; };
   const uint64_t recBytes = align(segs * cap * 16), cntBytes = align(segs * 4), ovBytes = align(n * 4);
-  const uint64_t total    = recBytes + cntBytes + ovBytes + 256;
-  if (s->binBuf.bytes < total) s->binBuf.allocate(total);
-  uint8_t *base = s->binBuf.as<uint8_t>();
-  BinScratch sc;
   sc.records       = reinterpret_cast<float4 *>(base);
   sc.segCount      = reinterpret_cast<uint32_t *>(base + recBytes);
   sc.overflow      = reinterpret_cast<uint32_t *>(base + recBytes + cntBytes);
   sc.overflowCount = reinterpret_cast<uint32_t *>(base + recBytes + cntBytes + ovBytes);
   sc.capacity      = (uint32_t)cap;
+  return recBytes + cntBytes + ovBytes + 256;
+}
+
+// Scratch of a binned call (sampler-owned, grown on demand).
+BinScratch binnedScratch(Sampler *s, const BinGeometry &geo, uint64_t n)
+{
+  BinScratch sc;
+  const uint64_t total = binScratchLayout(geo, n, nullptr, sc);
+  if (s->binBuf.bytes < total) s->binBuf.allocate(total);
+  binScratchLayout(geo, n, s->binBuf.as<uint8_t>(), sc);
   return sc;
 }
 
-// One attribute into out[i * stride]: binned or direct.
+// One attribute into out[i * stride]: binned or direct. The binned kernels
+// use the sampler's scratch, or with fixedScratch the caller's buffer of
+// fixedBytes (calls whose scratch does not fit are rejected).
 void sampleInto(Sampler *s, uint32_t attr, uint64_t n, const float *xyz, float *out, uint32_t stride,
-                const float *times)
+                const float *times, uint8_t *fixedScratch = nullptr, uint64_t fixedBytes = 0)
 {
-  Device *d         = s->device;
-  const AttrView &a = s->attr(attr);
+  Device *d            = s->device;
+  const AttrView &a    = s->attr(attr);
+  const SamplerView sv = s->viewCopy();
   BinGeometry geo;
-  const BinOp op = sampleOp(s->view.filter);
+  const BinOp op = sampleOp(sv.filter);
   if (binnedGeometry(s, a, n, op, geo)) {
+    if (fixedScratch) {
+      BinScratch sc;
+      const uint64_t need = binScratchLayout(geo, n, nullptr, sc);
+      if (need > fixedBytes)
+        throw VklError("binned sampling of " + std::to_string(n) + " points over " + std::to_string(geo.numBins) +
+                           " bins needs " + std::to_string(need) + " bytes of scratch; the fixed scratch holds " +
+                           std::to_string(fixedBytes),
+                       VKL_INVALID_ARGUMENT);
+      binScratchLayout(geo, n, fixedScratch, sc);
+      checkHip(launchBinned(sv, a, geo, op, n, xyz, out, stride, sc, d->stream), "binned sample kernels");
+      syncDevice(d);
+      return;
+    }
     std::lock_guard<std::mutex> lock(s->binMutex);
     const BinScratch sc = binnedScratch(s, geo, n);
-    checkHip(launchBinned(s->view, a, geo, op, n, xyz, out, stride, sc, d->stream), "binned sample kernels");
+    checkHip(launchBinned(sv, a, geo, op, n, xyz, out, stride, sc, d->stream), "binned sample kernels");
     // The scratch is reused by the next call: finish before releasing it.
     syncDevice(d);
     return;
   }
-  checkHip(launchSampleStrided(s->view, a, attr, n, xyz, out, stride, d->stream, times), "sample kernel");
+  checkHip(launchSampleStrided(sv, a, attr, n, xyz, out, stride, d->stream, times), "sample kernel");
 }
 
 // times: n floats (host or device) or null (time 0); only temporal volumes
@@ -363,22 +392,25 @@ void gradientN(Sampler *s, uint64_t n, const vkl_vec3f *oc, vkl_vec3f *out, uint
 {
   Device *d = s->device;
   d->makeCurrent();
-  const AttrView &a = s->attr(attr);
+  const AttrView &a    = s->attr(attr);
+  const SamplerView sv = s->viewCopy();
   if (n == 0) return;
-  const bool temporal = s->view.grid.temporalFormat != 0 && times;
+  const bool temporal = sv.grid.temporalFormat != 0 && times;
   Staged in(oc, n * 12, d->stream, true);
   Staged tm(temporal ? times : nullptr, temporal ? n * 4 : 0, d->stream, true);
   Staged o(out, n * 12, d->stream, false);
   BinGeometry geo;
-  const int gf   = s->view.gradientFilter;
+  const int gf   = sv.gradientFilter;
   const BinOp op = gf == kFilterCubic ? kBinGradientCubic : kBinGradientLinear;
   if (gf != kFilterNearest && binnedGeometry(s, a, n, op, geo)) {
     std::lock_guard<std::mutex> lock(s->binMutex);
     const BinScratch sc = binnedScratch(s, geo, n);
-    checkHip(launchBinned(s->view, a, geo, op, n, in.as<float>(), o.as<float>(), 1, sc, d->stream),
+    checkHip(launchBinned(sv, a, geo, op, n, in.as<float>(), o.as<float>(), 1, sc, d->stream),
              "binned gradient kernels");
+    // The scratch is reused by the next call: finish before releasing it.
+    syncDevice(d);
   } else {
-    checkHip(launchGradient(s->view, a, attr, n, in.as<float>(), o.as<float>(), d->stream, tm.as<float>()),
+    checkHip(launchGradient(sv, a, attr, n, in.as<float>(), o.as<float>(), d->stream, tm.as<float>()),
              "gradient kernel");
   }
   o.copyOut();
@@ -517,7 +549,6 @@ IteratorContext *contextOf(const VKLObject *c, bool hit)
   IteratorContext *ctx = static_cast<IteratorContext *>(c->host);
   if (ctx->isHit != hit) throw VklError("wrong iterator context type");
   if (!ctx->committed) throw VklError("iterator context is not committed");
-  ctx->refreshSampler();
   return ctx;
 }
 
@@ -530,7 +561,9 @@ void initInterval(IteratorContext *ctx, const vkl_vec3f *o, const vkl_vec3f *dir
   float ray[9] = {o->x, o->y, o->z, dir->x, dir->y, dir->z, t ? t->lower : 0.f,
                   t ? t->upper : std::numeric_limits<float>::infinity(), time};
   checkHip(hipMemcpyAsync(s, ray, sizeof(ray), hipMemcpyHostToDevice, d->stream), "hipMemcpyAsync");
-  checkHip(launchIntervalInit(ctx->view, reinterpret_cast<IntervalState *>(s + 64), reinterpret_cast<float *>(s),
+  (void)ctx->refreshSampler();
+  checkHip(launchIntervalInit(ctx->deviceView.as<IteratorContextView>(), reinterpret_cast<IntervalState *>(s + 64),
+                              reinterpret_cast<float *>(s),
                               d->stream),
            "interval init");
   checkHip(hipMemcpyAsync(&buf->state, s + 64, sizeof(IntervalState), hipMemcpyDeviceToHost, d->stream), "copy");
@@ -542,14 +575,14 @@ void initInterval(IteratorContext *ctx, const vkl_vec3f *o, const vkl_vec3f *dir
 int iterateInterval(IntervalBuffer *buf, VKLInterval *out)
 {
   if (buf->magic != kIntervalMagic) throw VklError("invalid interval iterator");
-  IteratorContext *ctx = buf->ctx;
-  ctx->refreshSampler();
-  Device *d            = ctx->device;
+  IteratorContext *ctx         = buf->ctx;
+  (void)ctx->refreshSampler();
+  Device *d = ctx->device;
   d->makeCurrent();
   uint8_t *s = stepScratch(d);
   // Layout: [0,64) result, [64, ...) state, [1024, ...) interval.
   checkHip(hipMemcpyAsync(s + 64, &buf->state, sizeof(IntervalState), hipMemcpyHostToDevice, d->stream), "copy");
-  checkHip(launchIntervalNext(ctx->view, reinterpret_cast<IntervalState *>(s + 64),
+  checkHip(launchIntervalNext(ctx->deviceView.as<IteratorContextView>(), reinterpret_cast<IntervalState *>(s + 64),
                               reinterpret_cast<IntervalOut *>(s + 1024), reinterpret_cast<int *>(s), d->stream),
            "interval step");
   struct
@@ -574,6 +607,7 @@ void initHit(IteratorContext *ctx, const vkl_vec3f *o, const vkl_vec3f *dir, con
   float ray[9] = {o->x, o->y, o->z, dir->x, dir->y, dir->z, t ? t->lower : 0.f,
                   t ? t->upper : std::numeric_limits<float>::infinity(), time};
   checkHip(hipMemcpyAsync(s, ray, sizeof(ray), hipMemcpyHostToDevice, d->stream), "hipMemcpyAsync");
+  (void)ctx->refreshSampler();
   checkHip(launchHitInit(ctx->deviceView.as<IteratorContextView>(), reinterpret_cast<HitState *>(s + 64), reinterpret_cast<float *>(s), d->stream),
            "hit init");
   checkHip(hipMemcpyAsync(&buf->state, s + 64, sizeof(HitState), hipMemcpyDeviceToHost, d->stream), "copy");
@@ -585,14 +619,14 @@ void initHit(IteratorContext *ctx, const vkl_vec3f *o, const vkl_vec3f *dir, con
 int iterateHit(HitBuffer *buf, VKLHit *out)
 {
   if (buf->magic != kHitMagic) throw VklError("invalid hit iterator");
-  IteratorContext *ctx = buf->ctx;
-  ctx->refreshSampler();
-  Device *d            = ctx->device;
+  IteratorContext *ctx         = buf->ctx;
+  const IteratorContextView cv = ctx->refreshSampler();
+  Device *d                    = ctx->device;
   d->makeCurrent();
   uint8_t *s = stepScratch(d);
   checkHip(hipMemcpyAsync(s + 64, &buf->state, sizeof(HitState), hipMemcpyHostToDevice, d->stream), "copy");
-  checkHip(launchHitNext(ctx->view, ctx->deviceView.as<IteratorContextView>(),
-                         ctx->sampler->attr(ctx->view.attributeIndex), reinterpret_cast<HitState *>(s + 64), reinterpret_cast<HitOut *>(s + 1024),
+  checkHip(launchHitNext(cv, ctx->deviceView.as<IteratorContextView>(),
+                         ctx->sampler->attr(cv.attributeIndex), reinterpret_cast<HitState *>(s + 64), reinterpret_cast<HitOut *>(s + 1024),
                          reinterpret_cast<int *>(s), d->stream),
            "hit step");
   int result = 0;
@@ -1258,7 +1292,7 @@ OPENVKL_INTERFACE void vklComputeSampleNAsync(const VKLSampler *s, unsigned int 
   VKL_TRY(samplerOf(s)->device)
   Sampler *sp = samplerOf(s);
   dev_->makeCurrent();
-  checkHip(launchSample(sp->view, sp->attr(a), a, N, reinterpret_cast<const float *>(oc), out,
+  checkHip(launchSample(sp->viewCopy(), sp->attr(a), a, N, reinterpret_cast<const float *>(oc), out,
                         static_cast<hipStream_t>(stream)),
            "sample kernel");
   VKL_CATCH()
@@ -1279,7 +1313,7 @@ OPENVKL_INTERFACE void vklComputeRayMarchN(const VKLSampler *s, unsigned int num
   Staged o(origins, numRays * 12ull, st, true), d(directions, numRays * 12ull, st, true);
   Staged t(tRanges, numRays * 8ull, st, true);
   Staged out(integrals, numRays * 4ull, st, false);
-  checkHip(launchRayMarch(sp->view, av, a, numRays, o.as<float>(), d.as<float>(), t.as<float>(), numSteps,
+  checkHip(launchRayMarch(sp->viewCopy(), av, a, numRays, o.as<float>(), d.as<float>(), t.as<float>(), numSteps,
                           out.as<float>(), st),
            "ray march kernel");
   out.copyOut();
@@ -1325,7 +1359,7 @@ OPENVKL_INTERFACE void vklComputeWoodcockN(const VKLSampler *s, unsigned int num
   w.eventT        = outT.as<float>();
   w.samples       = outS.as<float>();
   w.count         = cnt.as<uint32_t>();
-  checkHip(launchWoodcock(sp->view, av, a, w, st), "woodcock kernel");
+  checkHip(launchWoodcock(sp->viewCopy(), av, a, w, st), "woodcock kernel");
   rng.copyOut();
   outT.copyOut();
   outS.copyOut();
@@ -1340,7 +1374,7 @@ OPENVKL_INTERFACE void vklComputeGradientNAsync(const VKLSampler *s, unsigned in
   VKL_TRY(samplerOf(s)->device)
   Sampler *sp = samplerOf(s);
   dev_->makeCurrent();
-  checkHip(launchGradient(sp->view, sp->attr(a), a, N, reinterpret_cast<const float *>(oc),
+  checkHip(launchGradient(sp->viewCopy(), sp->attr(a), a, N, reinterpret_cast<const float *>(oc),
                           reinterpret_cast<float *>(out), static_cast<hipStream_t>(stream)),
            "gradient kernel");
   VKL_CATCH()
@@ -1361,7 +1395,8 @@ OPENVKL_INTERFACE void vklIterateIntervalsBatch(const VKLIntervalIteratorContext
   Staged t(tRanges, tRanges ? numRays * 8ull : 0, st, true);
   Staged out(intervals, (uint64_t)numRays * maxIntervalsPerRay * sizeof(VKLInterval), st, false);
   Staged cnt(numIntervals, numRays * 4ull, st, false);
-  checkHip(launchIntervalBatch(ctx->view, numRays, o.as<float>(), d.as<float>(), t.as<float>(),
+  (void)ctx->refreshSampler();
+  checkHip(launchIntervalBatch(ctx->deviceView.as<IteratorContextView>(), numRays, o.as<float>(), d.as<float>(), t.as<float>(),
                                out.as<IntervalOut>(), maxIntervalsPerRay, cnt.as<uint32_t>(), st),
            "interval batch");
   out.copyOut();
@@ -1385,8 +1420,8 @@ OPENVKL_INTERFACE void vklIterateHitsBatch(const VKLHitIteratorContext *c, unsig
   Staged tm(times, times ? numRays * 4ull : 0, st, true);
   Staged out(hits, (uint64_t)numRays * maxHitsPerRay * sizeof(VKLHit), st, false);
   Staged cnt(numHits, numRays * 4ull, st, false);
-  checkHip(launchHitBatch(ctx->view, ctx->deviceView.as<IteratorContextView>(),
-                          ctx->sampler->attr(ctx->view.attributeIndex), numRays, o.as<float>(), d.as<float>(), t.as<float>(), tm.as<float>(),
+  const IteratorContextView cv = ctx->refreshSampler();
+  checkHip(launchHitBatch(cv, ctx->deviceView.as<IteratorContextView>(), ctx->sampler->attr(cv.attributeIndex), numRays, o.as<float>(), d.as<float>(), t.as<float>(), tm.as<float>(),
                           out.as<HitOut>(), maxHitsPerRay, cnt.as<uint32_t>(), st),
            "hit batch");
   out.copyOut();
@@ -1428,8 +1463,9 @@ OPENVKL_INTERFACE void vklComputeRayMarchIntervalsN(const VKLIntervalIteratorCon
   m.tf           = tf.as<float>();
   m.rgba         = out.as<float>();
   m.count        = cnt.as<uint32_t>();
-  checkHip(launchRayMarchIntervals(ctx->view, ctx->deviceView.as<IteratorContextView>(),
-                                   ctx->sampler->attr(ctx->view.attributeIndex), m, st),
+  const IteratorContextView cv = ctx->refreshSampler();
+  checkHip(launchRayMarchIntervals(cv, ctx->deviceView.as<IteratorContextView>(), ctx->sampler->attr(cv.attributeIndex),
+                                   m, st),
            "interval ray march");
   out.copyOut();
   cnt.copyOut();
@@ -1468,8 +1504,9 @@ OPENVKL_INTERFACE void vklComputeHitRenderN(const VKLHitIteratorContext *c, unsi
   m.tf      = tf.as<float>();
   m.rgba    = out.as<float>();
   m.count   = cnt.as<uint32_t>();
-  checkHip(launchHitRender(ctx->view, ctx->deviceView.as<IteratorContextView>(),
-                           ctx->sampler->attr(ctx->view.attributeIndex), m, st),
+  const IteratorContextView cv = ctx->refreshSampler();
+  checkHip(launchHitRender(cv, ctx->deviceView.as<IteratorContextView>(), ctx->sampler->attr(cv.attributeIndex), m,
+                           st),
            "hit render");
   out.copyOut();
   cnt.copyOut();

@@ -76,6 +76,10 @@ hipError_t launchHitRender(const IteratorContextView &hostView, const IteratorCo
                            const HitRenderArgs &m, hipStream_t stream);
 hipError_t launchHitRenderSparse(const IteratorContextView &hostView, const IteratorContextView *c,
                                  const AttrView &a, const HitRenderArgs &m, hipStream_t stream);
+hipError_t launchHitRenderTemporal(const IteratorContextView &hostView, const IteratorContextView *c,
+                                   const AttrView &a, const HitRenderArgs &m, hipStream_t stream);
+hipError_t launchHitRenderObserved(const IteratorContextView &hostView, const IteratorContextView *c,
+                                   const AttrView &a, const HitRenderArgs &m, hipStream_t stream);
 
 // Copy an x-fastest dims[0] x dims[1] x dims[2] array of elemSize-byte
 // elements into brick layout L (total = padded element count).
@@ -103,8 +107,10 @@ hipError_t launchLeafRanges(const GridView &g, uint32_t attr, uint64_t numLeaves
 
 // Single-ray iterator steps for the scalar API (state in device memory).
 // ray: origin (3), direction (3), tRange (2), time (1) in device memory.
-hipError_t launchIntervalInit(const IteratorContextView &c, IntervalState *state, const float *ray, hipStream_t stream);
-hipError_t launchIntervalNext(const IteratorContextView &c, IntervalState *state, IntervalOut *out, int *result,
+// Interval steps and batches take the context in device memory
+// (IteratorContext::deviceView).
+hipError_t launchIntervalInit(const IteratorContextView *c, IntervalState *state, const float *ray, hipStream_t stream);
+hipError_t launchIntervalNext(const IteratorContextView *c, IntervalState *state, IntervalOut *out, int *result,
                               hipStream_t stream);
 // Hit steps take the context in device memory (IteratorContext::deviceView);
 // hostView (the same context on the host) and the attribute's view select
@@ -116,7 +122,7 @@ hipError_t launchHitNext(const IteratorContextView &hostView, const IteratorCont
 // Batched iteration: one ray per thread; ray r writes up to maxOut results at
 // out[r*maxOut], count[r] = number written. org / dir: 3 floats per ray;
 // tRanges: 2 floats per ray (null = [0, inf)); times may be null.
-hipError_t launchIntervalBatch(const IteratorContextView &c, uint32_t numRays, const float *org, const float *dir,
+hipError_t launchIntervalBatch(const IteratorContextView *c, uint32_t numRays, const float *org, const float *dir,
                                const float *tRanges, IntervalOut *out, uint32_t maxOut, uint32_t *count,
                                hipStream_t stream);
 hipError_t launchHitBatch(const IteratorContextView &hostView, const IteratorContextView *c, const AttrView &a,
@@ -124,8 +130,9 @@ hipError_t launchHitBatch(const IteratorContextView &hostView, const IteratorCon
                           const float *times, HitOut *out, uint32_t maxOut, uint32_t *count, hipStream_t stream);
 // The same for structuredRegular volumes of half / float / double voxels
 // (kernels_hit_dense.hip), for temporal structuredRegular and vdb volumes
-// (kernels_hit_other.hip) and for observed vdb samplers
-// (kernels_hit_observed.hip); launchHitNext / launchHitBatch forward to them.
+// (kernels_hit_temporal.hip), for vdb volumes (kernels_hit_other.hip) and for
+// observed vdb samplers (kernels_hit_observed.hip); launchHitNext /
+// launchHitBatch forward to them.
 hipError_t launchHitNextObserved(const IteratorContextView &hostView, const IteratorContextView *c,
                                  const AttrView &a, HitState *state, HitOut *out, int *result, hipStream_t stream);
 hipError_t launchHitBatchObserved(const IteratorContextView &hostView, const IteratorContextView *c,
@@ -138,6 +145,12 @@ hipError_t launchHitBatchDenseFloat(const IteratorContextView &hostView, const I
                                     const AttrView &a, uint32_t numRays, const float *org, const float *dir,
                                     const float *tRanges, const float *times, HitOut *out, uint32_t maxOut,
                                     uint32_t *count, hipStream_t stream);
+hipError_t launchHitNextTemporal(const IteratorContextView &hostView, const IteratorContextView *c,
+                                 const AttrView &a, HitState *state, HitOut *out, int *result, hipStream_t stream);
+hipError_t launchHitBatchTemporal(const IteratorContextView &hostView, const IteratorContextView *c,
+                                  const AttrView &a, uint32_t numRays, const float *org, const float *dir,
+                                  const float *tRanges, const float *times, HitOut *out, uint32_t maxOut,
+                                  uint32_t *count, hipStream_t stream);
 hipError_t launchHitNextOther(const IteratorContextView &hostView, const IteratorContextView *c, const AttrView &a,
                               HitState *state, HitOut *out, int *result, hipStream_t stream);
 hipError_t launchHitBatchOther(const IteratorContextView &hostView, const IteratorContextView *c, const AttrView &a,

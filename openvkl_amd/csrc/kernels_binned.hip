@@ -314,6 +314,13 @@ hipError_t launchBinnedOp(const GridView &g, const AttrView &a, const BinGeometr
   }
 }
 
+__global__ void kZeroCounters(uint32_t *__restrict__ seg, uint32_t n, uint32_t *__restrict__ overflowCount)
+{
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i < n) seg[i] = 0u;
+  if (i == 0) *overflowCount = 0u;
+}
+
 void haloOf(BinOp op, uint32_t &lo, uint32_t &hi)
 {
   const bool cubic = op == kBinSampleCubic || op == kBinGradientCubic;
@@ -358,9 +365,12 @@ hipError_t launchBinned(const SamplerView &sv, const AttrView &a, const BinGeome
                         const float *xyz, float *out, uint32_t outStride, const BinScratch &s, hipStream_t st)
 {
   if (n == 0) return hipSuccess;
-  hipError_t e = hipMemsetAsync(s.segCount, 0, (size_t)b.numBins * kXcds * sizeof(uint32_t), st);
-  if (e != hipSuccess) return e;
-  e = hipMemsetAsync(s.overflowCount, 0, sizeof(uint32_t), st);
+  // The segment counters and the overflow count start at zero. A fill kernel
+  // rather than hipMemsetAsync, so that callers may capture the call in a
+  // graph.
+  const uint32_t nc = b.numBins * kXcds;
+  hipLaunchKernelGGL(kZeroCounters, dim3((nc + 255) / 256), dim3(256), 0, st, s.segCount, nc, s.overflowCount);
+  hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   const uint64_t perBlock = (uint64_t)kScatterBlock * kScatterPts;
   const unsigned blocks   = (unsigned)((n + perBlock - 1) / perBlock);

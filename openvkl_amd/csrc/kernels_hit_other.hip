@@ -1,7 +1,6 @@
-// Hit iteration over temporal structuredRegular volumes and over vdb volumes:
-// the kernels of kernels_hit.hh instantiated for those sources, in a
-// translation unit of their own so that they compile beside kernels_iter.hip
-// (which instantiates them for the other dense sources).
+// Hit iteration over vdb volumes: the kernels of kernels_hit.hh instantiated
+// for unobserved vdb sources, in a translation unit of their own so that they
+// compile beside the others; temporal sources go to kernels_hit_temporal.hip.
 
 #include "kernels.h"
 #include "kernels_hit.hh"
@@ -11,8 +10,9 @@ namespace vklhip {
 hipError_t launchHitNextOther(const IteratorContextView &hostView, const IteratorContextView *c, const AttrView &a,
                               HitState *state, HitOut *out, int *result, hipStream_t st)
 {
-  return dispatchSource<kTemporalKinds | kSparseKinds>(hostView.sampler.grid, a, hostView.sampler.filter,
-                                                       HitNextLaunch{c, a, state, out, result, st});
+  if (hostView.sampler.grid.temporalFormat) return launchHitNextTemporal(hostView, c, a, state, out, result, st);
+  return dispatchSource<kSparseKinds>(hostView.sampler.grid, a, hostView.sampler.filter,
+                                      HitNextLaunch{c, a, state, out, result, st});
 }
 
 hipError_t launchHitBatchOther(const IteratorContextView &hostView, const IteratorContextView *c, const AttrView &a,
@@ -20,9 +20,11 @@ hipError_t launchHitBatchOther(const IteratorContextView &hostView, const Iterat
                                const float *times, HitOut *out, uint32_t maxOut, uint32_t *count, hipStream_t st)
 {
   if (numRays == 0) return hipSuccess;
-  return dispatchSource<kTemporalKinds | kSparseKinds>(hostView.sampler.grid, a, hostView.sampler.filter,
-                                                       HitBatchLaunch{c, a, numRays, org, dir, tRanges, times, out,
-                                                                      maxOut, count, st});
+  if (hostView.sampler.grid.temporalFormat)
+    return launchHitBatchTemporal(hostView, c, a, numRays, org, dir, tRanges, times, out, maxOut, count, st);
+  return dispatchSource<kSparseKinds>(hostView.sampler.grid, a, hostView.sampler.filter,
+                                      HitBatchLaunch{c, a, numRays, org, dir, tRanges, times, out, maxOut, count,
+                                                     st});
 }
 
 }  // namespace vklhip

@@ -1,0 +1,28 @@
+// Hit iteration over temporal (motion-blur) structuredRegular and vdb
+// volumes: the kernels of kernels_hit.hh for those sources, in a translation
+// unit of their own so that they compile beside the others.
+
+#include "kernels.h"
+#include "kernels_hit.hh"
+
+namespace vklhip {
+
+hipError_t launchHitNextTemporal(const IteratorContextView &hostView, const IteratorContextView *c,
+                                 const AttrView &a, HitState *state, HitOut *out, int *result, hipStream_t st)
+{
+  return dispatchSource<kTemporalKinds>(hostView.sampler.grid, a, hostView.sampler.filter,
+                                        HitNextLaunch{c, a, state, out, result, st});
+}
+
+hipError_t launchHitBatchTemporal(const IteratorContextView &hostView, const IteratorContextView *c,
+                                  const AttrView &a, uint32_t numRays, const float *org, const float *dir,
+                                  const float *tRanges, const float *times, HitOut *out, uint32_t maxOut,
+                                  uint32_t *count, hipStream_t st)
+{
+  if (numRays == 0) return hipSuccess;
+  return dispatchSource<kTemporalKinds>(hostView.sampler.grid, a, hostView.sampler.filter,
+                                        HitBatchLaunch{c, a, numRays, org, dir, tRanges, times, out, maxOut, count,
+                                                       st});
+}
+
+}  // namespace vklhip

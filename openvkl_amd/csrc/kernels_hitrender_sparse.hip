@@ -1,4 +1,5 @@
-// Hit-iterator renderer kernels for vdb and temporal sources.
+// Hit-iterator renderer kernels for vdb sources; temporal and observed vdb
+// sources have units of their own, so the three compile in parallel.
 #include "kernels_hitrender.hh"
 
 namespace vklhip {
@@ -6,8 +7,10 @@ namespace vklhip {
 hipError_t launchHitRenderSparse(const IteratorContextView &hostView, const IteratorContextView *c,
                                  const AttrView &a, const HitRenderArgs &m, hipStream_t st)
 {
-  return dispatchSource<kSparseKinds | kTemporalKinds | kSparseObservedKinds>(
-      hostView.sampler.grid, a, hostView.sampler.filter, HitRenderLaunch{c, a, m, st});
+  const GridView &g = hostView.sampler.grid;
+  if (g.temporalFormat) return launchHitRenderTemporal(hostView, c, a, m, st);
+  if (g.leafAccess) return launchHitRenderObserved(hostView, c, a, m, st);
+  return dispatchSource<kSparseKinds>(g, a, hostView.sampler.filter, HitRenderLaunch{c, a, m, st});
 }
 
 }  // namespace vklhip

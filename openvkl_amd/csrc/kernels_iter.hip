@@ -155,13 +155,19 @@ hipError_t launchLeafRanges(const GridView &g, uint32_t attr, uint64_t numLeaves
 
 // ---- single-ray steps ----------------------------------------------------------
 
-__global__ void kIntervalInit(IteratorContextView c, IntervalState *s, const float *ray)
+// The iterator kernels read their context through a pointer to device
+// memory: a context passed by value is copied to scratch memory as soon as
+// any field is indexed at run time.
+__global__ void kIntervalInit(const IteratorContextView *__restrict__ cp, IntervalState *s, const float *ray)
 {
+  const IteratorContextView &c = *cp;
   intervalInit(c, *s, F3{ray[0], ray[1], ray[2]}, F3{ray[3], ray[4], ray[5]}, Range1f{ray[6], ray[7]});
 }
 
-__global__ void kIntervalNext(IteratorContextView c, IntervalState *s, IntervalOut *out, int *result)
+__global__ void kIntervalNext(const IteratorContextView *__restrict__ cp, IntervalState *s, IntervalOut *out,
+                              int *result)
 {
+  const IteratorContextView &c = *cp;
   IntervalState st = *s;
   IntervalOut o;
   const bool r = intervalNext(c, st, o);
@@ -177,12 +183,12 @@ __global__ void kHitInit(const IteratorContextView *__restrict__ cp, HitState *s
 }
 
 
-hipError_t launchIntervalInit(const IteratorContextView &c, IntervalState *state, const float *ray, hipStream_t st)
+hipError_t launchIntervalInit(const IteratorContextView *c, IntervalState *state, const float *ray, hipStream_t st)
 {
   hipLaunchKernelGGL(kIntervalInit, dim3(1), dim3(1), 0, st, c, state, ray);
   return hipGetLastError();
 }
-hipError_t launchIntervalNext(const IteratorContextView &c, IntervalState *state, IntervalOut *out, int *result,
+hipError_t launchIntervalNext(const IteratorContextView *c, IntervalState *state, IntervalOut *out, int *result,
                               hipStream_t st)
 {
   hipLaunchKernelGGL(kIntervalNext, dim3(1), dim3(1), 0, st, c, state, out, result);
@@ -211,7 +217,7 @@ hipError_t launchHitNext(const IteratorContextView &hostView, const IteratorCont
 // ---- batches ---------------------------------------------------------------------
 
 
-__global__ __launch_bounds__(kIterBlock) void kIntervalBatch(IteratorContextView c, uint32_t numRays,
+__global__ __launch_bounds__(kIterBlock) void kIntervalBatch(const IteratorContextView *__restrict__ cp, uint32_t numRays,
                                                               const float *__restrict__ org,
                                                               const float *__restrict__ dir,
                                                               const float *__restrict__ tRanges,
@@ -220,6 +226,7 @@ __global__ __launch_bounds__(kIterBlock) void kIntervalBatch(IteratorContextView
 {
   const uint32_t r = blockIdx.x * kIterBlock + threadIdx.x;
   if (r >= numRays) return;
+  const IteratorContextView &c = *cp;
   IntervalState st;
   intervalInit(c, st, F3{org[3 * r], org[3 * r + 1], org[3 * r + 2]}, F3{dir[3 * r], dir[3 * r + 1], dir[3 * r + 2]},
                rayRange(tRanges, r));
@@ -231,7 +238,7 @@ __global__ __launch_bounds__(kIterBlock) void kIntervalBatch(IteratorContextView
 }
 
 
-hipError_t launchIntervalBatch(const IteratorContextView &c, uint32_t numRays, const float *org, const float *dir,
+hipError_t launchIntervalBatch(const IteratorContextView *c, uint32_t numRays, const float *org, const float *dir,
                                const float *tRanges, IntervalOut *out, uint32_t maxOut, uint32_t *count,
                                hipStream_t st)
 {

@@ -283,7 +283,7 @@ OPENVKL_INTERFACE void vklNeuralFieldSampleN(VKLNeuralField h, unsigned int N, c
     checkHip(hipMemcpyAsync(&fb, count, 4, hipMemcpyDeviceToHost, st), "copy");
     checkHip(hipStreamSynchronize(st), "network");
     if (fb)
-      checkHip(launchGatherSample(nf->sampler->view, nf->sampler->attr(nf->attr), nf->attr, list, count, fb, xyz,
+      checkHip(launchGatherSample(nf->sampler->viewCopy(), nf->sampler->attr(nf->attr), nf->attr, list, count, fb, xyz,
                                   samples, st),
                "fallback");
   }

@@ -552,7 +552,10 @@ hipError_t launchNfInfer(const NfModel &m, const GridView &g, uint32_t attr, con
   return hipGetLastError();
 }
 
-template <typename T>
+// Exact samples of the fallback points. kObserve: the sampler has a
+// LeafNodeAccess observer, so the traversals record their leaves as the
+// exact sampler's own kernels do.
+template <typename T, bool kObserve>
 __global__ void kGatherSample(SamplerView sv, AttrView a, uint32_t attr, const uint32_t *__restrict__ idx,
                               const uint32_t *__restrict__ count, uint64_t maxCount, const float *__restrict__ xyz,
                               float *__restrict__ out)
@@ -562,7 +565,7 @@ __global__ void kGatherSample(SamplerView sv, AttrView a, uint32_t attr, const u
   if (k >= cnt) return;
   const uint32_t i = idx[k];
   const F3 oc      = {xyz[3 * (uint64_t)i], xyz[3 * (uint64_t)i + 1], xyz[3 * (uint64_t)i + 2]};
-  const SparseSource<T> src(sv.grid, a, attr);
+  const SparseSource<T, kObserve> src(sv.grid, a, attr);
   out[i] = sampleFiltered(src, sv.filter, xfmPoint(sv.grid.o2i, oc), sv.grid.constantCellData ? 0.f : 0.5f);
 }
 
@@ -572,10 +575,19 @@ hipError_t launchGatherSample(const SamplerView &s, const AttrView &a, uint32_t 
 {
   if (maxCount == 0) return hipSuccess;
   const dim3 grid((unsigned)((maxCount + 255) / 256)), block(256);
-  if (a.type == kTypeHalf)
-    hipLaunchKernelGGL(kGatherSample<__half>, grid, block, 0, stream, s, a, attr, idx, count, maxCount, xyz, out);
+  const bool observed = s.grid.leafAccess != nullptr;
+  if (a.type == kTypeHalf && observed)
+    hipLaunchKernelGGL((kGatherSample<__half, true>), grid, block, 0, stream, s, a, attr, idx, count, maxCount, xyz,
+                       out);
+  else if (a.type == kTypeHalf)
+    hipLaunchKernelGGL((kGatherSample<__half, false>), grid, block, 0, stream, s, a, attr, idx, count, maxCount, xyz,
+                       out);
+  else if (observed)
+    hipLaunchKernelGGL((kGatherSample<float, true>), grid, block, 0, stream, s, a, attr, idx, count, maxCount, xyz,
+                       out);
   else
-    hipLaunchKernelGGL(kGatherSample<float>, grid, block, 0, stream, s, a, attr, idx, count, maxCount, xyz, out);
+    hipLaunchKernelGGL((kGatherSample<float, false>), grid, block, 0, stream, s, a, attr, idx, count, maxCount, xyz,
+                       out);
   return hipGetLastError();
 }
 

@@ -188,12 +188,21 @@ struct Sampler : ManagedObject
   uint32_t accessEpoch = 0;
   uint32_t liveObservers = 0;
   uint64_t viewVersion   = 0;  // bumped whenever view changes
+  // Guards view / viewVersion against observers created or released while
+  // other threads iterate (contexts copy view under it).
+  std::mutex viewMutex;
 
   Sampler(Device *d, Volume *v);
   ~Sampler() override;
   void commit() override;
   std::string toString() const override { return "openvkl::Sampler"; }
   const AttrView &attr(uint32_t a) const;
+  // A copy of view taken under viewMutex (what one call launches with).
+  SamplerView viewCopy()
+  {
+    std::lock_guard<std::mutex> lock(viewMutex);
+    return view;
+  }
   // Applies the observation state to view (observed samplers traverse the
   // node table: corner-packed cell records bypass it) and uploads deviceView.
   void applyObservation();
@@ -212,8 +221,12 @@ struct IteratorContext : ManagedObject
   IteratorContext(Device *d, Sampler *s, bool hit);
   ~IteratorContext() override;
   void commit() override;
-  // Picks up sampler state changed since commit (re-commit, observers).
-  void refreshSampler();
+  // Picks up sampler state changed since commit (re-commit, observers) and
+  // returns a copy of the view, both under viewMutex: host threads iterating
+  // one context while an observer is created or released each launch with a
+  // consistent view.
+  IteratorContextView refreshSampler();
+  std::mutex viewMutex;
   std::string toString() const override { return isHit ? "openvkl::HitIteratorContext" : "openvkl::IntervalIteratorContext"; }
 };
 

@@ -801,6 +801,18 @@ constexpr uint32_t kMaxDdaSteps = 1u << 22;
 #define DY(l) (4 + (l))
 #define DZ(l) (8 + (l))
 
+// Every DDA routine below indexes the state arrays with compile-time
+// constants (levels are template arguments or fully unrolled loop indices):
+// an array indexed at run time is placed in scratch memory, which costs the
+// iterators several times their voxel traffic.
+
+// Per-axis step of the level-L cells: tDelta = res * |1 / dir| (Dda.ih:184-186).
+template <int L>
+VKL_DEV float ddaTDelta(const DdaState &d, int axis)
+{
+  return (float)kRes[L + 1] * d.absRcp[axis];
+}
+
 // Dda.ih:145-196.
 VKL_DEV void ddaInit(DdaState &d, F3 org, F3 dir, float tMin, uint32_t level)
 {
@@ -815,26 +827,27 @@ VKL_DEV void ddaInit(DdaState &d, F3 org, F3 dir, float tMin, uint32_t level)
   d.rayDir[0] = dir.x;
   d.rayDir[1] = dir.y;
   d.rayDir[2] = dir.z;
-  for (uint32_t l = 0; l <= d.level; ++l) {
+  d.absRcp[0] = fabsf(id.x);
+  d.absRcp[1] = fabsf(id.y);
+  d.absRcp[2] = fabsf(id.z);
+#pragma unroll
+  for (int l = 0; l < 4; ++l) {
+    if ((uint32_t)l > d.level) continue;
     const int32_t res = (int32_t)kRes[l + 1];
     const int32_t ix = clampCell(oo.x, res), iy = clampCell(oo.y, res), iz = clampCell(oo.z, res);
-    d.idx[DX(l)]      = ix;
-    d.idx[DY(l)]      = iy;
-    d.idx[DZ(l)]      = iz;
-    d.idxDelta[DX(l)] = sx * res;
-    d.idxDelta[DY(l)] = sy * res;
-    d.idxDelta[DZ(l)] = sz * res;
-    d.tDelta[DX(l)]   = (float)res * fabsf(id.x);
-    d.tDelta[DY(l)]   = (float)res * fabsf(id.y);
-    d.tDelta[DZ(l)]   = (float)res * fabsf(id.z);
-    d.tNext[DX(l)]    = planeT((float)(ix + res * (sx >= 0)), org.x, id.x);
-    d.tNext[DY(l)]    = planeT((float)(iy + res * (sy >= 0)), org.y, id.y);
-    d.tNext[DZ(l)]    = planeT((float)(iz + res * (sz >= 0)), org.z, id.z);
+    d.idx[DX(l)]   = ix;
+    d.idx[DY(l)]   = iy;
+    d.idx[DZ(l)]   = iz;
+    d.tNext[DX(l)] = planeT((float)(ix + res * (sx >= 0)), org.x, id.x);
+    d.tNext[DY(l)] = planeT((float)(iy + res * (sy >= 0)), org.y, id.y);
+    d.tNext[DZ(l)] = planeT((float)(iz + res * (sz >= 0)), org.z, id.z);
   }
 }
 
-// Dda.ih:204-266.
-VKL_DEV void ddaReinitBelow(DdaState &d, uint32_t level, int32_t dx, int32_t dy)
+// Dda.ih:204-266: levels L + 1 .. d.level restart from the current t, nudged
+// 1e-2 across the plane the level-L step crossed.
+template <int L>
+VKL_DEV void ddaReinitBelow(DdaState &d, int32_t dx, int32_t dy)
 {
   const F3 dir = {d.rayDir[0], d.rayDir[1], d.rayDir[2]};
   const F3 id  = {dirRcp(dir.x), dirRcp(dir.y), dirRcp(dir.z)};
@@ -846,7 +859,9 @@ VKL_DEV void ddaReinitBelow(DdaState &d, uint32_t level, int32_t dx, int32_t dy)
     oo.y = oo.y + ((float)sy) * 1e-2f;
   else
     oo.z = oo.z + ((float)sz) * 1e-2f;
-  for (uint32_t l = level + 1; l <= d.level; ++l) {
+#pragma unroll
+  for (int l = L + 1; l < 4; ++l) {
+    if ((uint32_t)l > d.level) continue;
     const int32_t res = (int32_t)kRes[l + 1];
     d.idx[DX(l)]      = clampCell(oo.x, res);
     d.idx[DY(l)]      = clampCell(oo.y, res);
@@ -858,44 +873,89 @@ VKL_DEV void ddaReinitBelow(DdaState &d, uint32_t level, int32_t dx, int32_t dy)
 }
 
 // Dda.ih:271-309 (ties: x, then y, then z).
-VKL_DEV void ddaStep(DdaState &d, uint32_t l)
+template <int L>
+VKL_DEV void ddaStep(DdaState &d)
 {
-  const float tx = d.tNext[DX(l)], ty = d.tNext[DY(l)], tz = d.tNext[DZ(l)];
-  const float minT = vmin(vmin(tx, ty), tz);
-  int a = DZ(l);
-  if (minT == tx)
-    a = DX(l);
-  else if (minT == ty)
-    a = DY(l);
-  d.t = d.tNext[a];
-  d.tNext[a] += d.tDelta[a];
-  d.idx[a] += d.idxDelta[a];
+  // Every field is written on every path (values selected, not addresses),
+  // so the state stays in registers.
+  const float tx = d.tNext[DX(L)], ty = d.tNext[DY(L)], tz = d.tNext[DZ(L)];
+  const float minT  = vmin(vmin(tx, ty), tz);
+  const int32_t res = (int32_t)kRes[L + 1];
+  const bool sx     = minT == tx;
+  const bool sy     = !sx && minT == ty;
+  const bool sz     = !sx && !sy;
+  d.t               = sx ? tx : (sy ? ty : tz);
+  d.tNext[DX(L)]    = sx ? tx + ddaTDelta<L>(d, 0) : tx;
+  d.tNext[DY(L)]    = sy ? ty + ddaTDelta<L>(d, 1) : ty;
+  d.tNext[DZ(L)]    = sz ? tz + ddaTDelta<L>(d, 2) : tz;
+  d.idx[DX(L)]      = d.idx[DX(L)] + (sx ? dirSign(d.rayDir[0]) * res : 0);
+  d.idx[DY(L)]      = d.idx[DY(L)] + (sy ? dirSign(d.rayDir[1]) * res : 0);
+  d.idx[DZ(L)]      = d.idx[DZ(L)] + (sz ? dirSign(d.rayDir[2]) * res : 0);
 }
 
-// HDDA.ih.in:24-101: step on the level of the empty voxel, re-initialise the
-// levels below, and let parents follow a child that left their cell.
-VKL_DEV void hddaStep(DdaState &d, uint32_t descLevel)
+// HDDA.ih.in:52-101, parents of a stepped level from the nearest one up:
+// each one steps only if its child left its cell.
+template <int P>
+VKL_DEV void hddaParents(DdaState &d)
 {
-  // Recursion unrolled: find the step level, then propagate upwards.
-  if (descLevel >= 3u) {
-    ddaStep(d, 3);
-  } else {
-    const uint32_t L  = descLevel;
-    const int32_t ox = d.idx[DX(L)], oy = d.idx[DY(L)];
-    ddaStep(d, L);
-    if (L < d.level) ddaReinitBelow(d, L, d.idx[DX(L)] - ox, d.idx[DY(L)] - oy);
-  }
-  // Parents of the stepped level, from the nearest one upwards (each one only
-  // steps if its child stepped and left its cell).
-  for (int P = (int)(descLevel >= 3u ? 3 : descLevel) - 1; P >= 0; --P) {
+  if constexpr (P >= 0) {
+    constexpr int C   = P + 1;
     const int32_t res = (int32_t)kRes[P + 1];
-    const int C       = P + 1;
     const bool left   = d.idx[DX(C)] < d.idx[DX(P)] || d.idx[DY(C)] < d.idx[DY(P)] || d.idx[DZ(C)] < d.idx[DZ(P)] ||
                       d.idx[DX(C)] >= d.idx[DX(P)] + res || d.idx[DY(C)] >= d.idx[DY(P)] + res ||
                       d.idx[DZ(C)] >= d.idx[DZ(P)] + res;
-    if (!left) break;
-    ddaStep(d, (uint32_t)P);
+    if (!left) return;
+    ddaStep<P>(d);
+    hddaParents<P - 1>(d);
   }
+}
+
+// HDDA.ih.in:24-101: step on level L (the level of the empty voxel),
+// re-initialise the levels below, and let parents follow a child that left
+// their cell.
+template <int L>
+VKL_DEV void hddaStepAt(DdaState &d)
+{
+  if constexpr (L >= 3) {
+    ddaStep<3>(d);
+  } else {
+    const int32_t ox = d.idx[DX(L)], oy = d.idx[DY(L)];
+    ddaStep<L>(d);
+    if ((uint32_t)L < d.level) ddaReinitBelow<L>(d, d.idx[DX(L)] - ox, d.idx[DY(L)] - oy);
+  }
+  hddaParents<(L >= 3 ? 3 : L) - 1>(d);
+}
+
+VKL_DEV void hddaStep(DdaState &d, uint32_t descLevel)
+{
+  switch (descLevel) {
+  case 0: hddaStepAt<0>(d); break;
+  case 1: hddaStepAt<1>(d); break;
+  case 2: hddaStepAt<2>(d); break;
+  default: hddaStepAt<3>(d); break;
+  }
+}
+
+// Pins a value to a register: the optimizer can no longer turn a select
+// between array elements into a load from a selected address (a run-time
+// array index, which sends the whole array to scratch memory).
+VKL_DEV int32_t opaque(int32_t v)
+{
+  asm volatile("" : "+v"(v));
+  return v;
+}
+
+// The cell of the DDA's current level (d.level), selected without a
+// run-time array index.
+VKL_DEV I3 ddaCell(const DdaState &d)
+{
+  I3 c = {opaque(d.idx[DX(0)]), opaque(d.idx[DY(0)]), opaque(d.idx[DZ(0)])};
+#pragma unroll
+  for (int l = 1; l < 4; ++l) {
+    const I3 cl = {opaque(d.idx[DX(l)]), opaque(d.idx[DY(l)]), opaque(d.idx[DZ(l)])};
+    if (d.level == (uint32_t)l) c = cl;
+  }
+  return c;
 }
 
 // ValueRanges.ih:22-52.
@@ -1014,13 +1074,12 @@ VKL_DEV void intervalInit(const IteratorContextView &c, IntervalState &it, F3 or
   it.ck[0] = it.ck[1] = it.ck[2] = INT32_MIN;
   it.cnode = 0;
   const Range1f ti = intersectBox(org, dir, g.domainBox, tRange);
-  if (ti.upper <= ti.lower) {
-    it.tMax  = -vinf();
-    it.dda.t = 0.f;
-  } else {
-    it.tMax = ti.upper;
-    ddaInit(it.dda, org, dir, ti.lower, c.maxIteratorDepth);
-  }
+  // A ray that misses the box gets tMax = -inf (and t = 0), which ends its
+  // iteration; the DDA is initialised either way so that every field of the
+  // state is written on both paths.
+  const bool miss = ti.upper <= ti.lower;
+  it.tMax         = miss ? -vinf() : ti.upper;
+  ddaInit(it.dda, org, dir, miss ? 0.f : ti.lower, c.maxIteratorDepth);
 }
 
 // VdbIterator_iterateIntervalInternal, VdbIterator.ispc:96-159.
@@ -1043,10 +1102,11 @@ VKL_DEV bool intervalNext(const IteratorContextView &c, IntervalState &it, Inter
       return false;
     }
     const uint32_t l = it.dda.level;
+    const I3 cell    = ddaCell(it.dda);
     if (dense)
-      queryVoxelDense(c, it.dda.idx[DX(l)], it.dda.idx[DY(l)], it.dda.idx[DZ(l)], l, v);
+      queryVoxelDense(c, cell.x, cell.y, cell.z, l, v);
     else
-      queryVoxel<kNodeCache>(c, it.dda.idx[DX(l)], it.dda.idx[DY(l)], it.dda.idx[DZ(l)], l, v, it);
+      queryVoxel<kNodeCache>(c, cell.x, cell.y, cell.z, l, v, it);
     if (!v.isEmpty) break;
     hddaStep(it.dda, v.level);
   }

@@ -128,17 +128,20 @@ struct SamplerView
 
 // ---- iterator state (lives in caller-owned buffers; also used in batches) ----
 
-// Dda.ih:17-36 layout (4 levels x 3 axes).
+// Dda.ih:17-36 (4 levels x 3 axes, index axis * 4 + level). The reference's
+// per-level tDelta = res * |1 / dir| and idxDelta = sign(dir) * res are not
+// stored: the steps recompute them from absRcp (= |1 / dir| as ddaInit
+// computes it) and the sign of rayDir, which gives the same values, and the
+// state stays small enough to live in registers.
 struct DdaState
 {
   uint32_t level;
   float t;
   float rayOrg[3];
   float rayDir[3];
+  float absRcp[3];
   int32_t idx[12];
   float tNext[12];
-  float tDelta[12];
-  int32_t idxDelta[12];
 };
 
 struct IntervalState

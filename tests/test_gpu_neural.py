@@ -119,3 +119,26 @@ def test_ablation_network_vs_exact_and_no_hash_grid(trained, device):
     # measured on MI355X: 2.9e-5 vs 1.1e-4 (3.8x)
     assert full_mse < 0.5 * coarse_mse
     coarse.release()
+
+
+def test_fallback_records_leaf_access(trained):
+    """The exact samples of fallback points record their leaves in a
+    LeafNodeAccess observer of the exact sampler, the same leaves the exact
+    sampler marks when it samples those points itself."""
+    vol, s, origins, nf, _ = trained
+    span = float(np.diff(vol.value_range())[0])
+    nf.finalize(2e-2 * span, points_per_leaf=128, safety=1.0)
+    pts = torch.from_numpy(leaf_points(origins, 1 << 16, seed=31)).cuda()
+    ws = torch.zeros(pts.shape[0] + 1, dtype=torch.int32, device="cuda")
+    obs = s.observer("LeafNodeAccess")
+    _, nfb = nf.sample(pts, fallback=True, workspace=ws)
+    got = obs.map()
+    obs.release()
+    assert 0 < nfb < pts.shape[0]
+    fb_idx = ws[1:1 + nfb].long()
+    obs = s.observer("LeafNodeAccess")
+    s.sample(pts[fb_idx].contiguous())
+    want = obs.map()
+    obs.release()
+    assert want.sum() > 0
+    np.testing.assert_array_equal(got != 0, want != 0)

@@ -55,3 +55,25 @@ def test_no_non_inlined_device_functions():
         text = re.sub(r"//.*", "", open(path).read())
         assert "__noinline__" not in text, path
         assert "noinline" not in text, path
+
+
+def test_transfer_function_rows_are_checked():
+    """Transfer functions reach the kernels as float32 (r, g, b, opacity)
+    rows with their row count: a flat array or a float64 tensor is converted
+    (the kernels read 16 bytes per row), and a size that is not a multiple of
+    4 raises before any call."""
+    import numpy as np
+    import pytest
+    import torch
+
+    from openvkl_amd import _tf_rows
+
+    flat = torch.arange(32, dtype=torch.float64)
+    rows = _tf_rows(flat)
+    assert rows.dtype == torch.float32 and tuple(rows.shape) == (8, 4) and rows.is_contiguous()
+    assert torch.equal(rows.reshape(-1), flat.float())
+    assert _tf_rows(np.arange(12, dtype=np.float64)).shape == (3, 4)
+    assert _tf_rows(np.zeros((5, 4), np.float32)).shape == (5, 4)
+    for bad in (torch.zeros(6), np.zeros(7, np.float32), torch.zeros(0)):
+        with pytest.raises(ValueError):
+            _tf_rows(bad)

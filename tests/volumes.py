@@ -211,3 +211,71 @@ def temporal_vdb_nodes(fn, seed=SEED + 6, dtype=np.float32, n_leaves=120):
             tu_times.append(times)
     return dict(level=level, origin=origins, fmt=fmt, leaf_data=leaf_data, temporal_format=tformat, steps=steps,
                 tu_indices=tu_indices, tu_times=tu_times)
+
+
+def leaves_from_dense(arr):
+    """Split an [nz, ny, nx] array (every edge a multiple of 8) into 8^3 vdb
+    leaves. Returns (origins [n,3] int32, leaves [n,512]) with leaf-local
+    order x*64 + y*8 + z and leaves in x-slowest order."""
+    nz, ny, nx = arr.shape
+    a = arr.reshape(nz // 8, 8, ny // 8, 8, nx // 8, 8)        # kz lz ky ly kx lx
+    leaves = a.transpose(4, 2, 0, 5, 3, 1).reshape(-1, 512)     # kx ky kz | lx ly lz
+    k = np.stack(np.meshgrid(np.arange(nx // 8), np.arange(ny // 8), np.arange(nz // 8), indexing="ij"), -1)
+    origins = (k.reshape(-1, 3) * 8).astype(np.int32)
+    return origins, np.ascontiguousarray(leaves)
+
+
+def _gradient_noise(px, py, pz, perm):
+    """Improved gradient noise (Perlin, SIGGRAPH 2002: quintic fade, 12 edge
+    gradients picked by a hashed permutation) at float64 points; perm is a
+    permutation of 0..255 repeated twice."""
+    xi, yi, zi = (np.floor(px).astype(np.int64) & 255, np.floor(py).astype(np.int64) & 255,
+                  np.floor(pz).astype(np.int64) & 255)
+    x, y, z = px - np.floor(px), py - np.floor(py), pz - np.floor(pz)
+
+    def fade(t):
+        return t * t * t * (t * (t * 6 - 15) + 10)
+
+    def grad(h, gx, gy, gz):
+        h = h & 15
+        u = np.where(h < 8, gx, gy)
+        v = np.where(h < 4, gy, np.where((h == 12) | (h == 14), gx, gz))
+        return np.where(h & 1, -u, u) + np.where(h & 2, -v, v)
+
+    u, v, w = fade(x), fade(y), fade(z)
+    A, B = perm[xi] + yi, perm[xi + 1] + yi
+    AA, AB, BA, BB = perm[A] + zi, perm[A + 1] + zi, perm[B] + zi, perm[B + 1] + zi
+
+    def lerp(t, a, b):
+        return a + t * (b - a)
+
+    return lerp(w, lerp(v, lerp(u, grad(perm[AA], x, y, z), grad(perm[BA], x - 1, y, z)),
+                        lerp(u, grad(perm[AB], x, y - 1, z), grad(perm[BB], x - 1, y - 1, z))),
+                lerp(v, lerp(u, grad(perm[AA + 1], x, y, z - 1), grad(perm[BA + 1], x - 1, y, z - 1)),
+                     lerp(u, grad(perm[AB + 1], x, y - 1, z - 1), grad(perm[BB + 1], x - 1, y - 1, z - 1))))
+
+
+def torus_vdb_leaves(res=128, octaves=4, seed=SEED + 12):
+    """A turbulent torus (major radius 0.75, minor 0.375 of the half-extent,
+    space distorted by `octaves` of gradient-noise turbulence) with
+    0.5 + 0.5 noise inside, as the non-empty 8^3 leaves of a res^3 domain --
+    the shape of the reference's torus test volume
+    (testing/volume/TestingVdbTorusVolume.h:104-200), from this module's own
+    noise with a seeded permutation. Returns (origins, leaves [n,512])."""
+    perm = np.random.default_rng(seed).permutation(256)
+    perm = np.concatenate([perm, perm])
+    c = (np.arange(res) + 0.5) / res
+    z, y, x = np.meshgrid(c, c, c, indexing="ij")
+    turb = np.zeros_like(x)
+    scale = 1.0
+    for _ in range(octaves):
+        turb += _gradient_noise(scale * x * 12, scale * y * 12, scale * z * 12, perm) / scale
+        scale *= 2.0
+    k = 1.4 + 0.4 * turb
+    X, Y, Z = k * (2 * x - 1), k * (2 * y - 1), k * (2 * z - 1)
+    t = np.sqrt(X * X + Z * Z) - 0.75
+    inside = t * t + Y * Y - 0.375 * 0.375 < 0
+    vals = np.where(inside, 0.5 + 0.5 * _gradient_noise(x * 12, y * 12, z * 12, perm), 0.0).astype(np.float32)
+    origins, leaves = leaves_from_dense(vals)
+    keep = np.any(leaves != 0, axis=1)
+    return origins[keep], np.ascontiguousarray(leaves[keep])
