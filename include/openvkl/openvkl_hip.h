@@ -23,6 +23,22 @@ OPENVKL_INTERFACE void vklComputeGradientNAsync(const VKLSampler *sampler, unsig
                                                 const vkl_vec3f *objectCoordinates, vkl_vec3f *gradients,
                                                 unsigned int attributeIndex, void *stream);
 
+/* vklComputeSampleN that allocates nothing. It has no counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * hipGraph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, results and synchronous behaviour as vklComputeSampleN, but
+ * objectCoordinates, samples and times must be device memory, and the
+ * scratch of the binned sampling kernels comes from a fixed 64 MiB array per
+ * GPU (held until the process ends) instead of an allocation. A call whose
+ * binned scratch would exceed 64 MiB fails with VKL_INVALID_ARGUMENT. All
+ * calls on one GPU share that array, which keeps the last call's contents:
+ * calls that overlapped on one GPU would race on it, so the library
+ * serialises them (one lock per GPU; each call returns after its kernels
+ * have finished). */
+OPENVKL_INTERFACE void vklComputeSampleNNoAlloc(const VKLSampler *sampler, unsigned int N,
+                                                const vkl_vec3f *objectCoordinates, float *samples,
+                                                unsigned int attributeIndex, const float *times);
+
 /* Fixed-step ray marching fused with sampling (a consumer of the sampler, as
  * the reference's ray-march renderer, examples/interactive/renderer/
  * ray_march_iterator/RayMarchIterator.ispc:39-137, but without host round

@@ -140,6 +140,7 @@ def lib():
         "vklComputeGradient": (vec3f, [C.POINTER(O), C.POINTER(vec3f), C.c_uint, f32]),
         "vklComputeGradientN": (None, [C.POINTER(O), C.c_uint, vp, vp, C.c_uint, vp]),
         "vklComputeSampleNAsync": (None, [C.POINTER(O), C.c_uint, vp, vp, C.c_uint, vp]),
+        "vklComputeSampleNNoAlloc": (None, [C.POINTER(O), C.c_uint, vp, vp, C.c_uint, vp]),
         "vklComputeGradientNAsync": (None, [C.POINTER(O), C.c_uint, vp, vp, C.c_uint, vp]),
         "vklGetIntervalIteratorSize": (sz, [C.POINTER(O)]),
         "vklInitIntervalIterator": (vp, [C.POINTER(O), C.POINTER(vec3f), C.POINTER(vec3f), C.POINTER(range1f), f32, vp]),
@@ -564,6 +565,15 @@ class Sampler(Handle):
         n = _n_points(xyz)
         out = self._out(xyz, (n,)) if out is None else out
         lib().vklComputeSampleN(C.byref(self.h), n, _ptr(xyz), _ptr(out), attr, _ptr(times))
+        self.device.check()
+        return out
+
+    def sample_noalloc(self, xyz, attr=0, out=None, times=None):
+        """vklComputeSampleNNoAlloc: as sample(), device tensors only; the
+        binned kernels' scratch is the library's fixed per-GPU array."""
+        n = _n_points(xyz)
+        out = self._out(xyz, (n,)) if out is None else out
+        lib().vklComputeSampleNNoAlloc(C.byref(self.h), n, _ptr(xyz), _ptr(out), attr, _ptr(times))
         self.device.check()
         return out
 

@@ -17,6 +17,7 @@
 
 #include "binned.h"
 #include "kernels.h"
+#include "noalloc.h"
 #include "objects.h"
 #include "openvkl/openvkl_hip.h"
 
@@ -1295,6 +1296,26 @@ OPENVKL_INTERFACE void vklComputeSampleNAsync(const VKLSampler *s, unsigned int 
   checkHip(launchSample(sp->viewCopy(), sp->attr(a), a, N, reinterpret_cast<const float *>(oc), out,
                         static_cast<hipStream_t>(stream)),
            "sample kernel");
+  VKL_CATCH()
+}
+
+OPENVKL_INTERFACE void vklComputeSampleNNoAlloc(const VKLSampler *s, unsigned int N, const vkl_vec3f *oc,
+                                                float *out, unsigned int a, const float *times)
+{
+  VKL_TRY(samplerOf(s)->device)
+  Sampler *sp = samplerOf(s);
+  dev_->makeCurrent();
+  (void)sp->attr(a);
+  if (N == 0) return;
+  const bool temporal = sp->viewCopy().grid.temporalFormat != 0 && times;
+  if (!isDevicePointer(oc) || !isDevicePointer(out) || (temporal && !isDevicePointer(times)))
+    throw VklError("vklComputeSampleNNoAlloc takes device pointers only (host arrays would need staging buffers)",
+                   VKL_INVALID_ARGUMENT);
+  const NoAllocScratch fixed = noAllocScratch(dev_->hipDevice);
+  std::lock_guard<std::mutex> lock(*fixed.lock);
+  sampleInto(sp, a, N, reinterpret_cast<const float *>(oc), out, 1, temporal ? times : nullptr, fixed.ptr,
+             fixed.bytes);
+  syncDevice(dev_);
   VKL_CATCH()
 }
 
