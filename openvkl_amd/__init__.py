@@ -351,7 +351,7 @@ class Device:
 
     def vdb(self, level, origin, fmt, packed_dense=None, packed_tile=None, leaf_data=None, background=None,
             index_to_object=None, filter=None, corner_packed=None, max_sampling_depth=None, temporal_format=None,
-            steps=None, tu_indices=None, tu_times=None):
+            steps=None, tu_indices=None, tu_times=None, leaf_index_grid=None):
         """Sparse vdb volume from node arrays (packed or per-leaf data).
         Motion blur (per-leaf data only): temporal_format per node, steps per
         node (node.temporallyStructuredNumTimesteps), tu_indices / tu_times
@@ -396,6 +396,8 @@ class Device:
             v.set_data("indexToObject", d(np.asarray(index_to_object, np.float32)))
         if corner_packed is not None:
             v.set_bool("cornerPacked", corner_packed)
+        if leaf_index_grid is not None:
+            v.set_int("leafIndexGrid", int(leaf_index_grid))
         if max_sampling_depth is not None:
             v.set_int("maxSamplingDepth", max_sampling_depth)
         if temporal_format is not None:

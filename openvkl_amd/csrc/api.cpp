@@ -260,6 +260,19 @@ struct Staged
 
 void syncDevice(Device *d) { checkHip(hipStreamSynchronize(d->stream), "hipStreamSynchronize"); }
 
+// Compute units of the device (cached per HIP ordinal).
+int computeUnits(Device *d)
+{
+  static int cached[64] = {0};
+  int &c = cached[d->hipDevice & 63];
+  if (!c) {
+    int v = 256;
+    (void)hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, d->hipDevice);
+    c = v > 0 ? v : 256;
+  }
+  return c;
+}
+
 Sampler *samplerOf(const VKLSampler *s)
 {
   if (!s || !s->host) throw VklError("null sampler");
@@ -1442,8 +1455,13 @@ OPENVKL_INTERFACE void vklIterateHitsBatch(const VKLHitIteratorContext *c, unsig
   Staged out(hits, (uint64_t)numRays * maxHitsPerRay * sizeof(VKLHit), st, false);
   Staged cnt(numHits, numRays * 4ull, st, false);
   const IteratorContextView cv = ctx->refreshSampler();
-  checkHip(launchHitBatch(cv, ctx->deviceView.as<IteratorContextView>(), ctx->sampler->attr(cv.attributeIndex), numRays, o.as<float>(), d.as<float>(), t.as<float>(), tm.as<float>(),
-                          out.as<HitOut>(), maxHitsPerRay, cnt.as<uint32_t>(), st),
+  std::lock_guard<std::mutex> lock(ctx->batchMutex);
+  const uint32_t lanes = batchLaneCount(numRays, computeUnits(dev_));
+  if (!ctx->batchWork.ptr) ctx->batchWork.allocate(sizeof(uint32_t));
+  checkHip(launchSetWork(ctx->batchWork.as<uint32_t>(), lanes, st), "batch work counter");
+  checkHip(launchHitBatch(cv, ctx->deviceView.as<IteratorContextView>(), ctx->sampler->attr(cv.attributeIndex), numRays,
+                          o.as<float>(), d.as<float>(), t.as<float>(), tm.as<float>(), out.as<HitOut>(), maxHitsPerRay,
+                          cnt.as<uint32_t>(), ctx->batchWork.as<uint32_t>(), lanes, st),
            "hit batch");
   out.copyOut();
   cnt.copyOut();

@@ -91,11 +91,14 @@ hipError_t launchSwizzle(const void *src, void *dst, uint32_t elemSize, const ui
 hipError_t launchPackCorners(const void *src, void *dst, uint32_t elemSize, const uint32_t dims[3],
                              hipStream_t stream);
 
-// Corner-packed vdb (commit): the leaf index grid over g.leafDims (filled
-// with 0xffffffff, then the slot of each dense leaf; origins absolute, 3
-// int32 per leaf, packed-slot order) and the cell records (8 floats per cell,
-// 512 cells per dense leaf) of one attribute.
-hipError_t launchBuildLeafIndex(const GridView &g, const int32_t *origins, uint64_t numLeaves, uint32_t *index,
+// vdb commit: the leaf index grid over g.leafDims (filled with kLeafEmpty,
+// then the slot of each dense leaf -- origins absolute, 3 int32 per leaf,
+// packed-slot order -- then kLeafTraverse over every cell a tile node of
+// input level 1, 2 or 3 covers: tileOrigins[level - 1], numTiles[level - 1]);
+// corner-packed volumes: the cell records (8 floats per cell, 512 cells per
+// dense leaf) of one attribute.
+hipError_t launchBuildLeafIndex(const GridView &g, const int32_t *origins, uint64_t numLeaves,
+                                const int32_t *const tileOrigins[3], const uint64_t numTiles[3], uint32_t *index,
                                 hipStream_t stream);
 hipError_t launchPackLeafCells(const GridView &g, const AttrView &a, uint32_t attr, const int32_t *origins,
                                uint64_t numLeaves, float *cells, hipStream_t stream);
@@ -125,9 +128,14 @@ hipError_t launchHitNext(const IteratorContextView &hostView, const IteratorCont
 hipError_t launchIntervalBatch(const IteratorContextView *c, uint32_t numRays, const float *org, const float *dir,
                                const float *tRanges, IntervalOut *out, uint32_t maxOut, uint32_t *count,
                                hipStream_t stream);
+// Hit batches run a persistent schedule: `lanes` (batchLaneCount) lanes take
+// rays from the work counter `work`, which launchSetWork must set to `lanes`
+// on the same stream before each batch launch.
+uint32_t batchLaneCount(uint32_t numRays, int cus);
+hipError_t launchSetWork(uint32_t *work, uint32_t value, hipStream_t stream);
 hipError_t launchHitBatch(const IteratorContextView &hostView, const IteratorContextView *c, const AttrView &a,
                           uint32_t numRays, const float *org, const float *dir, const float *tRanges,
-                          const float *times, HitOut *out, uint32_t maxOut, uint32_t *count, hipStream_t stream);
+                          const float *times, HitOut *out, uint32_t maxOut, uint32_t *count, uint32_t *work, uint32_t lanes, hipStream_t stream);
 // The same for structuredRegular volumes of half / float / double voxels
 // (kernels_hit_dense.hip), for temporal structuredRegular and vdb volumes
 // (kernels_hit_temporal.hip), for vdb volumes (kernels_hit_other.hip) and for
@@ -138,23 +146,23 @@ hipError_t launchHitNextObserved(const IteratorContextView &hostView, const Iter
 hipError_t launchHitBatchObserved(const IteratorContextView &hostView, const IteratorContextView *c,
                                   const AttrView &a, uint32_t numRays, const float *org, const float *dir,
                                   const float *tRanges, const float *times, HitOut *out, uint32_t maxOut,
-                                  uint32_t *count, hipStream_t stream);
+                                  uint32_t *count, uint32_t *work, uint32_t lanes, hipStream_t stream);
 hipError_t launchHitNextDenseFloat(const IteratorContextView &hostView, const IteratorContextView *c,
                                    const AttrView &a, HitState *state, HitOut *out, int *result, hipStream_t stream);
 hipError_t launchHitBatchDenseFloat(const IteratorContextView &hostView, const IteratorContextView *c,
                                     const AttrView &a, uint32_t numRays, const float *org, const float *dir,
                                     const float *tRanges, const float *times, HitOut *out, uint32_t maxOut,
-                                    uint32_t *count, hipStream_t stream);
+                                    uint32_t *count, uint32_t *work, uint32_t lanes, hipStream_t stream);
 hipError_t launchHitNextTemporal(const IteratorContextView &hostView, const IteratorContextView *c,
                                  const AttrView &a, HitState *state, HitOut *out, int *result, hipStream_t stream);
 hipError_t launchHitBatchTemporal(const IteratorContextView &hostView, const IteratorContextView *c,
                                   const AttrView &a, uint32_t numRays, const float *org, const float *dir,
                                   const float *tRanges, const float *times, HitOut *out, uint32_t maxOut,
-                                  uint32_t *count, hipStream_t stream);
+                                  uint32_t *count, uint32_t *work, uint32_t lanes, hipStream_t stream);
 hipError_t launchHitNextOther(const IteratorContextView &hostView, const IteratorContextView *c, const AttrView &a,
                               HitState *state, HitOut *out, int *result, hipStream_t stream);
 hipError_t launchHitBatchOther(const IteratorContextView &hostView, const IteratorContextView *c, const AttrView &a,
                                uint32_t numRays, const float *org, const float *dir, const float *tRanges,
-                               const float *times, HitOut *out, uint32_t maxOut, uint32_t *count, hipStream_t stream);
+                               const float *times, HitOut *out, uint32_t maxOut, uint32_t *count, uint32_t *work, uint32_t lanes, hipStream_t stream);
 
 }  // namespace vklhip

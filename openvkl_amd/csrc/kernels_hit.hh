@@ -19,6 +19,61 @@ VKL_DEV Range1f rayRange(const float *tRanges, uint32_t r)
   return {tRanges[2 * r], tRanges[2 * r + 1]};
 }
 
+// Persistent ray schedule of the batched iterators. Each lane of a
+// wavefront iterates one ray at a time; when at least kRefill lanes of the
+// wavefront have no ray, the wavefront takes that many new rays from the
+// work counter with one atomic (ballot, then a rank per idle lane), so lanes
+// whose rays end early do not wait for the longest ray of their wavefront.
+// The counter starts at the number of launched lanes (lane i starts on ray
+// i) and only grows; a wavefront ends once the counter has passed numRays
+// and none of its lanes holds a ray. Every ray's results are those of the
+// one-ray-per-lane schedule.
+constexpr uint32_t kRefill = 16;
+
+struct RaySchedule
+{
+  uint32_t r;
+  bool has, fresh, drained;
+
+  VKL_DEV explicit RaySchedule(uint32_t numRays)
+      : r(blockIdx.x * kIterBlock + threadIdx.x), has(r < numRays), fresh(r < numRays), drained(false)
+  {
+  }
+
+  // Called by every lane of the wavefront after each step; false once the
+  // wavefront is done.
+  VKL_DEV bool refill(uint32_t numRays, uint32_t *work)
+  {
+    const uint64_t idle   = __ballot(!has);
+    const uint32_t nIdle  = (uint32_t)__popcll(idle);
+    if (!drained && nIdle >= kRefill) {
+      const int leader = __ffsll((unsigned long long)idle) - 1;
+      uint32_t base    = 0;
+      if ((int)__lane_id() == leader) base = atomicAdd(work, nIdle);
+      base    = (uint32_t)__shfl((int)base, leader);
+      drained = (uint64_t)base + nIdle >= numRays;
+      if (!has) {
+        const uint32_t k = base + (uint32_t)__popcll(idle & ((1ull << __lane_id()) - 1ull));
+        if (k < numRays) {
+          r     = k;
+          has   = true;
+          fresh = true;
+        }
+      }
+    }
+    return !(drained && __ballot(has) == 0);
+  }
+};
+
+// Lanes a persistent batch launch starts with (a multiple of the
+// wavefront), and the value its work counter starts at.
+inline uint32_t batchLanes(uint32_t numRays, int cus)
+{
+  const uint64_t want = ((uint64_t)numRays + kIterBlock - 1) / kIterBlock;
+  const uint64_t cap  = (uint64_t)(cus > 0 ? cus : 256) * 16;
+  return (uint32_t)(want < cap ? want : cap) * kIterBlock;
+}
+
 // The hit kernels read their context from device memory and sample through
 // a PointSampler specialised on the volume's source type and filter (chosen
 // on the host from the same context); temporal sources sample at the ray time.
@@ -43,21 +98,35 @@ __global__ __launch_bounds__(kIterBlock) void kHitBatch(const IteratorContextVie
                                                          const float *__restrict__ dir,
                                                          const float *__restrict__ tRanges,
                                                          const float *__restrict__ times, HitOut *__restrict__ out,
-                                                         uint32_t maxOut, uint32_t *__restrict__ count)
+                                                         uint32_t maxOut, uint32_t *__restrict__ count,
+                                                         uint32_t *__restrict__ work)
 {
-  const uint32_t r = blockIdx.x * kIterBlock + threadIdx.x;
-  if (r >= numRays) return;
   const IteratorContextView &c = *cp;
   PointSampler<Src, kFilter> smp(c.sampler.grid, a, c.attributeIndex);
+  RaySchedule q(numRays);
   HitState st;
-  hitInit(c, st, F3{org[3 * r], org[3 * r + 1], org[3 * r + 2]}, F3{dir[3 * r], dir[3 * r + 1], dir[3 * r + 2]},
-          rayRange(tRanges, r), times ? times[r] : 0.f);
-  smp.setTime(st.time);
   uint32_t n = 0;
-  HitOut o;
-  HitOut *dst = out + (uint64_t)r * maxOut;
-  while (n < maxOut && hitNext(c, smp, st, o)) dst[n++] = o;
-  count[r] = n;
+  do {
+    if (q.fresh) {
+      const uint32_t r = q.r;
+      hitInit<hitDepthOf<Src>()>(c, st, F3{org[3 * r], org[3 * r + 1], org[3 * r + 2]},
+                                 F3{dir[3 * r], dir[3 * r + 1], dir[3 * r + 2]}, rayRange(tRanges, r),
+                                 times ? times[r] : 0.f);
+      smp.setTime(st.time);
+      n       = 0;
+      q.fresh = false;
+    }
+    if (q.has) {
+      HitOut o;
+      if (n < maxOut && hitNext(c, smp, st, o)) {
+        out[(uint64_t)q.r * maxOut + n] = o;
+        n++;
+      } else {
+        count[q.r] = n;
+        q.has      = false;
+      }
+    }
+  } while (q.refill(numRays, work));
 }
 
 struct HitNextLaunch
@@ -76,6 +145,7 @@ struct HitNextLaunch
   }
 };
 
+// work: the batch's work counter, already set to `lanes` (batchLanes).
 struct HitBatchLaunch
 {
   const IteratorContextView *c;
@@ -85,12 +155,14 @@ struct HitBatchLaunch
   HitOut *out;
   uint32_t maxOut;
   uint32_t *count;
+  uint32_t *work;
+  uint32_t lanes;
   hipStream_t st;
   template <typename Src, int kFilter>
   hipError_t operator()() const
   {
-    hipLaunchKernelGGL((kHitBatch<Src, kFilter>), dim3((numRays + kIterBlock - 1) / kIterBlock), dim3(kIterBlock), 0,
-                       st, c, a, numRays, org, dir, tRanges, times, out, maxOut, count);
+    hipLaunchKernelGGL((kHitBatch<Src, kFilter>), dim3(lanes / kIterBlock), dim3(kIterBlock), 0, st, c, a, numRays,
+                       org, dir, tRanges, times, out, maxOut, count, work);
     return hipGetLastError();
   }
 };

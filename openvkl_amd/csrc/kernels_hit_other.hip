@@ -17,13 +17,13 @@ hipError_t launchHitNextOther(const IteratorContextView &hostView, const Iterato
 
 hipError_t launchHitBatchOther(const IteratorContextView &hostView, const IteratorContextView *c, const AttrView &a,
                                uint32_t numRays, const float *org, const float *dir, const float *tRanges,
-                               const float *times, HitOut *out, uint32_t maxOut, uint32_t *count, hipStream_t st)
+                               const float *times, HitOut *out, uint32_t maxOut, uint32_t *count, uint32_t *work, uint32_t lanes, hipStream_t st)
 {
   if (numRays == 0) return hipSuccess;
   if (hostView.sampler.grid.temporalFormat)
-    return launchHitBatchTemporal(hostView, c, a, numRays, org, dir, tRanges, times, out, maxOut, count, st);
+    return launchHitBatchTemporal(hostView, c, a, numRays, org, dir, tRanges, times, out, maxOut, count, work, lanes, st);
   return dispatchSource<kSparseKinds>(hostView.sampler.grid, a, hostView.sampler.filter,
-                                      HitBatchLaunch{c, a, numRays, org, dir, tRanges, times, out, maxOut, count,
+                                      HitBatchLaunch{c, a, numRays, org, dir, tRanges, times, out, maxOut, count, work, lanes,
                                                      st});
 }
 

@@ -17,11 +17,11 @@ hipError_t launchHitNextTemporal(const IteratorContextView &hostView, const Iter
 hipError_t launchHitBatchTemporal(const IteratorContextView &hostView, const IteratorContextView *c,
                                   const AttrView &a, uint32_t numRays, const float *org, const float *dir,
                                   const float *tRanges, const float *times, HitOut *out, uint32_t maxOut,
-                                  uint32_t *count, hipStream_t st)
+                                  uint32_t *count, uint32_t *work, uint32_t lanes, hipStream_t st)
 {
   if (numRays == 0) return hipSuccess;
   return dispatchSource<kTemporalKinds>(hostView.sampler.grid, a, hostView.sampler.filter,
-                                        HitBatchLaunch{c, a, numRays, org, dir, tRanges, times, out, maxOut, count,
+                                        HitBatchLaunch{c, a, numRays, org, dir, tRanges, times, out, maxOut, count, work, lanes,
                                                        st});
 }
 

@@ -54,7 +54,7 @@ __global__ __launch_bounds__(kIterBlock) void kHitRender(const IteratorContextVi
   const uint32_t maxShade = m.maxHits < (uint32_t)kMaxShadedHits ? m.maxHits : (uint32_t)kMaxShadedHits;
 
   HitState st;
-  hitInit(c, st, o, d, rayRange(m.tRanges, r), m.time);
+  hitInit<hitDepthOf<Src>()>(c, st, o, d, rayRange(m.tRanges, r), m.time);
   bool primary = true;
   uint32_t hits = 0, k = 0;
   float alphaP = 0.f;  // opacity after the hits collected so far
@@ -128,7 +128,7 @@ __global__ __launch_bounds__(kIterBlock) void kHitRender(const IteratorContextVi
       light = -1;
     }
     if (!query) break;
-    hitInit(c, st, p, light == 0 ? light0 : light1, Range1f{hitE[k][lane], vinf()}, m.time);
+    hitInit<hitDepthOf<Src>()>(c, st, p, light == 0 ? light0 : light1, Range1f{hitE[k][lane], vinf()}, m.time);
   }
   m.rgba[4 * r]     = cr;
   m.rgba[4 * r + 1] = cg;

@@ -43,22 +43,51 @@ __global__ __launch_bounds__(kPackBlock) void kPackLeafCells(GridView g, AttrVie
                 z = origins[3 * l + 2] + (int32_t)(cell & 7u);
   AttrView base = a;
   base.cells    = nullptr;  // evaluate through the node table
-  const SparseSource<T> src(g, base, attr);
+  GridView gt   = g;
+  gt.leafIndex  = nullptr;  // by traversal, not through the flattened index
+  const SparseSource<T> src(gt, base, attr);
   CellRecord<float> r;
 #pragma unroll
   for (int k = 0; k < 8; k++) r.v[k] = src.at(x + ((k >> 2) & 1), y + ((k >> 1) & 1), z + (k & 1));
   reinterpret_cast<CellRecord<float> *>(cells)[t] = r;
 }
 
-hipError_t launchBuildLeafIndex(const GridView &g, const int32_t *origins, uint64_t numLeaves, uint32_t *index,
+// Tile nodes of one level, each covering cells^3 leaf cells from its origin:
+// those cells are marked kLeafTraverse (the node table decides there, also
+// where a dense leaf overlaps the tile).
+__global__ __launch_bounds__(kPackBlock) void kMarkTiles(GridView g, const int32_t *__restrict__ origins,
+                                                          uint64_t numTiles, uint32_t cells, uint32_t *__restrict__ index)
+{
+  const uint64_t t     = (uint64_t)blockIdx.x * kPackBlock + threadIdx.x;
+  const uint64_t per   = (uint64_t)cells * cells * cells;
+  if (t >= numTiles * per) return;
+  const uint64_t k = t / per, c = t % per;
+  const uint32_t x = ((uint32_t)(origins[3 * k] - g.rootOrigin[0]) >> 3) + (uint32_t)(c % cells);
+  const uint32_t y = ((uint32_t)(origins[3 * k + 1] - g.rootOrigin[1]) >> 3) + (uint32_t)((c / cells) % cells);
+  const uint32_t z = ((uint32_t)(origins[3 * k + 2] - g.rootOrigin[2]) >> 3) + (uint32_t)(c / cells / cells);
+  if (x < g.leafDims[0] && y < g.leafDims[1] && z < g.leafDims[2])
+    index[((uint64_t)z * g.leafDims[1] + y) * g.leafDims[0] + x] = kLeafTraverse;
+}
+
+hipError_t launchBuildLeafIndex(const GridView &g, const int32_t *origins, uint64_t numLeaves,
+                                const int32_t *const tileOrigins[3], const uint64_t numTiles[3], uint32_t *index,
                                 hipStream_t stream)
 {
   const uint64_t n = (uint64_t)g.leafDims[0] * g.leafDims[1] * g.leafDims[2];
   if (n) hipLaunchKernelGGL(kFillU32, dim3((unsigned)((n + kPackBlock - 1) / kPackBlock)), dim3(kPackBlock), 0, stream,
-                            index, n, 0xffffffffu);
+                            index, n, kLeafEmpty);
   if (numLeaves)
     hipLaunchKernelGGL(kSetLeafIndex, dim3((unsigned)((numLeaves + kPackBlock - 1) / kPackBlock)), dim3(kPackBlock), 0,
                        stream, g, origins, numLeaves, index);
+  // Tiles of input levels 1, 2, 3 span kRes[level] voxels: 512, 16, 1 leaf
+  // cells per axis.
+  for (int l = 0; l < 3; l++) {
+    if (!numTiles[l]) continue;
+    const uint32_t cells = kRes[l + 1] / 8 ? kRes[l + 1] / 8 : 1;
+    const uint64_t total = numTiles[l] * cells * cells * cells;
+    hipLaunchKernelGGL(kMarkTiles, dim3((unsigned)((total + kPackBlock - 1) / kPackBlock)), dim3(kPackBlock), 0, stream,
+                       g, tileOrigins[l], numTiles[l], cells, index);
+  }
   return hipGetLastError();
 }
 

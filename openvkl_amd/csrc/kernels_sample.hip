@@ -20,7 +20,10 @@ __global__ __launch_bounds__(kBlock) void kSample(GridView g, AttrView a, uint32
 {
   const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
-  const F3 oc = {xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]};
+  // Points and samples stream through once: non-temporal accesses keep them
+  // from displacing voxel lines in the caches.
+  const F3 oc = {__builtin_nontemporal_load(xyz + 3 * i), __builtin_nontemporal_load(xyz + 3 * i + 1),
+                 __builtin_nontemporal_load(xyz + 3 * i + 2)};
   const F3 p  = xfmPoint(g.o2i, oc);
   Src src(g, a, attr);
   if (times) src.setTime(times[i]);
@@ -31,7 +34,7 @@ __global__ __launch_bounds__(kBlock) void kSample(GridView g, AttrView a, uint32
     r = sampleCubic(src, p);
   else
     r = sampleLinear(src, p);
-  out[i * outStride] = r;
+  __builtin_nontemporal_store(r, out + i * outStride);
 }
 
 // VdbSampler.ispc:258-335: index-space gradient, then xfmNormal with the

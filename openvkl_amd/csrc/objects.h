@@ -227,6 +227,10 @@ struct IteratorContext : ManagedObject
   // consistent view.
   IteratorContextView refreshSampler();
   std::mutex viewMutex;
+  // Work counter of the persistent batch kernels (4 bytes, allocated on the
+  // first batch call); batchMutex serialises batch calls on this context.
+  DeviceBuffer batchWork;
+  std::mutex batchMutex;
   std::string toString() const override { return isHit ? "openvkl::HitIteratorContext" : "openvkl::IntervalIteratorContext"; }
 };
 

@@ -231,6 +231,10 @@ struct SparseSource
   const T *tile;
   const float *cells;         // corner-packed cell records (or null)
   const uint32_t *leafIndex;  // leaf cell -> dense-leaf slot (corner-packed)
+  // The same leaf-index grid used as a flattened node table: with full-depth
+  // sampling (no error voxels) a voxel of a dense leaf is found with one
+  // lookup instead of a three-level traversal (null: traverse).
+  const uint32_t *flatIndex;
   uint32_t *leafAccess;       // observed samplers: per leaf index, last access stamp
   int32_t root[3];
   uint32_t size[3];
@@ -244,6 +248,7 @@ struct SparseSource
         tile((const T *)a.tile),
         cells(!kObserve && g.maxSamplingDepth == g.packedDepth ? a.cells : nullptr),
         leafIndex(g.leafIndex),
+        flatIndex(!kObserve && g.maxSamplingDepth >= 3u ? g.leafIndex : nullptr),
         leafAccess(g.leafAccess),
         maxDepth(g.maxSamplingDepth),
         numAttr(g.numAttributes),
@@ -320,11 +325,24 @@ struct SparseSource
     return ox < size[0] && oy < size[1] && oz < size[2];
   }
 
+  // Leaf index grid entry of the 8^3 leaf cell holding root-relative voxel
+  // (ox, oy, oz): a dense-leaf slot, kLeafEmpty or kLeafTraverse.
+  VKL_DEV uint32_t flatSlot(uint32_t ox, uint32_t oy, uint32_t oz) const
+  {
+    return flatIndex[((uint64_t)(oz >> 3) * ldim[1] + (oy >> 3)) * ldim[0] + (ox >> 3)];
+  }
+
   // VdbSampler_sample_inner, VdbSampler_traverseAndSample.ih:144-197.
   VKL_DEV float at(int32_t x, int32_t y, int32_t z) const
   {
     const uint32_t ox = (uint32_t)(x - root[0]), oy = (uint32_t)(y - root[1]), oz = (uint32_t)(z - root[2]);
     if (!inDomain(ox, oy, oz)) return bg;
+    if (flatIndex) {
+      const uint32_t slot = flatSlot(ox, oy, oz);
+      if (slot < kLeafTraverse)
+        return toFloat(dense[(uint64_t)slot * 512u + (((ox & 7u) << 6) | ((oy & 7u) << 3) | (oz & 7u))]);
+      if (slot == kLeafEmpty) return 0.f;  // in the domain, covered by no node
+    }
     const Range1f *er  = nullptr;
     uint32_t eo        = 0;
     const uint64_t v   = traverse(ox, oy, oz, er, eo);
@@ -343,7 +361,7 @@ struct SparseSource
     const uint32_t ox = (uint32_t)(ic.x - root[0]), oy = (uint32_t)(ic.y - root[1]), oz = (uint32_t)(ic.z - root[2]);
     if (cells && inDomain(ox, oy, oz)) {
       const uint32_t slot = leafIndex[((uint64_t)(oz >> 3) * ldim[1] + (oy >> 3)) * ldim[0] + (ox >> 3)];
-      if (slot != 0xffffffffu) {
+      if (slot < kLeafTraverse) {
         const CellRecord<float> c = *reinterpret_cast<const CellRecord<float> *>(
             cells + ((uint64_t)slot * 512u + (((ox & 7u) << 6) | ((oy & 7u) << 3) | (oz & 7u))) * 8u);
 #pragma unroll
@@ -353,6 +371,18 @@ struct SparseSource
     }
     if (inDomain(ox, oy, oz) && inDomain(ox + 1, oy + 1, oz + 1) && (ox & 7u) != 7u && (oy & 7u) != 7u &&
         (oz & 7u) != 7u) {
+      if (flatIndex) {
+        const uint32_t slot = flatSlot(ox, oy, oz);
+        if (slot < kLeafTraverse) {
+          leafCorners(((uint64_t)slot << 6) | 7u, nullptr, 0, ox, oy, oz, s, true);
+          return;
+        }
+        if (slot == kLeafEmpty) {  // the stencil lies in one empty cell
+#pragma unroll
+          for (int i = 0; i < 8; i++) s[i] = 0.f;
+          return;
+        }
+      }
       const Range1f *er = nullptr;
       uint32_t eo       = 0;
       const uint64_t v  = traverse(ox, oy, oz, er, eo);
@@ -365,12 +395,15 @@ struct SparseSource
 
   // The 8 corners of a stencil inside one 8^3 leaf cell, given that cell's
   // traversal result.
+  // slotCode: v carries the dense-leaf slot itself (from the flattened
+  // index), not a leaf index to map through leafSlot.
   VKL_DEV void leafCorners(uint64_t v, const Range1f *er, uint32_t eo, uint32_t ox, uint32_t oy, uint32_t oz,
-                           float *s) const
+                           float *s, bool slotCode = false) const
   {
     const uint32_t tag = (uint32_t)v & 3u;
     if (tag == 3u && ((v >> 2) & 3u) == 1u) {
-      const T *p = dense + slotOf(v) * 512u + (((ox & 7u) << 6) | ((oy & 7u) << 3) | (oz & 7u));
+      const uint64_t slot = slotCode ? (v >> 6) : slotOf(v);
+      const T *p = dense + slot * 512u + (((ox & 7u) << 6) | ((oy & 7u) << 3) | (oz & 7u));
       s[0] = toFloat(p[0]);
       s[1] = toFloat(p[1]);
       s[2] = toFloat(p[8]);
@@ -750,6 +783,26 @@ VKL_DEV float sampleFiltered(const Src &src, int filter, F3 p, float nearestOffs
   return sampleLinear(src, p);
 }
 
+// Hit iteration descends to depth 2 on structuredRegular volumes and 3 on vdb
+// volumes (HitIteratorContext::commit, IteratorContext.cpp:285-336).
+template <typename S>
+struct IsDenseSource : std::false_type
+{
+};
+template <typename T, bool B>
+struct IsDenseSource<DenseSource<T, B>> : std::true_type
+{
+};
+template <typename T>
+struct IsDenseSource<TemporalDenseSource<T>> : std::true_type
+{
+};
+template <typename Src>
+constexpr int hitDepthOf()
+{
+  return IsDenseSource<Src>::value ? 2 : 3;
+}
+
 // Object-space sample with one source type and filter per instantiation (the
 // sampler entry of VdbSampler_iterator_computeSample_varying,
 // VdbSampler.ispc:341-374). Every caller gets its own inlined copy: the
@@ -759,6 +812,7 @@ VKL_DEV float sampleFiltered(const Src &src, int filter, F3 p, float nearestOffs
 template <typename Src, int kFilter>
 struct PointSampler
 {
+  using Source = Src;
   Src src;
   const float *o2i;
   float noff;
@@ -813,7 +867,16 @@ VKL_DEV float ddaTDelta(const DdaState &d, int axis)
   return (float)kRes[L + 1] * d.absRcp[axis];
 }
 
+// The DDA's deepest level: kDepth when the caller knows it at compile time
+// (hit iteration: 2 on structuredRegular, 3 on vdb), else d.level.
+template <int kDepth>
+VKL_DEV uint32_t ddaLevel(const DdaState &d)
+{
+  return kDepth >= 0 ? (uint32_t)kDepth : d.level;
+}
+
 // Dda.ih:145-196.
+template <int kDepth = -1>
 VKL_DEV void ddaInit(DdaState &d, F3 org, F3 dir, float tMin, uint32_t level)
 {
   const F3 oo = {org.x + tMin * dir.x, org.y + tMin * dir.y, org.z + tMin * dir.z};
@@ -830,23 +893,25 @@ VKL_DEV void ddaInit(DdaState &d, F3 org, F3 dir, float tMin, uint32_t level)
   d.absRcp[0] = fabsf(id.x);
   d.absRcp[1] = fabsf(id.y);
   d.absRcp[2] = fabsf(id.z);
+  // Levels below d.level are never read; they are written as 0 so that no
+  // earlier state stays live through a re-initialisation.
 #pragma unroll
   for (int l = 0; l < 4; ++l) {
-    if ((uint32_t)l > d.level) continue;
+    const bool used   = (uint32_t)l <= ddaLevel<kDepth>(d);
     const int32_t res = (int32_t)kRes[l + 1];
     const int32_t ix = clampCell(oo.x, res), iy = clampCell(oo.y, res), iz = clampCell(oo.z, res);
-    d.idx[DX(l)]   = ix;
-    d.idx[DY(l)]   = iy;
-    d.idx[DZ(l)]   = iz;
-    d.tNext[DX(l)] = planeT((float)(ix + res * (sx >= 0)), org.x, id.x);
-    d.tNext[DY(l)] = planeT((float)(iy + res * (sy >= 0)), org.y, id.y);
-    d.tNext[DZ(l)] = planeT((float)(iz + res * (sz >= 0)), org.z, id.z);
+    d.idx[DX(l)]   = used ? ix : 0;
+    d.idx[DY(l)]   = used ? iy : 0;
+    d.idx[DZ(l)]   = used ? iz : 0;
+    d.tNext[DX(l)] = used ? planeT((float)(ix + res * (sx >= 0)), org.x, id.x) : 0.f;
+    d.tNext[DY(l)] = used ? planeT((float)(iy + res * (sy >= 0)), org.y, id.y) : 0.f;
+    d.tNext[DZ(l)] = used ? planeT((float)(iz + res * (sz >= 0)), org.z, id.z) : 0.f;
   }
 }
 
 // Dda.ih:204-266: levels L + 1 .. d.level restart from the current t, nudged
 // 1e-2 across the plane the level-L step crossed.
-template <int L>
+template <int kDepth, int L>
 VKL_DEV void ddaReinitBelow(DdaState &d, int32_t dx, int32_t dy)
 {
   const F3 dir = {d.rayDir[0], d.rayDir[1], d.rayDir[2]};
@@ -861,7 +926,7 @@ VKL_DEV void ddaReinitBelow(DdaState &d, int32_t dx, int32_t dy)
     oo.z = oo.z + ((float)sz) * 1e-2f;
 #pragma unroll
   for (int l = L + 1; l < 4; ++l) {
-    if ((uint32_t)l > d.level) continue;
+    if ((uint32_t)l > ddaLevel<kDepth>(d)) continue;
     const int32_t res = (int32_t)kRes[l + 1];
     d.idx[DX(l)]      = clampCell(oo.x, res);
     d.idx[DY(l)]      = clampCell(oo.y, res);
@@ -913,7 +978,7 @@ VKL_DEV void hddaParents(DdaState &d)
 // HDDA.ih.in:24-101: step on level L (the level of the empty voxel),
 // re-initialise the levels below, and let parents follow a child that left
 // their cell.
-template <int L>
+template <int kDepth, int L>
 VKL_DEV void hddaStepAt(DdaState &d)
 {
   if constexpr (L >= 3) {
@@ -921,18 +986,19 @@ VKL_DEV void hddaStepAt(DdaState &d)
   } else {
     const int32_t ox = d.idx[DX(L)], oy = d.idx[DY(L)];
     ddaStep<L>(d);
-    if ((uint32_t)L < d.level) ddaReinitBelow<L>(d, d.idx[DX(L)] - ox, d.idx[DY(L)] - oy);
+    if ((uint32_t)L < ddaLevel<kDepth>(d)) ddaReinitBelow<kDepth, L>(d, d.idx[DX(L)] - ox, d.idx[DY(L)] - oy);
   }
   hddaParents<(L >= 3 ? 3 : L) - 1>(d);
 }
 
+template <int kDepth>
 VKL_DEV void hddaStep(DdaState &d, uint32_t descLevel)
 {
   switch (descLevel) {
-  case 0: hddaStepAt<0>(d); break;
-  case 1: hddaStepAt<1>(d); break;
-  case 2: hddaStepAt<2>(d); break;
-  default: hddaStepAt<3>(d); break;
+  case 0: hddaStepAt<kDepth, 0>(d); break;
+  case 1: hddaStepAt<kDepth, 1>(d); break;
+  case 2: hddaStepAt<kDepth, 2>(d); break;
+  default: hddaStepAt<kDepth, 3>(d); break;
   }
 }
 
@@ -945,10 +1011,12 @@ VKL_DEV int32_t opaque(int32_t v)
   return v;
 }
 
-// The cell of the DDA's current level (d.level), selected without a
-// run-time array index.
+// The cell of the DDA's deepest level, selected without a run-time array
+// index.
+template <int kDepth>
 VKL_DEV I3 ddaCell(const DdaState &d)
 {
+  if constexpr (kDepth >= 0) return I3{d.idx[DX(kDepth)], d.idx[DY(kDepth)], d.idx[DZ(kDepth)]};
   I3 c = {opaque(d.idx[DX(0)]), opaque(d.idx[DY(0)]), opaque(d.idx[DZ(0)])};
 #pragma unroll
   for (int l = 1; l < 4; ++l) {
@@ -1063,6 +1131,7 @@ VKL_DEV Range1f intersectBox(F3 org, F3 dir, const float *box, Range1f lim)
 }
 
 // VdbIterator_Initialize, VdbIterator.ispc:36-94.
+template <int kDepth = -1>
 VKL_DEV void intervalInit(const IteratorContextView &c, IntervalState &it, F3 origin, F3 direction, Range1f tRange)
 {
   const GridView &g = c.sampler.grid;
@@ -1079,11 +1148,11 @@ VKL_DEV void intervalInit(const IteratorContextView &c, IntervalState &it, F3 or
   // state is written on both paths.
   const bool miss = ti.upper <= ti.lower;
   it.tMax         = miss ? -vinf() : ti.upper;
-  ddaInit(it.dda, org, dir, miss ? 0.f : ti.lower, c.maxIteratorDepth);
+  ddaInit<kDepth>(it.dda, org, dir, miss ? 0.f : ti.lower, c.maxIteratorDepth);
 }
 
 // VdbIterator_iterateIntervalInternal, VdbIterator.ispc:96-159.
-template <bool kNodeCache = false>
+template <bool kNodeCache = false, int kDepth = -1>
 VKL_DEV bool intervalNext(const IteratorContextView &c, IntervalState &it, IntervalOut &out)
 {
   if (!(it.dda.t <= it.tMax)) return false;
@@ -1101,20 +1170,20 @@ VKL_DEV bool intervalNext(const IteratorContextView &c, IntervalState &it, Inter
       it.tMax = -vinf();
       return false;
     }
-    const uint32_t l = it.dda.level;
-    const I3 cell    = ddaCell(it.dda);
+    const uint32_t l = ddaLevel<kDepth>(it.dda);
+    const I3 cell    = ddaCell<kDepth>(it.dda);
     if (dense)
       queryVoxelDense(c, cell.x, cell.y, cell.z, l, v);
     else
       queryVoxel<kNodeCache>(c, cell.x, cell.y, cell.z, l, v, it);
     if (!v.isEmpty) break;
-    hddaStep(it.dda, v.level);
+    hddaStep<kDepth>(it.dda, v.level);
   }
   if (it.dda.t < it.tMax) {
     out.vLower = v.valueRange.lower;
     out.vUpper = v.valueRange.upper;
     out.tLower = it.dda.t;
-    hddaStep(it.dda, v.level);
+    hddaStep<kDepth>(it.dda, v.level);
     out.tUpper        = vmin(it.dda.t, it.tMax);
     out.nominalDeltaT = it.nominalDeltaT;
     return true;
@@ -1135,6 +1204,7 @@ template <typename S>
 VKL_DEV float bisect(const S &smp, F3 o, F3 d, float t0, float s0, float t, float s, float iso, float tol)
 {
   float tMid = 0.f;
+#pragma unroll 1
   for (int iter = 0; iter < 10; ++iter) {
     tMid            = 0.5f * (t0 + t);
     const float err = 0.5f * (t - t0);
@@ -1212,9 +1282,10 @@ VKL_DEV bool intersectBisection(const IteratorContextView &c, const S &smp, F3 o
   return false;
 }
 
+template <int kDepth = -1>
 VKL_DEV void hitInit(const IteratorContextView &c, HitState &h, F3 origin, F3 direction, Range1f tRange, float time)
 {
-  intervalInit(c, h.iv, origin, direction, tRange);
+  intervalInit<kDepth>(c, h.iv, origin, direction, tRange);
   h.origin[0] = origin.x;
   h.origin[1] = origin.y;
   h.origin[2] = origin.z;
@@ -1241,7 +1312,7 @@ VKL_DEV bool hitNext(const IteratorContextView &c, const S &smp, HitState &h, Hi
     bool have       = !need;
     if (need) {
       IntervalOut iv;
-      have = intervalNext<true>(c, h.iv, iv);
+      have = intervalNext<true, hitDepthOf<typename S::Source>()>(c, h.iv, iv);
       if (have) {
         h.curLower         = iv.tLower;
         h.curUpper         = iv.tUpper;

@@ -26,6 +26,10 @@ enum : uint32_t {
 
 enum : int { kFilterNearest = 0, kFilterLinear = 100, kFilterCubic = 200 };
 
+// Leaf index grid entries that are not dense-leaf slots.
+constexpr uint32_t kLeafEmpty    = 0xffffffffu;
+constexpr uint32_t kLeafTraverse = 0xfffffffeu;
+
 struct Range1f
 {
   float lower, upper;
@@ -88,9 +92,11 @@ struct GridView
   const AttrView *attrs;      // device array [numAttributes]
   const uint64_t *leafSlot;   // vdb with per-leaf node.data: leaf index -> packed slot
   LevelView levels[3];
-  // vdb with "cornerPacked": dense-leaf slot (or 0xffffffff) per 8^3 leaf
-  // cell of the domain, x fastest, leafDims = ceil(activeSize / 8); the cell
-  // records hold values for maxSamplingDepth == packedDepth.
+  // vdb leaf index grid: per 8^3 leaf cell of the domain (x fastest,
+  // leafDims = ceil(activeSize / 8)) the dense-leaf slot, kLeafEmpty where
+  // no node covers the cell, kLeafTraverse where a tile node does (the node
+  // table decides there). With "cornerPacked" the cell records hold values
+  // for maxSamplingDepth == packedDepth. Null when not built.
   const uint32_t *leafIndex;
   uint32_t leafDims[3];
   uint32_t packedDepth;

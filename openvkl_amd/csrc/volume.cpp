@@ -764,18 +764,51 @@ void VdbVolume::commit()
   // "cornerPacked" (bool): a leaf index grid over the domain plus, per dense
   // leaf, 512 cell records of the 8 corner values the sampler reads (8 floats
   // per cell; baked for this volume's maxSamplingDepth).
+  //
+  // "leafIndexGrid" (int; -1 automatic, 0 off, 1 on): the same leaf index
+  // grid without the records, which full-depth sampling uses as a flattened
+  // node table (one lookup per stencil instead of three dependent node
+  // loads). Automatic: built when it takes at most 1 GiB (4 bytes per 8^3
+  // cell of the domain: 512 MiB for a 4096^3 domain).
   grid.leafIndex   = nullptr;
   grid.packedDepth = 0xffffffffu;
-  if (getBool("cornerPacked", false) && anyTemporal)
+  const bool cornerPacked = getBool("cornerPacked", false);
+  if (cornerPacked && anyTemporal)
     throw VklError("cornerPacked is not supported for temporal data", VKL_INVALID_ARGUMENT);
-  if (getBool("cornerPacked", false)) {
-    for (int c = 0; c < 3; c++) grid.leafDims[c] = (grid.activeSize[c] + 7) / 8;
-    const uint64_t cellsOfGrid = (uint64_t)grid.leafDims[0] * grid.leafDims[1] * grid.leafDims[2];
-    const uint64_t numDense    = denseLeafOrigins.size() / 3;
+  for (int c = 0; c < 3; c++) grid.leafDims[c] = (grid.activeSize[c] + 7) / 8;
+  const uint64_t cellsOfGrid = (uint64_t)grid.leafDims[0] * grid.leafDims[1] * grid.leafDims[2];
+  const int indexMode        = getInt("leafIndexGrid", -1);
+  const bool indexGrid = !anyTemporal && (cornerPacked || indexMode > 0 ||
+                                          (indexMode < 0 && cellsOfGrid * sizeof(uint32_t) <= (1ull << 30)));
+  if (indexGrid) {
+    const uint64_t numDense = denseLeafOrigins.size() / 3;
+    DeviceBuffer origins(std::max<size_t>(denseLeafOrigins.size() * sizeof(int32_t), 4));
+    if (numDense) origins.upload(denseLeafOrigins.data(), denseLeafOrigins.size() * sizeof(int32_t), st);
+    // tile nodes per input level 1..3
+    std::vector<int32_t> tileOrg[3];
+    for (uint64_t i = 0; i < n; i++)
+      if (format[i] == VKL_FORMAT_TILE && level[i] >= 1 && level[i] <= 3)
+        tileOrg[level[i] - 1].insert(tileOrg[level[i] - 1].end(), &origin[3 * i], &origin[3 * i] + 3);
+    DeviceBuffer tileBuf[3];
+    const int32_t *tilePtr[3] = {nullptr, nullptr, nullptr};
+    uint64_t numTiles[3]      = {0, 0, 0};
+    for (int l = 0; l < 3; l++) {
+      numTiles[l] = tileOrg[l].size() / 3;
+      if (!numTiles[l]) continue;
+      tileBuf[l].allocate(tileOrg[l].size() * sizeof(int32_t));
+      tileBuf[l].upload(tileOrg[l].data(), tileOrg[l].size() * sizeof(int32_t), st);
+      tilePtr[l] = tileBuf[l].as<int32_t>();
+    }
+    DeviceBuffer *index = newBuffer(cellsOfGrid * sizeof(uint32_t));
+    checkHip(launchBuildLeafIndex(grid, origins.as<int32_t>(), numDense, tilePtr, numTiles, index->as<uint32_t>(), st),
+             "leaf index");
+    checkHip(hipStreamSynchronize(st), "leaf index");
+    grid.leafIndex = index->as<uint32_t>();
+  }
+  if (cornerPacked) {
+    const uint64_t numDense = denseLeafOrigins.size() / 3;
     DeviceBuffer origins(denseLeafOrigins.size() * sizeof(int32_t));
     origins.upload(denseLeafOrigins.data(), denseLeafOrigins.size() * sizeof(int32_t), st);
-    DeviceBuffer *index = newBuffer(cellsOfGrid * sizeof(uint32_t));
-    checkHip(launchBuildLeafIndex(grid, origins.as<int32_t>(), numDense, index->as<uint32_t>(), st), "leaf index");
     for (uint32_t a = 0; a < grid.numAttributes; a++) {
       DeviceBuffer *cells = newBuffer(numDense * 512 * 8 * sizeof(float));
       checkHip(launchPackLeafCells(grid, attrViews[a], a, origins.as<int32_t>(), numDense, cells->as<float>(), st),
@@ -784,7 +817,6 @@ void VdbVolume::commit()
     }
     checkHip(hipStreamSynchronize(st), "corner packing");
     uploadAttrViews(*this, st);
-    grid.leafIndex   = index->as<uint32_t>();
     grid.packedDepth = grid.maxSamplingDepth;
   }
   committed = true;

@@ -37,6 +37,17 @@ def build(force: bool = False) -> str:
     return _LIB_PATH
 
 
+def use_native():
+    """Load the -O3 -march=native build of the same source (built here, for
+    the host that runs it) instead of the default one: the CPU baseline's
+    timed copy. Call before the first lib() use."""
+    global _LIB_PATH, _lib
+    if _lib is not None:
+        raise RuntimeError("the oracle library is already loaded")
+    subprocess.run(["make", "-B", "-C", _HERE, "native"], check=True, stdout=subprocess.DEVNULL)
+    _LIB_PATH = os.path.join(_HERE, "_build", "liboracle_native.so")
+
+
 def lib():
     global _lib
     if _lib is None:

@@ -155,3 +155,43 @@ def test_index_to_object_transform(device):
     pts = V.uniform_points(100000, box[:3], box[3:])
     assert_same(vol.sampler().sample(pts), grid.sample(pts))
     assert_same(vol.sampler().gradient(pts), grid.gradient(pts))
+
+
+@pytest.mark.parametrize("index_grid", [0, 1])
+def test_leaf_index_grid_on_and_off(device, index_grid):
+    """The leaf index grid as a flattened node table (dense-leaf slots, empty
+    cells, cells under tiles of levels 1..3 left to the node table) samples
+    exactly as the traversal: every filter, gradients, maxSamplingDepth 1..3,
+    points in empty space, in leaves, under tiles and outside the domain."""
+    origins, leaves = V.sparse_shell_leaves(res=128, radius=0.3, width=0.08)
+    n = len(origins)
+    tile_orig = np.array([[256, 0, 0], [144, 8, 8], [0, 0, 4096], [136, 0, 0]], np.int32)
+    tile_lvl = np.array([2, 3, 1, 3], np.uint32)
+    lv = np.concatenate([np.full(n, 3, np.uint32), tile_lvl])
+    org = np.concatenate([origins, tile_orig])
+    fmt = np.concatenate([np.full(n, vkl.VKL_FORMAT_DENSE_ZYX, np.uint32), np.zeros(4, np.uint32)])
+    leaf_data = [[leaves[i]] for i in range(n)] + [[np.array([0.75 + k], np.float32)] for k in range(4)]
+    vol = device.vdb(lv, org, fmt, leaf_data=leaf_data, leaf_index_grid=index_grid)
+    grid = O.Grid.vdb(lv, org, fmt, leaf_data=leaf_data)
+    box = grid.bounding_box()
+    pts = np.concatenate([V.uniform_points(100000, (-2, -2, -2), (130, 130, 130), seed=81),
+                          V.uniform_points(50000, (250, -2, -2), (390, 130, 130), seed=82),
+                          V.uniform_points(30000, box[:3] - 2, box[3:] + 2, seed=83)])
+    for depth in (3, 2, 1):
+        for f in (vkl.VKL_FILTER_NEAREST, vkl.VKL_FILTER_LINEAR):
+            grid.set_sampler(f, f, max_sampling_depth=depth)
+            s = vol.sampler(filter=f, gradient_filter=f, max_sampling_depth=depth)
+            assert_same(s.sample(pts), grid.sample(pts))
+            assert_same(s.gradient(pts[:20000]), grid.gradient(pts[:20000]))
+            s.release()
+    grid.set_sampler(vkl.VKL_FILTER_CUBIC)
+    s = vol.sampler(filter=vkl.VKL_FILTER_CUBIC)
+    got = s.sample(pts[:20000])
+    grid.set_cubic_double(True)
+    want = grid.sample(pts[:20000])
+    grid.set_cubic_double(False)
+    assert np.array_equal(np.isnan(got), np.isnan(want))
+    ok = ~np.isnan(want)
+    assert np.abs(got[ok] - want[ok]).max() <= 1e-5 * max(np.abs(want[ok]).max(), 1e-30)
+    grid.set_sampler()
+    vol.release()
