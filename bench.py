@@ -469,12 +469,11 @@ def copy_peak(torch, steps=20):
     return {"gbs": round(2 * 4 * n / (ms * 1e-3) / 1e9, 1), "bytes_per_call": 2 * 4 * n, "ms": round(ms, 4)}
 
 
-def random_line_ceiling(torch, npts, steps=10):
+def random_line_ceiling(torch, npts, steps=10, rows=1 << 27):
     """The measured ceiling of a sampler that reads one random 32-byte record
     per point in random order: torch.take of one 4-byte element from each of
-    npts uniformly random 32-byte records of a 4 GiB array (8 B index in,
-    4 B value out, one random cache line per point)."""
-    rows = 1 << 27
+    npts uniformly random 32-byte records of a rows * 32 B array (4 GiB by
+    default: 8 B index in, 4 B value out, one random cache line per point)."""
     buf = torch.zeros(rows * 8, device="cuda")
     g = torch.Generator(device="cuda").manual_seed(SEED + 20)
     idx = torch.randint(0, rows, (npts,), device="cuda", generator=g) * 8
@@ -574,6 +573,10 @@ def main():
     log("measuring the copy peak and the random-record gather ceiling ...")
     peak = copy_peak(torch)
     ceiling = random_line_ceiling(torch, npts)
+    # the same random gathers over smaller footprints (Infinity Cache / L2)
+    by_footprint = {f"{(r * 32) >> 20} MiB": random_line_ceiling(torch, npts, rows=r)["gsamples_s"]
+                    for r in (1 << 19, 1 << 24)}
+    by_footprint[f"{(1 << 27) * 32 >> 20} MiB"] = ceiling["gsamples_s"]
     log(f"copy {peak['gbs']} GB/s; random 32-B record gather {ceiling['gsamples_s']} Gsamples/s")
     result = {
         "metric": METRIC,
@@ -606,7 +609,8 @@ def main():
                          "what": "torch.take of one element from each of 2^26 uniformly random 32-byte records "
                                  "of a 4 GiB array: one random cache line per point, the floor of any sampler "
                                  "that reads random points in random order",
-                         "kernel_frac_of_ceiling": round(npts / (ms_event * 1e-3) / 1e9 / ceiling["gsamples_s"], 4)}},
+                         "kernel_frac_of_ceiling": round(npts / (ms_event * 1e-3) / 1e9 / ceiling["gsamples_s"], 4),
+                         "gsamples_s_by_footprint": by_footprint}},
     }
     variants = [v for v in args.variants.split(",") if v]
     extra = {}
