@@ -150,15 +150,19 @@ OPENVKL_INTERFACE int vklHipGetVolumeLevel(VKLVolume volume, unsigned int level,
 
 typedef struct VKLSampleBatchPlan_ *VKLSampleBatchPlan;
 
-/* Plan one launch that samples numBatches (sampler, points) pairs:
+/* Plan the sampling of numBatches (sampler, points) pairs:
  * samples[i][k] = sample of attribute attributeIndices[i] (NULL = 0) of
  * samplers[i] at objectCoordinates[i][k], k < counts[i]. Point and sample
  * arrays must be device memory and stay valid while the plan is used; the
- * samplers are retained. The launch is a persistent kernel over the
- * flattened point space, captured in a hipGraph; with autotune != 0 its
- * shape is chosen by timing candidates once per workload signature (cached,
- * and persisted to the file OPENVKL_HIP_TUNE_CACHE names). Results equal one
- * vklComputeSampleN per batch. */
+ * samplers are retained. Each group of batches (see
+ * vklGetSampleBatchPlanLaunchCount) is one persistent kernel over its
+ * flattened point space; the launches are captured in one hipGraph.
+ * autotune: 0 = fixed shape, batches grouped per voxel source type and
+ * filter; 1 = shape and grouping chosen by timing candidates once per
+ * workload signature (cached, and persisted to the file
+ * OPENVKL_HIP_TUNE_CACHE names); 2 = fixed shape, structuredRegular batches
+ * of a filter merged into one group. Results equal one vklComputeSampleN per
+ * batch. */
 OPENVKL_INTERFACE VKLSampleBatchPlan vklNewSampleBatchPlan(unsigned int numBatches, const VKLSampler *samplers,
                                                            const unsigned int *attributeIndices,
                                                            const unsigned int *counts,
@@ -169,6 +173,11 @@ OPENVKL_INTERFACE void vklExecuteSampleBatchPlan(VKLSampleBatchPlan plan, void *
 /* The launch shape the plan uses. */
 OPENVKL_INTERFACE void vklGetSampleBatchPlanLaunch(VKLSampleBatchPlan plan, unsigned int *blocks,
                                                    unsigned int *threads, unsigned int *tile);
+/* Kernel launches one execution of the plan makes: one per group of batches.
+ * Batches are grouped per voxel source type and filter, or, where the
+ * autotuner measured it faster, with all structuredRegular volumes of 32-bit
+ * voxel offsets and one filter in one group whatever their element types. */
+OPENVKL_INTERFACE unsigned int vklGetSampleBatchPlanLaunchCount(VKLSampleBatchPlan plan);
 OPENVKL_INTERFACE void vklReleaseSampleBatchPlan(VKLSampleBatchPlan plan);
 
 /* ---- neural-field sampler ---------------------------------------------------

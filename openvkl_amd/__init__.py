@@ -160,6 +160,7 @@ def lib():
         "vklNewSampleBatchPlan": (vp, [C.c_uint, vp, vp, vp, vp, vp, i32]),
         "vklExecuteSampleBatchPlan": (None, [vp, vp]),
         "vklGetSampleBatchPlanLaunch": (None, [vp, C.POINTER(C.c_uint), C.POINTER(C.c_uint), C.POINTER(C.c_uint)]),
+        "vklGetSampleBatchPlanLaunchCount": (C.c_uint, [vp]),
         "vklReleaseSampleBatchPlan": (None, [vp]),
         "vklHipGetVolumeLevel": (i32, [O, C.c_uint, C.POINTER(C.c_uint64), C.POINTER(C.POINTER(C.c_uint64)),
                                        C.POINTER(C.POINTER(range1f))]),
@@ -781,6 +782,9 @@ class SampleBatchPlan:
     many (sampler, device point array) pairs."""
 
     def __init__(self, device, samplers, points, outs, attrs=None, autotune=True):
+        """autotune: True / 1 tunes shape and grouping; False / 0 fixed shape,
+        per-type groups; 2 fixed shape, structuredRegular batches of a filter
+        merged into one launch."""
         n = len(samplers)
         self.device = device
         self._samplers = (VKLObject * n)(*[s.h for s in samplers])
@@ -800,6 +804,10 @@ class SampleBatchPlan:
         b, t, k = C.c_uint(), C.c_uint(), C.c_uint()
         lib().vklGetSampleBatchPlanLaunch(self.h, C.byref(b), C.byref(t), C.byref(k))
         return b.value, t.value, k.value
+
+    def launch_count(self):
+        """Kernel launches per execution (one per group of batches)."""
+        return int(lib().vklGetSampleBatchPlanLaunchCount(self.h))
 
     def execute(self, stream=None):
         lib().vklExecuteSampleBatchPlan(self.h, stream)

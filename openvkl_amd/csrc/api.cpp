@@ -119,8 +119,11 @@ void IteratorContext::commit()
   const int attrIndex = getInt("attributeIndex", 0);
   if (attrIndex < 0 || (uint32_t)attrIndex >= sampler->volume->numAttributes())
     throw VklError("illegal attributeIndex requested on volume");
-  view.sampler        = sampler->view;
-  samplerVersion      = sampler->viewVersion;
+  {
+    std::lock_guard<std::mutex> slock(sampler->viewMutex);
+    view.sampler   = sampler->view;
+    samplerVersion = sampler->viewVersion;
+  }
   view.attributeIndex = (uint32_t)attrIndex;
   std::vector<Range1f> ranges;
   std::vector<float> values;

@@ -1,15 +1,22 @@
 // Batched sampling plans: a device descriptor table for many (sampler,
-// ragged point batch) pairs, grouped by kernel specialisation (voxel source
-// type x filter); one persistent kernel launch per group, all captured in one
-// hipGraph; a small autotuner for the launch shape.
+// ragged point batch) pairs, grouped for launch; one persistent kernel launch
+// per group, all captured in one hipGraph; a small autotuner for the launch
+// shape and the grouping.
+//
+// Two groupings are kept: per kernel specialisation (voxel source type x
+// filter), and "merged", where the structuredRegular batches with 32-bit
+// voxel offsets of one filter share one launch whatever their element types
+// (the type is chosen per batch inside the kernel; its registers are the
+// maximum over the six element types, which can cost occupancy).
 //
 // Autotuning: on plan creation the candidate shapes (threads per workgroup x
-// points per tile x workgroups per CU) are timed with HIP events over a few
-// executions of all groups and the fastest is kept. Results are cached in
-// memory per workload signature (batch count and, per group, its
-// specialisation key, batch count and total points rounded to a power of two)
-// and, if OPENVKL_HIP_TUNE_CACHE names a file, persisted there as text lines
-// "signature blocks threads tile".
+// points per tile x workgroups per CU, each with either grouping when they
+// differ) are timed with HIP events over a few executions of all groups and
+// the fastest is kept. Results are cached in memory per workload signature
+// (batch count and, per group, its specialisation key, batch count and total
+// points rounded to a power of two) and, if OPENVKL_HIP_TUNE_CACHE names a
+// file, persisted there as text lines "signature blocks threads tile merged".
+// Without autotuning the per-specialisation grouping is used.
 
 #include <hip/hip_runtime.h>
 
@@ -31,8 +38,10 @@ struct SampleBatchPlan
 {
   Device *device = nullptr;
   std::vector<Sampler *> samplers;  // retained
-  DeviceBuffer items, begins;       // all groups' items / offsets, group after group
-  std::vector<BatchGroup> groups;
+  // [0]: per specialisation, [1]: merged structuredRegular groups; each the
+  // groups' items / offsets, group after group
+  DeviceBuffer items[2], begins[2];
+  std::vector<BatchGroup> groups[2];
   uint32_t numItems = 0;
   uint64_t total    = 0;
   BatchLaunch launch{};
@@ -61,9 +70,14 @@ void loadTuneFile()
   const char *path = std::getenv("OPENVKL_HIP_TUNE_CACHE");
   if (!path) return;
   std::ifstream f(path);
-  std::string sig;
-  BatchLaunch l;
-  while (f >> sig >> l.blocks >> l.threads >> l.tile) g_tuneCache[sig] = l;
+  std::string line;
+  while (std::getline(f, line)) {
+    std::istringstream in(line);
+    std::string sig;
+    BatchLaunch l{};
+    if (in >> sig >> l.blocks >> l.threads >> l.tile >> l.merged && l.merged <= 1 && l.threads && l.blocks)
+      g_tuneCache[sig] = l;
+  }
 }
 
 void storeTune(const std::string &sig, const BatchLaunch &l)
@@ -71,17 +85,24 @@ void storeTune(const std::string &sig, const BatchLaunch &l)
   g_tuneCache[sig] = l;
   if (const char *path = std::getenv("OPENVKL_HIP_TUNE_CACHE")) {
     std::ofstream f(path, std::ios::app);
-    f << sig << " " << l.blocks << " " << l.threads << " " << l.tile << "\n";
+    f << sig << " " << l.blocks << " " << l.threads << " " << l.tile << " " << l.merged << "\n";
   }
 }
 
-std::vector<BatchLaunch> candidates(int cus)
+std::vector<BatchLaunch> candidates(int cus, bool canMerge)
 {
   std::vector<BatchLaunch> c;
-  for (uint32_t threads : {256u, 512u, 1024u})
-    for (uint32_t ppt : {1u, 4u, 16u})
-      for (uint32_t perCu : {2u, 8u}) c.push_back({(uint32_t)cus * perCu, threads, threads * ppt});
+  for (uint32_t merged = 0; merged <= (canMerge ? 1u : 0u); merged++)
+    for (uint32_t threads : {256u, 512u, 1024u})
+      for (uint32_t ppt : {1u, 4u, 16u})
+        for (uint32_t perCu : {2u, 8u}) c.push_back({(uint32_t)cus * perCu, threads, threads * ppt, merged});
   return c;
+}
+
+hipError_t launchPlan(const SampleBatchPlan &p, const BatchLaunch &c, hipStream_t st)
+{
+  const std::vector<BatchGroup> &g = p.groups[c.merged];
+  return launchSampleBatch(g.data(), (uint32_t)g.size(), c, st);
 }
 
 BatchLaunch tune(SampleBatchPlan &p, hipStream_t st)
@@ -98,11 +119,10 @@ BatchLaunch tune(SampleBatchPlan &p, hipStream_t st)
   checkHip(hipEventCreate(&b), "hipEventCreate");
   BatchLaunch best{};
   float bestMs = 1e30f;
-  for (const BatchLaunch &c : candidates(cus)) {
-    checkHip(launchSampleBatch(p.groups.data(), (uint32_t)p.groups.size(), c, st), "batch tune");
+  for (const BatchLaunch &c : candidates(cus, p.groups[1].size() < p.groups[0].size())) {
+    checkHip(launchPlan(p, c, st), "batch tune");
     checkHip(hipEventRecord(a, st), "hipEventRecord");
-    for (int r = 0; r < 3; r++)
-      checkHip(launchSampleBatch(p.groups.data(), (uint32_t)p.groups.size(), c, st), "batch tune");
+    for (int r = 0; r < 3; r++) checkHip(launchPlan(p, c, st), "batch tune");
     checkHip(hipEventRecord(b, st), "hipEventRecord");
     checkHip(hipEventSynchronize(b), "hipEventSynchronize");
     float ms = 0.f;
@@ -141,8 +161,6 @@ OPENVKL_INTERFACE VKLSampleBatchPlan vklNewSampleBatchPlan(unsigned int numBatch
     dev->makeCurrent();
     auto *p   = new SampleBatchPlan();
     p->device = dev;
-    // Group the batches by kernel specialisation (order kept within a group).
-    std::map<uint32_t, std::vector<unsigned>> byKey;
     std::vector<BatchItem> all(numBatches);
     for (unsigned i = 0; i < numBatches; i++) {
       Sampler *s = static_cast<Sampler *>(samplers[i].host);
@@ -154,7 +172,7 @@ OPENVKL_INTERFACE VKLSampleBatchPlan vklNewSampleBatchPlan(unsigned int numBatch
       p->samplers.push_back(s);
       const uint32_t a = attributeIndices ? attributeIndices[i] : 0;
       BatchItem &it    = all[i];
-      it.sampler       = s->view;
+      it.sampler       = s->viewCopy();
       it.attr          = s->attr(a);
       it.attrIndex     = a;
       it.xyz           = reinterpret_cast<const float *>(objectCoordinates[i]);
@@ -164,56 +182,75 @@ OPENVKL_INTERFACE VKLSampleBatchPlan vklNewSampleBatchPlan(unsigned int numBatch
         delete p;
         throw VklError("sample batch: point and sample arrays must be device memory");
       }
-      byKey[sourceKey(s->view.grid, it.attr, s->view.filter)].push_back(i);
     }
-    std::vector<BatchItem> items;
-    std::vector<uint64_t> begins;
-    std::vector<size_t> firstOf;
     std::ostringstream sig;
     sig << "b" << numBatches;
-    uint64_t total = 0;
-    for (const auto &kv : byKey) {
-      BatchGroup g{};
-      const BatchItem &first = all[kv.second.front()];
-      g.grid                 = first.sampler.grid;
-      g.attr                 = first.attr;
-      g.filter               = first.sampler.filter;
-      g.numItems             = (uint32_t)kv.second.size();
-      firstOf.push_back(items.size());
-      uint64_t gt = 0;
-      for (unsigned i : kv.second) {
-        BatchItem it = all[i];
-        it.begin     = gt;
-        begins.push_back(gt);
-        items.push_back(it);
-        gt += it.count;
+    for (int merged = 0; merged < 2; merged++) {
+      // Group the batches (order kept within a group): per kernel
+      // specialisation, or with the structuredRegular 32-bit-offset batches
+      // of a filter in one group.
+      std::map<uint32_t, std::vector<unsigned>> byKey;
+      for (unsigned i = 0; i < numBatches; i++) {
+        const BatchItem &it = all[i];
+        const int f         = it.sampler.filter;
+        byKey[merged ? groupKey(it.sampler.grid, it.attr, f) : sourceKey(it.sampler.grid, it.attr, f)].push_back(i);
       }
-      g.total = gt;
-      total += gt;
-      p->groups.push_back(g);
-      uint64_t pow2 = 1;
-      while (pow2 < gt) pow2 <<= 1;
-      sig << "_k" << kv.first << "x" << kv.second.size() << "n" << pow2;
+      std::vector<BatchItem> items;
+      std::vector<uint64_t> begins;
+      std::vector<size_t> firstOf;
+      uint64_t total = 0;
+      for (const auto &kv : byKey) {
+        BatchGroup g{};
+        const BatchItem &first = all[kv.second.front()];
+        g.grid                 = first.sampler.grid;
+        g.attr                 = first.attr;
+        g.filter               = first.sampler.filter;
+        g.numItems             = (uint32_t)kv.second.size();
+        // A merged group whose batches share one element type runs the
+        // kernel specialised for it.
+        if (merged && mergesDense(g.grid, g.attr))
+          for (unsigned i : kv.second)
+            if (all[i].attr.type != first.attr.type) g.mergedDense = 1;
+        firstOf.push_back(items.size());
+        uint64_t gt = 0;
+        for (unsigned i : kv.second) {
+          BatchItem it = all[i];
+          it.begin     = gt;
+          begins.push_back(gt);
+          items.push_back(it);
+          gt += it.count;
+        }
+        g.total = gt;
+        total += gt;
+        p->groups[merged].push_back(g);
+        if (!merged) {
+          uint64_t pow2 = 1;
+          while (pow2 < gt) pow2 <<= 1;
+          sig << "_k" << kv.first << "x" << kv.second.size() << "n" << pow2;
+        }
+      }
+      p->total = total;
+      p->items[merged].allocate(items.size() * sizeof(BatchItem));
+      p->items[merged].upload(items.data(), items.size() * sizeof(BatchItem), dev->stream);
+      p->begins[merged].allocate(begins.size() * sizeof(uint64_t));
+      p->begins[merged].upload(begins.data(), begins.size() * sizeof(uint64_t), dev->stream);
+      for (size_t g = 0; g < p->groups[merged].size(); g++) {
+        p->groups[merged][g].items  = p->items[merged].as<BatchItem>() + firstOf[g];
+        p->groups[merged][g].begins = p->begins[merged].as<uint64_t>() + firstOf[g];
+      }
     }
     p->numItems = numBatches;
-    p->total    = total;
-    p->items.allocate(items.size() * sizeof(BatchItem));
-    p->items.upload(items.data(), items.size() * sizeof(BatchItem), dev->stream);
-    p->begins.allocate(begins.size() * sizeof(uint64_t));
-    p->begins.upload(begins.data(), begins.size() * sizeof(uint64_t), dev->stream);
-    for (size_t g = 0; g < p->groups.size(); g++) {
-      p->groups[g].items  = p->items.as<BatchItem>() + firstOf[g];
-      p->groups[g].begins = p->begins.as<uint64_t>() + firstOf[g];
-    }
     p->signature = sig.str();
     int cus = 256;
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev->hipDevice);
-    p->launch = autotune ? tune(*p, dev->stream) : BatchLaunch{(uint32_t)cus * 8u, 256u, 1024u};
+    if (autotune < 0 || autotune > 2) throw VklError("sample batch: autotune must be 0, 1 or 2");
+    p->launch = autotune == 1 ? tune(*p, dev->stream)
+                              : BatchLaunch{(uint32_t)cus * 8u, 256u, 1024u, autotune == 2 ? 1u : 0u};
     // Capture the persistent launch in a graph (kernel node only).
     hipStream_t cap;
     checkHip(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking), "hipStreamCreate");
     checkHip(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-    checkHip(launchSampleBatch(p->groups.data(), (uint32_t)p->groups.size(), p->launch, cap), "batch capture");
+    checkHip(launchPlan(*p, p->launch, cap), "batch capture");
     checkHip(hipStreamEndCapture(cap, &p->graph), "hipStreamEndCapture");
     checkHip(hipGraphInstantiate(&p->graphExec, p->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
     (void)hipStreamDestroy(cap);
@@ -247,6 +284,15 @@ OPENVKL_INTERFACE void vklGetSampleBatchPlanLaunch(VKLSampleBatchPlan plan, unsi
   if (blocks) *blocks = p->launch.blocks;
   if (threads) *threads = p->launch.threads;
   if (tile) *tile = p->launch.tile;
+}
+
+OPENVKL_INTERFACE unsigned int vklGetSampleBatchPlanLaunchCount(VKLSampleBatchPlan plan)
+{
+  SampleBatchPlan *p = reinterpret_cast<SampleBatchPlan *>(plan);
+  if (!p) return 0;
+  unsigned int n = 0;
+  for (const BatchGroup &g : p->groups[p->launch.merged]) n += (g.total && g.numItems) ? 1u : 0u;
+  return n;
 }
 
 OPENVKL_INTERFACE void vklReleaseSampleBatchPlan(VKLSampleBatchPlan plan)

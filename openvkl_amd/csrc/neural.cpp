@@ -184,7 +184,7 @@ OPENVKL_INTERFACE float vklNeuralFieldTrain(VKLNeuralField h, unsigned int steps
                                 nf->leafOrigins.as<int32_t>(), nf->numLeaves, leafFraction, batchSize, s,
                                 pts.as<float>(), st),
                "generate");
-      checkHip(launchSample(nf->sampler->view, nf->sampler->attr(nf->attr), nf->attr, batchSize, pts.as<float>(),
+      checkHip(launchSample(nf->sampler->viewCopy(), nf->sampler->attr(nf->attr), nf->attr, batchSize, pts.as<float>(),
                             tgt.as<float>(), st),
                "exact targets");
       checkHip(launchNfTrainStep(nf->model, batchSize, pts.as<float>(), tgt.as<float>(), nf->grads, st), "train");
@@ -227,17 +227,17 @@ OPENVKL_INTERFACE unsigned int vklNeuralFieldFinalize(VKLNeuralField h, float to
   checkHip(launchNfLeafPoints(nf->i2o.as<float>(), nf->leafOrigins.as<int32_t>(), nf->numLeaves, pointsPerLeaf,
                               0xA11CEULL + seed, pts.as<float>(), st),
            "leaf points");
-  checkHip(launchNfInfer(nf->model, nf->sampler->view.grid, nf->attr, nullptr, n, pts.as<float>(), a.as<float>(),
+  checkHip(launchNfInfer(nf->model, nf->sampler->viewCopy().grid, nf->attr, nullptr, n, pts.as<float>(), a.as<float>(),
                          nullptr, nullptr, 0, 0, st),
            "network");
-  checkHip(launchSample(nf->sampler->view, nf->sampler->attr(nf->attr), nf->attr, n, pts.as<float>(), b.as<float>(),
+  checkHip(launchSample(nf->sampler->viewCopy(), nf->sampler->attr(nf->attr), nf->attr, n, pts.as<float>(), b.as<float>(),
                         st),
            "exact");
-  checkHip(launchNfLeafError(nf->sampler->view.grid, a.as<float>(), b.as<float>(), nf->numLeaves, pointsPerLeaf,
+  checkHip(launchNfLeafError(nf->sampler->viewCopy().grid, a.as<float>(), b.as<float>(), nf->numLeaves, pointsPerLeaf,
                              nf->leafErr.as<float>(), st),
            "leaf error");
   zero(cnt, st);
-  checkHip(launchNfApprove(nf->sampler->view.grid, nf->leafOrigins.as<int32_t>(), nf->leafErr.as<float>(),
+  checkHip(launchNfApprove(nf->sampler->viewCopy().grid, nf->leafOrigins.as<int32_t>(), nf->leafErr.as<float>(),
                            nf->numLeaves, nf->attr, tolerance, safety, nf->band ? 1 : 0, nf->leafOk.as<uint8_t>(),
                            cnt.as<uint32_t>(), st),
            "approve");
@@ -275,7 +275,7 @@ OPENVKL_INTERFACE void vklNeuralFieldSampleN(VKLNeuralField h, unsigned int N, c
   uint32_t *list  = useFallback ? reinterpret_cast<uint32_t *>(ws + 4) : nullptr;
   if (useFallback) checkHip(hipMemsetAsync(count, 0, 4, st), "hipMemsetAsync");
   const float *xyz = reinterpret_cast<const float *>(objectCoordinates);
-  checkHip(launchNfInfer(nf->model, nf->sampler->view.grid, nf->attr, nf->leafOk.as<uint8_t>(), N, xyz, samples, list,
+  checkHip(launchNfInfer(nf->model, nf->sampler->viewCopy().grid, nf->attr, nf->leafOk.as<uint8_t>(), N, xyz, samples, list,
                          count, useFallback, useFallback && nf->band ? 1 : 0, st),
            "network");
   uint32_t fb = 0;

@@ -42,10 +42,13 @@ def use_native():
     the host that runs it) instead of the default one: the CPU baseline's
     timed copy. Call before the first lib() use."""
     global _LIB_PATH, _lib
+    native = os.path.join(_HERE, "_build", "liboracle_native.so")
     if _lib is not None:
+        if _LIB_PATH == native:
+            return
         raise RuntimeError("the oracle library is already loaded")
     subprocess.run(["make", "-B", "-C", _HERE, "native"], check=True, stdout=subprocess.DEVNULL)
-    _LIB_PATH = os.path.join(_HERE, "_build", "liboracle_native.so")
+    _LIB_PATH = native
 
 
 def lib():
@@ -81,6 +84,8 @@ def lib():
         L.oracle_iterate_intervals.argtypes = [vp, u32, i32, i32, vp, vp, vp, vp, vp, u32]
         L.oracle_iterate_hits.restype = u32
         L.oracle_iterate_hits.argtypes = [vp, u32, i32, i32, vp, vp, vp, vp, f32, vp, u32]
+        L.oracle_iterate_many.restype = None
+        L.oracle_iterate_many.argtypes = [vp, u32, i32, i32, i32, vp, C.c_uint64, vp, vp, u32, u32, vp, vp]
         L.oracle_tricubic_matrix.argtypes = [vp]
         L.oracle_set_leaf_access.argtypes = [vp, vp]
         L.oracle_dense_new_temporal.restype = vp
@@ -385,6 +390,24 @@ class Grid:
         out, cnt = np.empty((n, 4), np.float32), np.empty(n, np.uint32)
         lib().oracle_hit_render(self.h, attr, len(v), _p(v), n, _p(org), _p(dir), _p(tr), float(time), _p(tf),
                                 len(tf), float(tf_range[0]), float(tf_range[1]), int(max_hits), _p(out), _p(cnt))
+        return out, cnt
+
+    def iterate_many(self, org, dir, max_per_ray, values=None, depth=None, attr=0, threads=0):
+        """Interval (values None; depth default map_depth(0.5)) or hit
+        iteration (iso values; default depth as hits()) of many rays with t
+        range [0, inf) on `threads` host threads (0 = all). Returns (results
+        [R, max_per_ray, 5 or 3], counts [R])."""
+        org = np.ascontiguousarray(org, np.float32).reshape(-1, 3)
+        dir = np.ascontiguousarray(dir, np.float32).reshape(-1, 3)
+        hits = values is not None
+        v = np.ascontiguousarray(values if hits else [], np.float32)
+        if depth is None:
+            depth = -1 if hits else map_depth(0.5)
+        n = len(org)
+        out = np.empty((n, max_per_ray, 3 if hits else 5), np.float32)
+        cnt = np.empty(n, np.uint32)
+        lib().oracle_iterate_many(self.h, attr, depth, int(hits), v.shape[0], _p(v), n, _p(org), _p(dir),
+                                  max_per_ray, threads, _p(cnt), _p(out))
         return out, cnt
 
     def hits(self, org, dir, values, t_range=(0.0, np.inf), attr=0, depth=-1, time=0.0, max_out=1024):

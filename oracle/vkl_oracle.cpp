@@ -850,9 +850,21 @@ R1f leafValueRange(const OracleGrid &g, V3u center, uint32_t attr)
 void buildValueRanges(OracleGrid &g, const LeafInput &in, const std::vector<V3u> &offsets)
 {
   const uint32_t nattr = g.numAttributes;
+  // The leaf ranges read the finished node table only; they are computed on
+  // all host threads, then folded into the levels in leaf order.
+  std::vector<R1f> leafRanges(g.numLeaves * nattr);
+  {
+    const uint64_t nt = std::max(1u, std::min(64u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> pool;
+    for (uint64_t t = 0; t < nt; ++t)
+      pool.emplace_back([&, t] {
+        for (uint64_t idx = t; idx < g.numLeaves; idx += nt)
+          for (uint32_t a = 0; a < nattr; ++a) leafRanges[idx * nattr + a] = leafValueRange(g, offsets[idx], a);
+      });
+    for (auto &th : pool) th.join();
+  }
   for (uint64_t idx = 0; idx < g.numLeaves; ++idx) {
-    std::vector<R1f> lr(nattr);
-    for (uint32_t a = 0; a < nattr; ++a) lr[a] = leafValueRange(g, offsets[idx], a);
+    const R1f *lr = &leafRanges[idx * nattr];
     uint64_t nodeIndex = 0;
     for (uint32_t l = 0; l < in.level[idx]; ++l) {
       OracleLevel &L = g.levels[l];
@@ -1465,6 +1477,8 @@ OracleGrid *oracle_vdb_new_temporal(uint64_t numLeaves, const uint32_t *level, c
                                     (const uint8_t *)packedDense[a] + nd * 512 * es);
         if (packedTile && packedTile[a])
           g->packedTile.emplace_back((const uint8_t *)packedTile[a], (const uint8_t *)packedTile[a] + nt * es);
+        else if (nt > 0)
+          throw std::runtime_error("tile nodes need packed tile data");
         else
           g->packedTile.emplace_back();
       }
@@ -1756,11 +1770,11 @@ Ranges isoRanges(const std::vector<float> &vals)
   return makeRanges((int)vals.size(), pr.data());
 }
 
-uint32_t oracle_iterate_hits(const OracleGrid *g, uint32_t attr, int depth, int numValues, const float *values,
-                             const float org[3], const float dir[3], const float tRange[2], float time,
-                             float *out, uint32_t maxOut)
+// Hit iteration of one ray at the grid's current sample time.
+static uint32_t iterateHits(const OracleGrid *g, uint32_t attr, int depth, int numValues, const float *values,
+                            const float org[3], const float dir[3], const float tRange[2], float *out,
+                            uint32_t maxOut)
 {
-  g->curTime = time;  // temporal volumes sample at the ray time
   if (depth < 0) depth = g->dense ? oracle_map_depth(0.5f) : oracle_map_depth(1.f);
   const std::vector<float> vals(values, values + numValues);
   const Ranges q = isoRanges(vals);
@@ -1776,6 +1790,43 @@ uint32_t oracle_iterate_hits(const OracleGrid *g, uint32_t attr, int depth, int 
     n++;
   }
   return n;
+}
+
+uint32_t oracle_iterate_hits(const OracleGrid *g, uint32_t attr, int depth, int numValues, const float *values,
+                             const float org[3], const float dir[3], const float tRange[2], float time,
+                             float *out, uint32_t maxOut)
+{
+  g->curTime = time;  // temporal volumes sample at the ray time
+  return iterateHits(g, attr, depth, numValues, values, org, dir, tRange, out, maxOut);
+}
+
+// Many rays on `threads` host threads (0 = all): interval or hit iteration
+// per ray as oracle_iterate_intervals / oracle_iterate_hits (no value
+// selector, t range [0, inf)), at most maxPerRay results per ray, written to
+// out[r * maxPerRay * width] (width 5 for intervals, 3 for hits) with the
+// count in counts[r]. For timing the CPU baseline of the batched iterators
+// and checking their output.
+void oracle_iterate_many(const OracleGrid *g, uint32_t attr, int depth, int hits, int numValues,
+                         const float *values, uint64_t numRays, const float *org, const float *dir,
+                         uint32_t maxPerRay, uint32_t threads, uint32_t *counts, float *out)
+{
+  const uint64_t nt = threads ? threads : std::max(1u, std::thread::hardware_concurrency());
+  const float tRange[2] = {0.f, kInf};
+  g->curTime = 0.f;
+  const int hitDepth = depth >= 0 ? depth : (g->dense ? oracle_map_depth(0.5f) : oracle_map_depth(1.f));
+  const uint64_t width = hits ? 3 : 5;
+  std::vector<std::thread> pool;
+  for (uint64_t t = 0; t < nt; ++t)
+    pool.emplace_back([&, t] {
+      for (uint64_t r = t; r < numRays; r += nt) {
+        float *o = out + r * maxPerRay * width;
+        counts[r] = hits ? iterateHits(g, attr, hitDepth, numValues, values, org + 3 * r, dir + 3 * r, tRange, o,
+                                       maxPerRay)
+                         : oracle_iterate_intervals(g, attr, depth, 0, nullptr, org + 3 * r, dir + 3 * r,
+                                                    tRange, o, maxPerRay);
+      }
+    });
+  for (auto &th : pool) th.join();
 }
 
 // ---- Woodcock (delta) tracking --------------------------------------------

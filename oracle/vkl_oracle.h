@@ -147,6 +147,9 @@ uint32_t oracle_iterate_hits(const OracleGrid *g, uint32_t attr, int depth,
                              int numValues, const float *values,
                              const float org[3], const float dir[3], const float tRange[2], float time,
                              float *out, uint32_t maxOut);
+void oracle_iterate_many(const OracleGrid *g, uint32_t attr, int depth, int hits, int numValues,
+                         const float *values, uint64_t numRays, const float *org, const float *dir,
+                         uint32_t maxPerRay, uint32_t threads, uint32_t *counts, float *out);
 
 /* The Lekien-Marsden 64x64 tricubic matrix (row = coefficient index
  * 16i+4j+k, column = constraint index), derived at load time. */

@@ -201,3 +201,23 @@ def dims_for_64_32bit_addressing(byte_stride):
 def test_64_32bit_addressing_dimensions(stride, dim):
     assert dims_for_64_32bit_addressing(stride) == dim
     assert dim ** 3 * stride > 1 << 31 >= (dim - 1) ** 3 * stride
+
+
+def test_iterate_many_equals_per_ray_calls():
+    """The threaded many-ray entry point (the iterators' CPU baseline) gives
+    exactly the per-ray interval and hit iteration results."""
+    origins, leaves = V.sparse_shell_leaves(res=64)
+    n = len(origins)
+    g = O.Grid.vdb(np.full(n, 3, np.uint32), origins, np.full(n, 1, np.uint32), packed_dense=[leaves.reshape(-1)])
+    rng = np.random.default_rng(5)
+    org = (rng.random((64, 3)) * 80 - 8).astype(np.float32)
+    d = rng.standard_normal((64, 3)).astype(np.float32)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    out, cnt = g.iterate_many(org, d, 64, threads=3)
+    hout, hcnt = g.iterate_many(org, d, 8, values=[0.5], threads=3)
+    assert cnt.sum() > 0 and hcnt.sum() > 0
+    for r in range(len(org)):
+        iv = g.intervals(org[r], d[r], max_out=64)
+        np.testing.assert_array_equal(out[r, :cnt[r]].view(np.uint32), iv.view(np.uint32))
+        h = g.hits(org[r], d[r], [0.5], max_out=8)
+        np.testing.assert_array_equal(hout[r, :hcnt[r]].view(np.uint32), h.view(np.uint32))
