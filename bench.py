@@ -470,6 +470,20 @@ def variant_iterators(torch, vkl, dev, res, steps, warmup, rays_log2=21, cpu=Fal
     acc = torch.empty(r, device="cuda")
     ms_m = timed(torch, lambda: s.ray_march(org, d, tr, march, out=acc), max(1, steps // 2), 1)
     log("  ray march done")
+    # ragged ray batches of four contexts (two interval depths, two iso sets)
+    # in one plan, against one batch call per context
+    ctxs = [ictx, s.interval_context(hint=1.0), hctx, s.hit_context([0.25])]
+    sizes, caps = [1 << 20, 1 << 18, 1 << 19, 1 << 17], [256, 1024, 16, 16]
+    batches, off = [], 0
+    for c, k, cap in zip(ctxs, sizes, caps):
+        batches.append((c, org[off:off + k], d[off:off + k], cap))
+        off += k
+    plan = vkl.RayBatchPlan(dev, batches)
+    ms_plan = timed(torch, plan.execute, max(1, steps // 2), 1)
+    ms_calls = timed(torch, lambda: [c.batch(o, dd, max_per_ray=cap) for c, o, dd, cap in batches],
+                     max(1, steps // 2), 1)
+    plan.release()
+    log("  ray batch plan done")
     cpu_line = None
     if cpu:
         log("  oracle iterators on the host ...")
@@ -482,6 +496,10 @@ def variant_iterators(torch, vkl, dev, res, steps, warmup, rays_log2=21, cpu=Fal
             "intervals_per_ray": round(float(ci.mean()), 2), "max_intervals_per_ray": int(ci.max()),
             "hit_mrays_s": round(r / ms_h / 1e3, 2), "hit_ms": round(ms_h, 3),
             "hits_per_ray": round(float(ch.mean()), 3),
+            "ray_batch_plan": {"batches": len(batches), "rays": off, "plan_ms": round(ms_plan, 3),
+                               "one_call_per_batch_ms": round(ms_calls, 3),
+                               "what": "intervals at hint 0.5 and 1.0, hits at iso 0.5 and 0.25; 2^20, 2^18, "
+                                       "2^19, 2^17 rays; one graph with a branch per batch"},
             "raymarch_steps_per_ray": march, "raymarch_mrays_s": round(r / ms_m / 1e3, 2),
             "raymarch_msamples_s": round(r * march / ms_m / 1e3, 1)}
 

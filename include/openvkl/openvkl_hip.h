@@ -180,6 +180,30 @@ OPENVKL_INTERFACE void vklGetSampleBatchPlanLaunch(VKLSampleBatchPlan plan, unsi
 OPENVKL_INTERFACE unsigned int vklGetSampleBatchPlanLaunchCount(VKLSampleBatchPlan plan);
 OPENVKL_INTERFACE void vklReleaseSampleBatchPlan(VKLSampleBatchPlan plan);
 
+/* ---- ray batch plans ---------------------------------------------------------
+ * Interval and hit iteration of numBatches (iterator context, rays) pairs in
+ * one graph replay: batch i iterates numRays[i] rays of contexts[i] (an
+ * interval or a hit iterator context; the batches may use different volumes)
+ * with the contract of vklIterateIntervalsBatch / vklIterateHitsBatch:
+ * results[i] receives VKLInterval or VKLHit records, maxPerRay[i] per ray,
+ * counts[i] the number per ray; tRanges (or tRanges[i]) NULL means [0, inf),
+ * times (or times[i]) NULL means time 0 (hit contexts). Every array must be
+ * device memory and stay valid while the plan is used. The batches run on
+ * parallel branches of one hipGraph; the plan keeps the context state it was
+ * built with and retains the contexts. */
+typedef struct VKLRayBatchPlan_ *VKLRayBatchPlan;
+OPENVKL_INTERFACE VKLRayBatchPlan vklNewRayBatchPlan(unsigned int numBatches, const VKLObject *contexts,
+                                                     const unsigned int *numRays, const vkl_vec3f *const *origins,
+                                                     const vkl_vec3f *const *directions,
+                                                     const vkl_range1f *const *tRanges, const float *const *times,
+                                                     void *const *results, const unsigned int *maxPerRay,
+                                                     unsigned int *const *counts);
+/* Replay the plan's graph on `stream` (hipStream_t; NULL = null stream). */
+OPENVKL_INTERFACE void vklExecuteRayBatchPlan(VKLRayBatchPlan plan, void *stream);
+/* Kernel launches one execution makes (one per non-empty batch). */
+OPENVKL_INTERFACE unsigned int vklGetRayBatchPlanLaunchCount(VKLRayBatchPlan plan);
+OPENVKL_INTERFACE void vklReleaseRayBatchPlan(VKLRayBatchPlan plan);
+
 /* ---- neural-field sampler ---------------------------------------------------
  * A multiresolution hash-grid (16 levels x 2 binary16 features, 2^19 entries
  * per level) + 32-64-64-1 ReLU MLP that stands in for a vdb volume's exact

@@ -162,6 +162,10 @@ def lib():
         "vklGetSampleBatchPlanLaunch": (None, [vp, C.POINTER(C.c_uint), C.POINTER(C.c_uint), C.POINTER(C.c_uint)]),
         "vklGetSampleBatchPlanLaunchCount": (C.c_uint, [vp]),
         "vklReleaseSampleBatchPlan": (None, [vp]),
+        "vklNewRayBatchPlan": (vp, [C.c_uint, vp, vp, vp, vp, vp, vp, vp, vp, vp]),
+        "vklExecuteRayBatchPlan": (None, [vp, vp]),
+        "vklGetRayBatchPlanLaunchCount": (C.c_uint, [vp]),
+        "vklReleaseRayBatchPlan": (None, [vp]),
         "vklHipGetVolumeLevel": (i32, [O, C.c_uint, C.POINTER(C.c_uint64), C.POINTER(C.POINTER(C.c_uint64)),
                                        C.POINTER(C.POINTER(range1f))]),
         "vklNewNeuralField": (vp, [C.POINTER(O), C.c_uint, C.c_uint, C.c_uint]),
@@ -264,13 +268,22 @@ class Device:
             self.h = None
 
     # -- object creation ------------------------------------------------------------------
-    def data(self, array, dtype=None, shared=False):
-        """vklNewData from a numpy array (copied) or a torch tensor (shared)."""
+    def data(self, array, dtype=None, shared=False, stride_factor=None):
+        """vklNewData from a numpy array (copied unless shared) or a torch
+        tensor (always shared). stride_factor f: the items are placed f
+        elements apart in a padded buffer and passed with byteStride f x the
+        element size (f = 0 passes byteStride 0, the natural stride)."""
         if isinstance(array, np.ndarray):
             a = np.ascontiguousarray(array)
             t = dtype if dtype is not None else NP_TO_VKL[a.dtype]
             n = a.size if t not in (VKL_VEC3I, VKL_VEC3F, VKL_BOX1F) else a.size // (3 if t != VKL_BOX1F else 2)
-            h = lib().vklNewData(self.h, n, t, a.ctypes.data, VKL_DATA_SHARED_BUFFER if shared else 0, 0)
+            stride = 0
+            if stride_factor is not None:
+                f = max(int(stride_factor), 1)
+                buf = np.zeros(a.size * f, a.dtype)
+                buf[::f] = a.reshape(-1)
+                a, stride = buf, int(stride_factor) * a.itemsize
+            h = lib().vklNewData(self.h, n, t, a.ctypes.data, VKL_DATA_SHARED_BUFFER if shared else 0, stride)
             obj = Handle(self, h, keep=a if shared else None)
         else:  # torch tensor, always shared (device memory stays in place)
             import torch
@@ -278,7 +291,13 @@ class Device:
             tmap = {torch.float32: VKL_FLOAT, torch.float16: VKL_HALF, torch.uint8: VKL_UCHAR,
                     torch.int16: VKL_SHORT, torch.float64: VKL_DOUBLE, torch.int32: VKL_INT}
             t = dtype if dtype is not None else tmap[array.dtype]
-            h = lib().vklNewData(self.h, array.numel(), t, array.data_ptr(), VKL_DATA_SHARED_BUFFER, 0)
+            n, stride = array.numel(), 0
+            if stride_factor is not None:
+                f = max(int(stride_factor), 1)
+                buf = torch.zeros(n * f, dtype=array.dtype, device=array.device)
+                buf[::f] = array.reshape(-1)
+                array, stride = buf, int(stride_factor) * buf.element_size()
+            h = lib().vklNewData(self.h, n, t, array.data_ptr(), VKL_DATA_SHARED_BUFFER, stride)
             obj = Handle(self, h, keep=array)
         self.check()
         return obj
@@ -297,11 +316,13 @@ class Device:
 
     def structured_regular(self, voxels, origin=(0, 0, 0), spacing=(1, 1, 1), filter=None, cell_centered=None,
                            background=None, index_origin=None, index_to_object=None, brick_shift=None,
-                           corner_packed=None, dims=None, structured_steps=None, tu_indices=None, tu_times=None):
+                           corner_packed=None, dims=None, structured_steps=None, tu_indices=None, tu_times=None,
+                           data_shared=False, data_stride_factor=None):
         """voxels: array [nz, ny, nx] (x fastest) or a list of them. Motion
         blur: pass dims (nx, ny, nz) and 1-D voxels with structured_steps
         values per voxel, or tu_indices / tu_times (temporallyUnstructured*;
-        uint32 or uint64 indices)."""
+        uint32 or uint64 indices). data_shared / data_stride_factor: how the
+        voxel VKLData are made (see data())."""
         arrs = voxels if isinstance(voxels, (list, tuple)) else [voxels]
         v = self.new_volume("structuredRegular")
         nx, ny, nz = dims if dims is not None else arrs[0].shape[::-1]
@@ -319,7 +340,8 @@ class Device:
             d.release()
         v.set_vec3f("gridOrigin", *origin)
         v.set_vec3f("gridSpacing", *spacing)
-        datas = [self.data(a) if isinstance(a, np.ndarray) else self.data(a.reshape(-1)) for a in arrs]
+        datas = [self.data(a if isinstance(a, np.ndarray) else a.reshape(-1), shared=data_shared,
+                           stride_factor=data_stride_factor) for a in arrs]
         # Device tensors are shared (VKL_DATA_SHARED_BUFFER): the volume reads
         # them in place, so it keeps them alive.
         v._shared = [d._keep for d in datas if d._keep is not None]
@@ -352,11 +374,13 @@ class Device:
 
     def vdb(self, level, origin, fmt, packed_dense=None, packed_tile=None, leaf_data=None, background=None,
             index_to_object=None, filter=None, corner_packed=None, max_sampling_depth=None, temporal_format=None,
-            steps=None, tu_indices=None, tu_times=None, leaf_index_grid=None):
+            steps=None, tu_indices=None, tu_times=None, leaf_index_grid=None, data_shared=False,
+            data_stride_factor=None):
         """Sparse vdb volume from node arrays (packed or per-leaf data).
         Motion blur (per-leaf data only): temporal_format per node, steps per
         node (node.temporallyStructuredNumTimesteps), tu_indices / tu_times
-        per node (arrays, None for other nodes)."""
+        per node (arrays, None for other nodes). data_shared /
+        data_stride_factor: how the voxel VKLData are made (see data())."""
         v = self.new_volume("vdb")
         tmp = []
 
@@ -365,12 +389,17 @@ class Device:
             tmp.append(h)
             return h
 
+        def vox(a):
+            h = self.data(a if isinstance(a, np.ndarray) else a.reshape(-1), shared=data_shared,
+                          stride_factor=data_stride_factor)
+            tmp.append(h)
+            return h
+
         v.set_data("node.level", d(np.ascontiguousarray(level, np.uint32)))
         v.set_data("node.origin", d(np.ascontiguousarray(origin, np.int32).reshape(-1), VKL_VEC3I))
         v.set_data("node.format", d(np.ascontiguousarray(fmt, np.uint32)))
         if packed_dense is not None:
-            pd = self.data_of_data([d(a) if isinstance(a, np.ndarray) else d(a.reshape(-1)) for a in packed_dense])
-            v._shared = [h._keep for h in tmp if h._keep is not None]
+            pd = self.data_of_data([vox(a) for a in packed_dense])
             tmp.append(pd)
             v.set_data("nodesPackedDense", pd)
             if packed_tile is not None:
@@ -381,9 +410,9 @@ class Device:
             per_leaf = []
             for leaf in leaf_data:
                 if len(leaf) == 1:
-                    per_leaf.append(d(leaf[0]))
+                    per_leaf.append(vox(leaf[0]))
                 else:
-                    x = self.data_of_data([d(a) for a in leaf])
+                    x = self.data_of_data([vox(a) for a in leaf])
                     tmp.append(x)
                     per_leaf.append(x)
             nd = self.data_of_data(per_leaf)
@@ -419,6 +448,9 @@ class Device:
             dd = self.data_of_data(items)
             tmp.append(dd)
             v.set_data(name, dd)
+        # Shared buffers (device tensors, data_shared host arrays) are read in
+        # place by the volume: it keeps them alive.
+        v._shared = [h._keep for h in tmp if h._keep is not None]
         v.commit()
         for h in tmp:
             h.release()
@@ -816,6 +848,55 @@ class SampleBatchPlan:
     def release(self):
         if self.h:
             lib().vklReleaseSampleBatchPlan(self.h)
+            self.h = None
+
+
+class RayBatchPlan:
+    """vklNewRayBatchPlan: interval / hit iteration of many (context, device
+    rays) batches, one graph replay. batches: list of (context, org, dir,
+    max_per_ray) or (context, org, dir, max_per_ray, t_ranges, times) with
+    torch device tensors. Allocates the outputs: self.results[i] [R, max, 5 or
+    3] float32, self.counts[i] [R] int32."""
+
+    def __init__(self, device, batches):
+        import torch
+
+        n = len(batches)
+        self.device = device
+        self.results, self.counts, keep = [], [], []
+        ctx = (VKLObject * n)(*[b[0].h for b in batches])
+        nr = (C.c_uint * n)(*[_n_points(b[1]) for b in batches])
+        mx = (C.c_uint * n)(*[int(b[3]) for b in batches])
+        org, dr, tr, tm, res, cnt = ([None] * n for _ in range(6))
+        for i, b in enumerate(batches):
+            r = _n_points(b[1])
+            width = 3 if b[0].hit else 5
+            self.results.append(torch.empty((r, int(b[3]), width), dtype=torch.float32, device=b[1].device))
+            self.counts.append(torch.empty(r, dtype=torch.int32, device=b[1].device))
+            org[i], dr[i] = _ptr(b[1]), _ptr(b[2])
+            tr[i] = _ptr(b[4]) if len(b) > 4 else None
+            tm[i] = _ptr(b[5]) if len(b) > 5 else None
+            res[i], cnt[i] = _ptr(self.results[-1]), _ptr(self.counts[-1])
+            keep.append(b)
+        arrs = [(C.c_void_p * n)(*a) for a in (org, dr, tr, tm, res, cnt)]
+        self._keep = (keep, arrs, ctx, nr, mx)
+        self.h = lib().vklNewRayBatchPlan(n, C.addressof(ctx), C.addressof(nr), C.addressof(arrs[0]),
+                                          C.addressof(arrs[1]), C.addressof(arrs[2]), C.addressof(arrs[3]),
+                                          C.addressof(arrs[4]), C.addressof(mx), C.addressof(arrs[5]))
+        device.check()
+        if not self.h:
+            raise RuntimeError("vklNewRayBatchPlan failed")
+
+    def launch_count(self):
+        return int(lib().vklGetRayBatchPlanLaunchCount(self.h))
+
+    def execute(self, stream=None):
+        lib().vklExecuteRayBatchPlan(self.h, stream)
+        self.device.check()
+
+    def release(self):
+        if self.h:
+            lib().vklReleaseRayBatchPlan(self.h)
             self.h = None
 
 

@@ -1432,8 +1432,8 @@ OPENVKL_INTERFACE void vklIterateIntervalsBatch(const VKLIntervalIteratorContext
   Staged t(tRanges, tRanges ? numRays * 8ull : 0, st, true);
   Staged out(intervals, (uint64_t)numRays * maxIntervalsPerRay * sizeof(VKLInterval), st, false);
   Staged cnt(numIntervals, numRays * 4ull, st, false);
-  (void)ctx->refreshSampler();
-  checkHip(launchIntervalBatch(ctx->deviceView.as<IteratorContextView>(), numRays, o.as<float>(), d.as<float>(), t.as<float>(),
+  const IteratorContextView hv = ctx->refreshSampler();
+  checkHip(launchIntervalBatch(hv.maxIteratorDepth, ctx->deviceView.as<IteratorContextView>(), numRays, o.as<float>(), d.as<float>(), t.as<float>(),
                                out.as<IntervalOut>(), maxIntervalsPerRay, cnt.as<uint32_t>(), st),
            "interval batch");
   out.copyOut();

@@ -125,7 +125,7 @@ hipError_t launchHitNext(const IteratorContextView &hostView, const IteratorCont
 // Batched iteration: one ray per thread; ray r writes up to maxOut results at
 // out[r*maxOut], count[r] = number written. org / dir: 3 floats per ray;
 // tRanges: 2 floats per ray (null = [0, inf)); times may be null.
-hipError_t launchIntervalBatch(const IteratorContextView *c, uint32_t numRays, const float *org, const float *dir,
+hipError_t launchIntervalBatch(uint32_t depth, const IteratorContextView *c, uint32_t numRays, const float *org, const float *dir,
                                const float *tRanges, IntervalOut *out, uint32_t maxOut, uint32_t *count,
                                hipStream_t stream);
 // Hit batches run a persistent schedule: `lanes` (batchLaneCount) lanes take

@@ -219,8 +219,14 @@ hipError_t launchHitNext(const IteratorContextView &hostView, const IteratorCont
 
 // Interval batches keep one ray per lane: a persistent schedule (as the hit
 // batches use) measured 12 % slower here, where ray lengths vary less than
-// the cost of the schedule's per-step bookkeeping.
-__global__ __launch_bounds__(kIterBlock) void kIntervalBatch(const IteratorContextView *__restrict__ cp, uint32_t numRays,
+// the cost of the schedule's per-step bookkeeping. The kernel is specialised
+// on the context's iterator depth, so the DDA keeps only the levels it walks.
+// The iteration is bound by dependent-load latency, so occupancy pays: at
+// depth 2 (82 VGPRs, 5 waves per SIMD) asking for 6 waves costs a 12-byte
+// spill and measured 11 % faster on the 4096^3 vdb.
+template <int kDepth>
+__global__ __launch_bounds__(kIterBlock) __attribute__((amdgpu_waves_per_eu(kDepth <= 2 ? 6 : 4)))
+void kIntervalBatch(const IteratorContextView *__restrict__ cp, uint32_t numRays,
                                                               const float *__restrict__ org,
                                                               const float *__restrict__ dir,
                                                               const float *__restrict__ tRanges,
@@ -231,12 +237,12 @@ __global__ __launch_bounds__(kIterBlock) void kIntervalBatch(const IteratorConte
   if (r >= numRays) return;
   const IteratorContextView &c = *cp;
   IntervalState st;
-  intervalInit(c, st, F3{org[3 * r], org[3 * r + 1], org[3 * r + 2]}, F3{dir[3 * r], dir[3 * r + 1], dir[3 * r + 2]},
-               rayRange(tRanges, r));
+  intervalInit<kDepth>(c, st, F3{org[3 * r], org[3 * r + 1], org[3 * r + 2]},
+                       F3{dir[3 * r], dir[3 * r + 1], dir[3 * r + 2]}, rayRange(tRanges, r));
   uint32_t n = 0;
   IntervalOut o;
   IntervalOut *dst = out + (uint64_t)r * maxOut;
-  while (n < maxOut && intervalNext(c, st, o)) dst[n++] = o;
+  while (n < maxOut && intervalNext<false, kDepth>(c, st, o)) dst[n++] = o;
   count[r] = n;
 }
 
@@ -250,13 +256,18 @@ hipError_t launchSetWork(uint32_t *work, uint32_t value, hipStream_t st)
 
 uint32_t batchLaneCount(uint32_t numRays, int cus) { return batchLanes(numRays, cus); }
 
-hipError_t launchIntervalBatch(const IteratorContextView *c, uint32_t numRays, const float *org, const float *dir,
-                               const float *tRanges, IntervalOut *out, uint32_t maxOut, uint32_t *count,
-                               hipStream_t st)
+hipError_t launchIntervalBatch(uint32_t depth, const IteratorContextView *c, uint32_t numRays, const float *org,
+                               const float *dir, const float *tRanges, IntervalOut *out, uint32_t maxOut,
+                               uint32_t *count, hipStream_t st)
 {
   if (numRays == 0) return hipSuccess;
-  hipLaunchKernelGGL(kIntervalBatch, dim3((numRays + kIterBlock - 1) / kIterBlock), dim3(kIterBlock), 0, st, c,
-                     numRays, org, dir, tRanges, out, maxOut, count);
+  const dim3 grid((numRays + kIterBlock - 1) / kIterBlock), block(kIterBlock);
+  switch (depth) {
+  case 0: hipLaunchKernelGGL(kIntervalBatch<0>, grid, block, 0, st, c, numRays, org, dir, tRanges, out, maxOut, count); break;
+  case 1: hipLaunchKernelGGL(kIntervalBatch<1>, grid, block, 0, st, c, numRays, org, dir, tRanges, out, maxOut, count); break;
+  case 2: hipLaunchKernelGGL(kIntervalBatch<2>, grid, block, 0, st, c, numRays, org, dir, tRanges, out, maxOut, count); break;
+  default: hipLaunchKernelGGL(kIntervalBatch<3>, grid, block, 0, st, c, numRays, org, dir, tRanges, out, maxOut, count); break;
+  }
   return hipGetLastError();
 }
 

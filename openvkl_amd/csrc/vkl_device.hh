@@ -991,14 +991,17 @@ VKL_DEV void hddaStepAt(DdaState &d)
   hddaParents<(L >= 3 ? 3 : L) - 1>(d);
 }
 
+// descLevel never exceeds the DDA's deepest level (queries descend no
+// deeper), so with a compile-time depth the deeper cases are not generated.
 template <int kDepth>
 VKL_DEV void hddaStep(DdaState &d, uint32_t descLevel)
 {
+  constexpr int kMax = kDepth >= 0 ? kDepth : 3;
   switch (descLevel) {
   case 0: hddaStepAt<kDepth, 0>(d); break;
-  case 1: hddaStepAt<kDepth, 1>(d); break;
-  case 2: hddaStepAt<kDepth, 2>(d); break;
-  default: hddaStepAt<kDepth, 3>(d); break;
+  case 1: hddaStepAt<kDepth, (kMax < 1 ? kMax : 1)>(d); break;
+  case 2: hddaStepAt<kDepth, (kMax < 2 ? kMax : 2)>(d); break;
+  default: hddaStepAt<kDepth, kMax>(d); break;
   }
 }
 
