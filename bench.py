@@ -92,18 +92,27 @@ def variant_layouts(torch, vkl, dev, vol_t, pts, out, steps, warmup, alg_bytes):
 def variant_config2(torch, vkl, dev, vol_t, pts, steps, warmup, U):
     """The rest of the 512^3 / 2^26-point matrix on corner-packed volumes:
     half and uint8 (raw codes; the scale is applied by the caller) linear,
-    float32 cubic, float32 linear gradient."""
+    float32 cubic, float32 linear gradient; and half / uint8 linear on their
+    raw x-fastest arrays (256 / 128 MiB, within the Infinity Cache) and in
+    128-byte bricks (direct gathers, binned path off)."""
     npts = pts.shape[0]
     out = torch.empty(npts, device="cuda")
     gout = torch.empty((npts, 3), device="cuda")
     res = {}
-    cases = (("half_linear", vol_t.half(), vkl.VKL_FILTER_LINEAR, False, 2),
-             ("uint8_linear", (vol_t * 100.0 + 100.0).clamp(0, 255).to(torch.uint8), vkl.VKL_FILTER_LINEAR, False, 1),
-             ("float32_cubic", vol_t, vkl.VKL_FILTER_CUBIC, False, 4),
-             ("float32_linear_gradient", vol_t, vkl.VKL_FILTER_LINEAR, True, 4))
-    for name, data, filt, grad, es in cases:
-        v = dev.structured_regular(data.contiguous(), corner_packed=True)
-        sm = v.sampler(filter=filt)
+    half = vol_t.half()
+    u8 = (vol_t * 100.0 + 100.0).clamp(0, 255).to(torch.uint8)
+    packed = {"corner_packed": True}
+    cases = (("half_linear", half, vkl.VKL_FILTER_LINEAR, False, 2, packed),
+             ("uint8_linear", u8, vkl.VKL_FILTER_LINEAR, False, 1, packed),
+             ("float32_cubic", vol_t, vkl.VKL_FILTER_CUBIC, False, 4, packed),
+             ("float32_linear_gradient", vol_t, vkl.VKL_FILTER_LINEAR, True, 4, packed),
+             ("half_linear_x_fastest", half, vkl.VKL_FILTER_LINEAR, False, 2, {}),
+             ("half_linear_bricks_4x4x4", half, vkl.VKL_FILTER_LINEAR, False, 2, {"brick_shift": (2, 2, 2)}),
+             ("uint8_linear_x_fastest", u8, vkl.VKL_FILTER_LINEAR, False, 1, {}),
+             ("uint8_linear_bricks_8x4x4", u8, vkl.VKL_FILTER_LINEAR, False, 1, {"brick_shift": (3, 2, 2)}))
+    for name, data, filt, grad, es, layout in cases:
+        v = dev.structured_regular(data.contiguous(), **layout)
+        sm = v.sampler(filter=filt, binned=False)
         if grad:
             ms = timed(torch, lambda: sm.gradient(pts, out=gout), steps, warmup)
             alg = npts * (12 + 12) + es * U
@@ -117,6 +126,7 @@ def variant_config2(torch, vkl, dev, vol_t, pts, steps, warmup, U):
         sm.release()
         v.release()
         del data
+    del half, u8
     return res
 
 

@@ -92,8 +92,13 @@ __global__ void kHitNext(const IteratorContextView *__restrict__ cp, AttrView a,
   *result = r ? 1 : 0;
 }
 
+// Hit iteration waits on dependent loads, so occupancy pays: asking for 4
+// waves per SIMD (128 VGPRs instead of ~155 at 3 waves) costs a ~30-byte
+// spill and measured 13 % faster on the 4096^3 vdb. Cubic filters keep the
+// compiler's choice.
 template <typename Src, int kFilter>
-__global__ __launch_bounds__(kIterBlock) void kHitBatch(const IteratorContextView *__restrict__ cp, AttrView a,
+__global__ __launch_bounds__(kIterBlock) __attribute__((amdgpu_waves_per_eu(kFilter == kFilterCubic ? 1 : 4)))
+void kHitBatch(const IteratorContextView *__restrict__ cp, AttrView a,
                                                          uint32_t numRays, const float *__restrict__ org,
                                                          const float *__restrict__ dir,
                                                          const float *__restrict__ tRanges,
