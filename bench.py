@@ -92,9 +92,9 @@ def variant_layouts(torch, vkl, dev, vol_t, pts, out, steps, warmup, alg_bytes):
 def variant_config2(torch, vkl, dev, vol_t, pts, steps, warmup, U):
     """The rest of the 512^3 / 2^26-point matrix on corner-packed volumes:
     half and uint8 (raw codes; the scale is applied by the caller) linear,
-    float32 cubic, float32 linear gradient; and half / uint8 linear on their
-    raw x-fastest arrays (256 / 128 MiB, within the Infinity Cache) and in
-    128-byte bricks (direct gathers, binned path off)."""
+    float32 cubic, float32 linear gradient; half / uint8 linear on their raw
+    x-fastest arrays (256 / 128 MiB, within the Infinity Cache) and in
+    128-byte bricks (direct gathers, binned path off); half cubic."""
     npts = pts.shape[0]
     out = torch.empty(npts, device="cuda")
     gout = torch.empty((npts, 3), device="cuda")
@@ -109,7 +109,8 @@ def variant_config2(torch, vkl, dev, vol_t, pts, steps, warmup, U):
              ("half_linear_x_fastest", half, vkl.VKL_FILTER_LINEAR, False, 2, {}),
              ("half_linear_bricks_4x4x4", half, vkl.VKL_FILTER_LINEAR, False, 2, {"brick_shift": (2, 2, 2)}),
              ("uint8_linear_x_fastest", u8, vkl.VKL_FILTER_LINEAR, False, 1, {}),
-             ("uint8_linear_bricks_8x4x4", u8, vkl.VKL_FILTER_LINEAR, False, 1, {"brick_shift": (3, 2, 2)}))
+             ("uint8_linear_bricks_8x4x4", u8, vkl.VKL_FILTER_LINEAR, False, 1, {"brick_shift": (3, 2, 2)}),
+             ("half_cubic", half, vkl.VKL_FILTER_CUBIC, False, 2, packed))
     for name, data, filt, grad, es, layout in cases:
         v = dev.structured_regular(data.contiguous(), **layout)
         sm = v.sampler(filter=filt, binned=False)
@@ -467,6 +468,12 @@ def variant_iterators(torch, vkl, dev, res, steps, warmup, rays_log2=21, cpu=Fal
 
     ms_i = timed(torch, run_i, max(1, steps // 2), 1)
     ms_h = timed(torch, run_h, max(1, steps // 2), 1)
+    # the same rays at the finest interval resolution (hint 1: depth 3)
+    ictx1 = s.interval_context(hint=1.0)
+    cnt1 = []
+    ms_i1 = timed(torch, lambda: cnt1.append(ictx1.batch(org, d, max_per_ray=1024)[1]), max(1, steps // 2), 1)
+    per_ray1 = float(cnt1[-1].double().mean())
+    del cnt1
     ci, ch = res_i["cnt"].double(), res_h["cnt"].double()
     r = org.shape[0]
     # fused ray march over the same rays: 256 samples between box entry (the
@@ -482,7 +489,7 @@ def variant_iterators(torch, vkl, dev, res, steps, warmup, rays_log2=21, cpu=Fal
     log("  ray march done")
     # ragged ray batches of four contexts (two interval depths, two iso sets)
     # in one plan, against one batch call per context
-    ctxs = [ictx, s.interval_context(hint=1.0), hctx, s.hit_context([0.25])]
+    ctxs = [ictx, ictx1, hctx, s.hit_context([0.25])]
     sizes, caps = [1 << 20, 1 << 18, 1 << 19, 1 << 17], [256, 1024, 16, 16]
     batches, off = [], 0
     for c, k, cap in zip(ctxs, sizes, caps):
@@ -504,6 +511,8 @@ def variant_iterators(torch, vkl, dev, res, steps, warmup, rays_log2=21, cpu=Fal
             "cpu_baseline": cpu_line,
             "rays": r, "interval_mrays_s": round(r / ms_i / 1e3, 2), "interval_ms": round(ms_i, 3),
             "intervals_per_ray": round(float(ci.mean()), 2), "max_intervals_per_ray": int(ci.max()),
+            "interval_hint1_mrays_s": round(r / ms_i1 / 1e3, 2), "interval_hint1_ms": round(ms_i1, 3),
+            "intervals_per_ray_hint1": round(per_ray1, 2),
             "hit_mrays_s": round(r / ms_h / 1e3, 2), "hit_ms": round(ms_h, 3),
             "hits_per_ray": round(float(ch.mean()), 3),
             "ray_batch_plan": {"batches": len(batches), "rays": off, "plan_ms": round(ms_plan, 3),

@@ -223,9 +223,10 @@ hipError_t launchHitNext(const IteratorContextView &hostView, const IteratorCont
 // on the context's iterator depth, so the DDA keeps only the levels it walks.
 // The iteration is bound by dependent-load latency, so occupancy pays: at
 // depth 2 (82 VGPRs, 5 waves per SIMD) asking for 6 waves costs a 12-byte
-// spill and measured 11 % faster on the 4096^3 vdb.
+// spill and measured 11 % faster on the 4096^3 vdb; at depth 3 (100 VGPRs,
+// 4 waves) 5 waves cost a 20-byte spill and measured 6 % faster.
 template <int kDepth>
-__global__ __launch_bounds__(kIterBlock) __attribute__((amdgpu_waves_per_eu(kDepth <= 2 ? 6 : 4)))
+__global__ __launch_bounds__(kIterBlock) __attribute__((amdgpu_waves_per_eu(kDepth <= 2 ? 6 : 5)))
 void kIntervalBatch(const IteratorContextView *__restrict__ cp, uint32_t numRays,
                                                               const float *__restrict__ org,
                                                               const float *__restrict__ dir,

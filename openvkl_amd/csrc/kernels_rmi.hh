@@ -41,8 +41,12 @@ VKL_DEV void tfSampleRgba(const float *__restrict__ tf, uint32_t n, float lo, fl
 VKL_DEV float minf_(float a, float b) { return a < b ? a : b; }
 
 // rgba[4r..4r+3] = composited colour and accumulated opacity of ray r.
+// 5 waves per SIMD (96 VGPRs instead of 126 at 4 waves, a spill of 128 bytes)
+// measured 4 % faster on a 1080p frame of the 512^3 volume; cubic filters
+// keep the compiler's choice.
 template <typename Src, int kFilter>
-__global__ __launch_bounds__(kIterBlock) void kRayMarchIntervals(const IteratorContextView *__restrict__ cp,
+__global__ __launch_bounds__(kIterBlock) __attribute__((amdgpu_waves_per_eu(kFilter == kFilterCubic ? 1 : 5)))
+void kRayMarchIntervals(const IteratorContextView *__restrict__ cp,
                                                                   AttrView a, RayMarchIntervalsArgs m)
 {
   const uint32_t r = xcdBandBlock(blockIdx.x, gridDim.x) * kIterBlock + threadIdx.x;
