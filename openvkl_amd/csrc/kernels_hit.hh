@@ -78,7 +78,7 @@ inline uint32_t batchLanes(uint32_t numRays, int cus)
 // a PointSampler specialised on the volume's source type and filter (chosen
 // on the host from the same context); temporal sources sample at the ray time.
 template <typename Src, int kFilter>
-__global__ void kHitNext(const IteratorContextView *__restrict__ cp, AttrView a, HitState *s, HitOut *out,
+__global__ __launch_bounds__(1) void kHitNext(const IteratorContextView *__restrict__ cp, AttrView a, HitState *s, HitOut *out,
                          int *result)
 {
   const IteratorContextView &c = *cp;

@@ -158,13 +158,13 @@ hipError_t launchLeafRanges(const GridView &g, uint32_t attr, uint64_t numLeaves
 // The iterator kernels read their context through a pointer to device
 // memory: a context passed by value is copied to scratch memory as soon as
 // any field is indexed at run time.
-__global__ void kIntervalInit(const IteratorContextView *__restrict__ cp, IntervalState *s, const float *ray)
+__global__ __launch_bounds__(1) void kIntervalInit(const IteratorContextView *__restrict__ cp, IntervalState *s, const float *ray)
 {
   const IteratorContextView &c = *cp;
   intervalInit(c, *s, F3{ray[0], ray[1], ray[2]}, F3{ray[3], ray[4], ray[5]}, Range1f{ray[6], ray[7]});
 }
 
-__global__ void kIntervalNext(const IteratorContextView *__restrict__ cp, IntervalState *s, IntervalOut *out,
+__global__ __launch_bounds__(1) void kIntervalNext(const IteratorContextView *__restrict__ cp, IntervalState *s, IntervalOut *out,
                               int *result)
 {
   const IteratorContextView &c = *cp;
@@ -176,7 +176,7 @@ __global__ void kIntervalNext(const IteratorContextView *__restrict__ cp, Interv
   *result = r ? 1 : 0;
 }
 
-__global__ void kHitInit(const IteratorContextView *__restrict__ cp, HitState *s, const float *ray)
+__global__ __launch_bounds__(1) void kHitInit(const IteratorContextView *__restrict__ cp, HitState *s, const float *ray)
 {
   const IteratorContextView &c = *cp;
   hitInit(c, *s, F3{ray[0], ray[1], ray[2]}, F3{ray[3], ray[4], ray[5]}, Range1f{ray[6], ray[7]}, ray[8]);

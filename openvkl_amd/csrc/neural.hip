@@ -555,8 +555,10 @@ hipError_t launchNfInfer(const NfModel &m, const GridView &g, uint32_t attr, con
 // Exact samples of the fallback points. kObserve: the sampler has a
 // LeafNodeAccess observer, so the traversals record their leaves as the
 // exact sampler's own kernels do.
-template <typename T, bool kObserve>
-__global__ void kGatherSample(SamplerView sv, AttrView a, uint32_t attr, const uint32_t *__restrict__ idx,
+// The filter is a template parameter (the same point sampler as the exact
+// sampling kernel), so the linear kernel carries no cubic code.
+template <typename T, bool kObserve, int kFilter>
+__global__ __launch_bounds__(256) void kGatherSample(SamplerView sv, AttrView a, uint32_t attr, const uint32_t *__restrict__ idx,
                               const uint32_t *__restrict__ count, uint64_t maxCount, const float *__restrict__ xyz,
                               float *__restrict__ out)
 {
@@ -565,30 +567,48 @@ __global__ void kGatherSample(SamplerView sv, AttrView a, uint32_t attr, const u
   if (k >= cnt) return;
   const uint32_t i = idx[k];
   const F3 oc      = {xyz[3 * (uint64_t)i], xyz[3 * (uint64_t)i + 1], xyz[3 * (uint64_t)i + 2]};
-  const SparseSource<T, kObserve> src(sv.grid, a, attr);
-  out[i] = sampleFiltered(src, sv.filter, xfmPoint(sv.grid.o2i, oc), sv.grid.constantCellData ? 0.f : 0.5f);
+  const PointSampler<SparseSource<T, kObserve>, kFilter> smp(sv.grid, a, attr);
+  out[i] = smp(oc);
 }
+
+namespace {
+struct GatherLaunch
+{
+  const SamplerView &s;
+  const AttrView &a;
+  uint32_t attr;
+  const uint32_t *idx, *count;
+  uint64_t maxCount;
+  const float *xyz;
+  float *out;
+  hipStream_t st;
+  template <typename T, bool kObserve, int kFilter>
+  hipError_t run() const
+  {
+    const dim3 grid((unsigned)((maxCount + 255) / 256)), block(256);
+    hipLaunchKernelGGL((kGatherSample<T, kObserve, kFilter>), grid, block, 0, st, s, a, attr, idx, count, maxCount,
+                       xyz, out);
+    return hipGetLastError();
+  }
+  template <typename T, bool kObserve>
+  hipError_t byFilter() const
+  {
+    if (s.filter == kFilterNearest) return run<T, kObserve, kFilterNearest>();
+    if (s.filter == kFilterCubic) return run<T, kObserve, kFilterCubic>();
+    return run<T, kObserve, kFilterLinear>();
+  }
+};
+}  // namespace
 
 hipError_t launchGatherSample(const SamplerView &s, const AttrView &a, uint32_t attr, const uint32_t *idx,
                               const uint32_t *count, uint64_t maxCount, const float *xyz, float *out,
                               hipStream_t stream)
 {
   if (maxCount == 0) return hipSuccess;
-  const dim3 grid((unsigned)((maxCount + 255) / 256)), block(256);
+  const GatherLaunch g{s, a, attr, idx, count, maxCount, xyz, out, stream};
   const bool observed = s.grid.leafAccess != nullptr;
-  if (a.type == kTypeHalf && observed)
-    hipLaunchKernelGGL((kGatherSample<__half, true>), grid, block, 0, stream, s, a, attr, idx, count, maxCount, xyz,
-                       out);
-  else if (a.type == kTypeHalf)
-    hipLaunchKernelGGL((kGatherSample<__half, false>), grid, block, 0, stream, s, a, attr, idx, count, maxCount, xyz,
-                       out);
-  else if (observed)
-    hipLaunchKernelGGL((kGatherSample<float, true>), grid, block, 0, stream, s, a, attr, idx, count, maxCount, xyz,
-                       out);
-  else
-    hipLaunchKernelGGL((kGatherSample<float, false>), grid, block, 0, stream, s, a, attr, idx, count, maxCount, xyz,
-                       out);
-  return hipGetLastError();
+  if (a.type == kTypeHalf) return observed ? g.byFilter<__half, true>() : g.byFilter<__half, false>();
+  return observed ? g.byFilter<float, true>() : g.byFilter<float, false>();
 }
 
 // ---- per-leaf error bounds ---------------------------------------------------------------------

@@ -5,7 +5,7 @@
 // context's depth; persistent hit kernel with its own ray counter), and the
 // batches are captured on parallel branches of the graph, so small batches of
 // different volumes run side by side on the GPU instead of one after the
-// other.
+// other. Batches are spread round-robin over at most kMaxBranches branches.
 //
 // A plan keeps copies of the context views it was built with (in its own
 // device memory), so later changes of a context or its sampler (observers)
@@ -22,6 +22,8 @@
 #include "openvkl/openvkl_hip.h"
 
 namespace vklhip {
+
+constexpr unsigned kMaxBranches = 16;
 
 struct RayBatchPlan
 {
@@ -93,20 +95,22 @@ OPENVKL_INTERFACE VKLRayBatchPlan vklNewRayBatchPlan(unsigned int numBatches, co
     int cus = 256;
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev->hipDevice);
 
-    // Capture: the main stream forks one branch per batch and joins them all.
+    // Capture: the main stream forks the branches, batch i goes to branch
+    // i mod (number of branches), and the main stream joins them all.
+    const unsigned nb = numBatches < kMaxBranches ? numBatches : kMaxBranches;
     checkHip(hipStreamCreateWithFlags(&main, hipStreamNonBlocking), "hipStreamCreate");
-    events.resize(numBatches + 1, nullptr);
+    events.resize(nb + 1, nullptr);
     for (auto &e : events) checkHip(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
-    branches.resize(numBatches, nullptr);
+    branches.resize(nb, nullptr);
     for (auto &b : branches) checkHip(hipStreamCreateWithFlags(&b, hipStreamNonBlocking), "hipStreamCreate");
     checkHip(hipStreamBeginCapture(main, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
     capturing = true;
-    checkHip(hipEventRecord(events[numBatches], main), "fork");
+    checkHip(hipEventRecord(events[nb], main), "fork");
+    for (hipStream_t b : branches) checkHip(hipStreamWaitEvent(b, events[nb], 0), "fork wait");
     const IteratorContextView *dv = p->views.as<IteratorContextView>();
     uint32_t *work                = p->work.as<uint32_t>();
     for (unsigned i = 0; i < numBatches; i++) {
-      hipStream_t b = branches[i];
-      checkHip(hipStreamWaitEvent(b, events[numBatches], 0), "fork wait");
+      hipStream_t b = branches[i % nb];
       if (numRays[i]) {
         const IteratorContextView &hv = hostViews[i];
         const float *o  = reinterpret_cast<const float *>(origins[i]);
@@ -126,8 +130,10 @@ OPENVKL_INTERFACE VKLRayBatchPlan vklNewRayBatchPlan(unsigned int numBatches, co
         }
         p->numLaunches++;
       }
-      checkHip(hipEventRecord(events[i], b), "join");
-      checkHip(hipStreamWaitEvent(main, events[i], 0), "join wait");
+    }
+    for (unsigned k = 0; k < nb; k++) {
+      checkHip(hipEventRecord(events[k], branches[k]), "join");
+      checkHip(hipStreamWaitEvent(main, events[k], 0), "join wait");
     }
     capturing = false;
     checkHip(hipStreamEndCapture(main, &p->graph), "hipStreamEndCapture");

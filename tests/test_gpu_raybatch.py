@@ -77,3 +77,31 @@ def test_ray_batch_plan_matches_batch_calls_and_oracle(device, volumes):
     plan.release()
     for c in contexts:
         c.release()
+
+
+def test_ray_batch_plan_more_batches_than_branches(device, volumes):
+    """20 batches over 16 graph branches (two batches share a branch in
+    order), alternating an interval and a hit context of one volume."""
+    vdb = volumes[0]
+    s = vdb.sampler()
+    ictx, hctx = s.interval_context(), s.hit_context([0.5])
+    box = np.asarray(vdb.bounding_box(), np.float32)
+    batches = []
+    for k in range(20):
+        org, d = rays_through(box, 40 + 7 * k, seed=600 + k)
+        c = ictx if k % 2 == 0 else hctx
+        batches.append((c, torch.from_numpy(org).cuda(), torch.from_numpy(d).cuda(), 64 if k % 2 == 0 else 8))
+    plan = vkl.RayBatchPlan(device, batches)
+    assert plan.launch_count() == 20
+    plan.execute()
+    torch.cuda.synchronize()
+    for k, b in enumerate(batches):
+        w_out, w_cnt = b[0].batch(b[1], b[2], max_per_ray=b[3])
+        got_cnt = plan.counts[k].cpu().numpy()
+        assert np.array_equal(got_cnt, w_cnt.cpu().numpy()), k
+        gr, wr = plan.results[k].cpu().numpy(), w_out.cpu().numpy()
+        live = np.arange(gr.shape[1])[None, :] < got_cnt[:, None]
+        assert same(gr[live], wr[live]), k
+    plan.release()
+    ictx.release()
+    hctx.release()
