@@ -107,6 +107,29 @@ def test_fallback_bounds_error(trained):
     assert float((err > tol).float().mean()) < 0.01
 
 
+def test_tolerance_trades_fallback_for_error(trained):
+    """Network plus fallback against the exact sampler: tightening the
+    per-leaf tolerance sends more points to the exact sampler and lowers the
+    error monotonically, and at the tightest tolerance the normalised RMSE is
+    under 1e-3 with every point still answered (fallback points exactly)."""
+    vol, s, origins, nf, _ = trained
+    span = float(np.diff(vol.value_range())[0])
+    pts = torch.from_numpy(leaf_points(origins, 1 << 19, seed=41)).cuda()
+    exact = s.sample(pts)
+    defined = torch.isfinite(exact)
+    fallback, nrmse = [], []
+    for rel_tol in (4e-2, 1e-2, 2.5e-3):
+        nf.finalize(rel_tol * span, points_per_leaf=128, safety=1.0)
+        y, nfb = nf.sample(pts, fallback=True)
+        err = (y - exact)[defined].double()
+        fallback.append(nfb / pts.shape[0])
+        nrmse.append(float(torch.sqrt((err ** 2).mean())) / span)
+    print(f"fallback fractions {fallback}, normalised RMSE {nrmse}")
+    assert fallback[0] < fallback[1] < fallback[2]
+    assert nrmse[0] > nrmse[1] > nrmse[2]
+    assert nrmse[2] < 1e-3
+
+
 def test_ablation_network_vs_exact_and_no_hash_grid(trained, device):
     """Ablation of the multiresolution grid: the same network and training with
     every level at the coarsest resolution (16-17 cells across the volume)
