@@ -189,8 +189,9 @@ OPENVKL_INTERFACE void vklReleaseSampleBatchPlan(VKLSampleBatchPlan plan);
  * counts[i] the number per ray; tRanges (or tRanges[i]) NULL means [0, inf),
  * times (or times[i]) NULL means time 0 (hit contexts). Every array must be
  * device memory and stay valid while the plan is used. The batches run on
- * parallel branches of one hipGraph (at most 16, batches spread round-robin); the plan keeps the context state it was
- * built with and retains the contexts. */
+ * parallel branches of one hipGraph (at most 16, batches spread round-robin);
+ * the plan keeps the context state it was built with and retains the
+ * contexts. */
 typedef struct VKLRayBatchPlan_ *VKLRayBatchPlan;
 OPENVKL_INTERFACE VKLRayBatchPlan vklNewRayBatchPlan(unsigned int numBatches, const VKLObject *contexts,
                                                      const unsigned int *numRays, const vkl_vec3f *const *origins,
