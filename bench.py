@@ -89,6 +89,43 @@ def variant_layouts(torch, vkl, dev, vol_t, pts, out, steps, warmup, alg_bytes):
     return res
 
 
+def variant_config1(torch, vkl, dev, steps, warmup, cpu=False):
+    """The reference's own CPU-runnable case: structuredRegular 128^3 float32
+    (the same wavelet + noise generator), 2^20 uniform random points,
+    linear, vklComputeSampleN on the x-fastest volume (8 MiB: cache
+    resident). With `cpu`, the oracle on every usable host core beside it."""
+    n, npts = 128, 1 << 20
+    vol = make_volume(torch, n)
+    v = dev.structured_regular(vol)
+    s = v.sampler(filter=vkl.VKL_FILTER_LINEAR)
+    g = torch.Generator(device="cuda").manual_seed(SEED + 30)
+    pts = torch.rand((npts, 3), device="cuda", generator=g) * float(n - 1)
+    out = torch.empty(npts, device="cuda")
+    ms = timed(torch, lambda: s.sample(pts, out=out), steps, warmup)
+    res = {"volume": "structuredRegular 128^3 float32", "points": npts, "ms": round(ms, 4),
+           "msamples_s": round(npts / ms / 1e3, 1)}
+    if cpu:
+        import oracle as O
+        O.use_native()
+        threads, _, _ = host_cpu()
+        grid = O.Grid.dense(vol.cpu().numpy())
+        p_np = pts.cpu().numpy()
+        got = grid.sample(p_np, threads=threads)
+        best = float("inf")
+        for _ in range(5):
+            t0 = time.perf_counter()
+            got = grid.sample(p_np, threads=threads)
+            best = min(best, time.perf_counter() - t0)
+        want = out.cpu().numpy()
+        res["cpu_msamples_s"] = round(npts / best / 1e6, 2)
+        res["cpu_threads"] = threads
+        res["cpu_bitwise_equal_to_gpu"] = bool(np.all((got.view(np.uint32) == want.view(np.uint32)) |
+                                                      (np.isnan(got) & np.isnan(want))))
+    s.release()
+    v.release()
+    return res
+
+
 def variant_config2(torch, vkl, dev, vol_t, pts, steps, warmup, U):
     """The rest of the 512^3 / 2^26-point matrix on corner-packed volumes:
     half and uint8 (raw codes; the scale is applied by the caller) linear,
@@ -605,8 +642,9 @@ def main():
                     help="device voxel layout: the caller's x-fastest array (with --brick) or corner-packed cells")
     ap.add_argument("--binned", choices=("auto", "on", "off"), default="auto",
                     help="spatially binned sampling (auto: on for large volumes and batches)")
-    ap.add_argument("--variants", default="layouts,config2,raymarch,woodcock,batched,neural,iterators",
-                    help="comma list of extra measurements: layouts, config2, raymarch, woodcock, batched, neural, "
+    ap.add_argument("--variants", default="config1,layouts,config2,raymarch,woodcock,batched,neural,iterators",
+                    help="comma list of extra measurements: config1, layouts, config2, raymarch, woodcock, batched, "
+                         "neural, "
                          "iterators "
                          "('' = none)")
     ap.add_argument("--nf-res", type=int, default=4096, help="index-domain edge of the neural-field vdb")
@@ -725,7 +763,9 @@ def main():
     extra = {}
     for v in variants:
         log(f"variant {v} ...")
-        if v == "layouts":
+        if v == "config1":
+            extra["config1"] = variant_config1(torch, vkl, dev, args.steps, args.warmup, cpu=not args.no_cpu_baseline)
+        elif v == "layouts":
             extra["layouts"] = variant_layouts(torch, vkl, dev, vol_t, pts, out, args.steps, args.warmup,
                                                bytes_per_launch)
         elif v == "raymarch":
