@@ -524,6 +524,21 @@ def variant_iterators(torch, vkl, dev, res, steps, warmup, rays_log2=21, cpu=Fal
     acc = torch.empty(r, device="cuda")
     ms_m = timed(torch, lambda: s.ray_march(org, d, tr, march, out=acc), max(1, steps // 2), 1)
     log("  ray march done")
+    # sampling along the rays at nominalDeltaT inside the intervals, with
+    # transfer-function compositing (vklComputeRayMarchIntervalsN; a faint
+    # opacity so that most rays cross the whole shell)
+    lo_v, hi_v = vol.value_range()
+    tf = torch.rand((32, 4), device="cuda", generator=torch.Generator(device="cuda").manual_seed(SEED + 12))
+    tf[:, 3] = torch.linspace(0.0, 0.002, 32, device="cuda")
+    rmi = {}
+
+    def run_rmi():
+        rmi["rgba"], rmi["cnt"] = ictx.ray_march(org, d, tf, (lo_v, hi_v), sampling_rate=1.0)
+
+    ms_rmi = timed(torch, run_rmi, max(1, steps // 2), 1)
+    rmi_samples = float(rmi["cnt"].double().sum())
+    rmi_opaque = float((rmi["rgba"][:, 3] >= 0.99).double().mean())
+    log("  interval ray march done")
     # ragged ray batches of four contexts (two interval depths, two iso sets)
     # in one plan, against one batch call per context
     ctxs = [ictx, ictx1, hctx, s.hit_context([0.25])]
@@ -556,6 +571,12 @@ def variant_iterators(torch, vkl, dev, res, steps, warmup, rays_log2=21, cpu=Fal
                                "one_call_per_batch_ms": round(ms_calls, 3),
                                "what": "intervals at hint 0.5 and 1.0, hits at iso 0.5 and 0.25; 2^20, 2^18, "
                                        "2^19, 2^17 rays; one graph with a branch per batch"},
+            "interval_raymarch": {"ms": round(ms_rmi, 3), "mrays_s": round(r / ms_rmi / 1e3, 2),
+                                  "samples_per_ray": round(rmi_samples / r, 1),
+                                  "msamples_s": round(rmi_samples / ms_rmi / 1e3, 1),
+                                  "opaque_rays": round(rmi_opaque, 4),
+                                  "what": "samples at nominalDeltaT inside the intervals (hint 0.5) with "
+                                          "transfer-function compositing, vklComputeRayMarchIntervalsN"},
             "raymarch_steps_per_ray": march, "raymarch_mrays_s": round(r / ms_m / 1e3, 2),
             "raymarch_msamples_s": round(r * march / ms_m / 1e3, 1)}
 
@@ -717,6 +738,18 @@ def main():
     if packed and n == 512 and npts == 1 << 26 and os.path.exists(tfile):
         kv = dict(line.split()[:2] for line in open(tfile) if line.strip() and not line.startswith("#"))
         traffic = int(kv["fetch_bytes_per_launch"]) + int(kv["write_bytes_per_launch"])
+    # The same call on host arrays (pinned): the library stages the points in
+    # and the samples out over PCIe; wall time of the synchronous call.
+    log("timing the call on host (pinned) arrays ...")
+    pts_h = pts.cpu().pin_memory()
+    out_h = torch.empty(npts, dtype=torch.float32).pin_memory()
+    sampler.sample(pts_h, out=out_h)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        sampler.sample(pts_h, out=out_h)
+    ms_host = (time.perf_counter() - t0) / 3 * 1e3
+    host_same = bool(torch.equal(out_h.view(torch.int32), out.cpu().view(torch.int32)))
+    del pts_h, out_h
     log("measuring the copy peak and the random-record gather ceiling ...")
     peak = copy_peak(torch)
     ceiling = random_line_ceiling(torch, npts)
@@ -743,7 +776,8 @@ def main():
                    "volume_dims": [n, n, n], "points": npts, "filter": "linear", "exact": True,
                    "device_layout": "corner-packed cells (8 x 4 B per cell)" if packed else
                    "x-fastest" if brick == (0, 0, 0) else "bricks %dx%dx%d" % tuple(1 << b for b in brick),
-                   "sampling_path": "binned (bin scatter + LDS-brick sample)" if binned else "direct gather"},
+                   "sampling_path": "binned (bin scatter + LDS-brick sample)" if binned else "direct gather",
+                   "seed": SEED},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "traffic_source": f"profiles/{tname} (stored rocprofv3 FETCH_SIZE + WRITE_SIZE of this command)"
@@ -758,6 +792,10 @@ def main():
                                  "that reads random points in random order",
                          "kernel_frac_of_ceiling": round(npts / (ms_event * 1e-3) / 1e9 / ceiling["gsamples_s"], 4),
                          "gsamples_s_by_footprint": by_footprint}},
+        "host_pointers": {"ms": round(ms_host, 3), "msamples_s": round(npts / ms_host / 1e3, 1),
+                          "bitwise_equal_to_device_call": host_same,
+                          "what": "the same vklComputeSampleN on pinned host arrays: points staged in and "
+                                  "samples out over PCIe inside the call (wall time); not the headline value"},
     }
     variants = [v for v in args.variants.split(",") if v]
     extra = {}
