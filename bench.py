@@ -131,7 +131,10 @@ def variant_config2(torch, vkl, dev, vol_t, pts, steps, warmup, U):
     half and uint8 (raw codes; the scale is applied by the caller) linear,
     float32 cubic, float32 linear gradient; half / uint8 linear on their raw
     x-fastest arrays (256 / 128 MiB, within the Infinity Cache) and in
-    128-byte bricks (direct gathers, binned path off); half cubic."""
+    128-byte bricks (direct gathers, binned path off); half cubic; float32
+    and half cubic samples and float32 cubic gradients with cubic records
+    (wavefront-cooperative stencil loads) beside the same on the
+    corner-packed records alone."""
     npts = pts.shape[0]
     out = torch.empty(npts, device="cuda")
     gout = torch.empty((npts, 3), device="cuda")
@@ -147,16 +150,22 @@ def variant_config2(torch, vkl, dev, vol_t, pts, steps, warmup, U):
              ("half_linear_bricks_4x4x4", half, vkl.VKL_FILTER_LINEAR, False, 2, {"brick_shift": (2, 2, 2)}),
              ("uint8_linear_x_fastest", u8, vkl.VKL_FILTER_LINEAR, False, 1, {}),
              ("uint8_linear_bricks_8x4x4", u8, vkl.VKL_FILTER_LINEAR, False, 1, {"brick_shift": (3, 2, 2)}),
-             ("half_cubic", half, vkl.VKL_FILTER_CUBIC, False, 2, packed))
+             ("half_cubic", half, vkl.VKL_FILTER_CUBIC, False, 2, packed),
+             ("float32_cubic_gradient", vol_t, vkl.VKL_FILTER_CUBIC, True, 4, packed),
+             ("float32_cubic_records", vol_t, vkl.VKL_FILTER_CUBIC, False, 4, dict(packed, cubic_records=True)),
+             ("float32_cubic_gradient_records", vol_t, vkl.VKL_FILTER_CUBIC, True, 4,
+              dict(packed, cubic_records=True)),
+             ("half_cubic_records", half, vkl.VKL_FILTER_CUBIC, False, 2, dict(packed, cubic_records=True)))
     for name, data, filt, grad, es, layout in cases:
         v = dev.structured_regular(data.contiguous(), **layout)
         sm = v.sampler(filter=filt, binned=False)
+        vox = es * (U if filt != vkl.VKL_FILTER_CUBIC else data.numel())
         if grad:
             ms = timed(torch, lambda: sm.gradient(pts, out=gout), steps, warmup)
-            alg = npts * (12 + 12) + es * U
+            alg = npts * (12 + 12) + vox
         else:
             ms = timed(torch, lambda: sm.sample(pts, out=out), steps, warmup)
-            alg = npts * (12 + 4) + es * (U if filt != vkl.VKL_FILTER_CUBIC else data.numel())
+            alg = npts * (12 + 4) + vox
         gbs = alg / (ms * 1e-3) / 1e9
         res[name] = {"msamples_s": round(npts / ms / 1e3, 1), "ms": round(ms, 4), "algorithmic_gbs": round(gbs, 1),
                      "frac": round(gbs / HBM_PEAK_GBS, 4)}

@@ -312,6 +312,8 @@ bool binnedGeometry(Sampler *s, const AttrView &a, uint64_t n, BinOp op, BinGeom
   if (s->binnedMode < 0) {
     // One record load per stencil already: the direct kernel is the better fit.
     if (g.layout.cornerPacked) return false;
+    // Tricubic with cubic records: the stencil-cooperative kernel.
+    if (a.cubic && (op == kBinSampleCubic || op == kBinGradientCubic)) return false;
     const uint64_t bytes = (uint64_t)g.activeSize[0] * g.activeSize[1] * g.activeSize[2] * es;
     if (n < kBinnedMinPoints || bytes < kBinnedMinVolumeBytes) return false;
   }

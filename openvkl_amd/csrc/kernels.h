@@ -91,6 +91,16 @@ hipError_t launchSwizzle(const void *src, void *dst, uint32_t elemSize, const ui
 hipError_t launchPackCorners(const void *src, void *dst, uint32_t elemSize, const uint32_t dims[3],
                              hipStream_t stream);
 
+// Cubic records (16 elements per voxel, see AttrView::cubic) of an x-fastest
+// dims[0] x dims[1] x dims[2] array; dst holds 16 * voxels elements.
+hipError_t launchPackCubicRecords(const void *src, void *dst, uint32_t elemSize, const uint32_t dims[3],
+                                  hipStream_t stream);
+
+// Tricubic samples (out[i * stride]) or gradients (out[3 i .. 3 i + 2]) of n
+// points of a structuredRegular volume whose attribute has cubic records.
+hipError_t launchCubicRecords(const SamplerView &s, const AttrView &a, uint32_t attr, uint64_t n, const float *xyz,
+                              float *out, uint32_t stride, bool gradient, hipStream_t stream);
+
 // vdb commit: the leaf index grid over g.leafDims (filled with kLeafEmpty,
 // then the slot of each dense leaf -- origins absolute, 3 int32 per leaf,
 // packed-slot order -- then kLeafTraverse over every cell a tile node of

@@ -176,6 +176,11 @@ hipError_t dispatchDense(const SamplerView &s, const AttrView &a, uint32_t attr,
 hipError_t dispatch(const SamplerView &s, uint32_t attr, const AttrView &a, uint64_t n, const float *xyz,
                     float *out, uint32_t stride, const float *times, hipStream_t st, bool gradient)
 {
+  // Tricubic on volumes with cubic records: wavefront-cooperative stencil
+  // loads (kernels_cubic.hip), the same samples.
+  if (s.grid.dense && !s.grid.temporalFormat && a.cubic && s.grid.activeSize[0] >= 4 &&
+      (gradient ? s.gradientFilter : s.filter) == kFilterCubic)
+    return launchCubicRecords(s, a, attr, n, xyz, out, stride, gradient, st);
   if (s.grid.dense) {
     switch (a.type) {
     case kTypeUchar: return dispatchDense<uint8_t>(s, a, attr, n, xyz, out, stride, times, st, gradient);

@@ -717,16 +717,14 @@ VKL_DEV void cubicTaps(const TemporalDenseSource<T> &src, I3 ic, float *S)
   for (int i = 0; i < 64; i++) S[i] = src.at(ic.x + (i >> 4) - 1, ic.y + ((i >> 2) & 3) - 1, ic.z + (i & 3) - 1);
 }
 
-template <typename Src>
-VKL_DEV float sampleCubic(const Src &src, F3 p)
+// The tricubic interpolant at index-space point p from its 64 taps
+// S[16 i + 4 j + l] = voxel ic + (i - 1, j - 1, l - 1), ic = floor(p).
+VKL_DEV float cubicFromTaps(const float *S, F3 p, I3 ic)
 {
-  const I3 ic = {floorToInt(p.x), floorToInt(p.y), floorToInt(p.z)};
   float wx[4], wy[4], wz[4];
   crWeights(p.x - (float)ic.x, wx);
   crWeights(p.y - (float)ic.y, wy);
   crWeights(p.z - (float)ic.z, wz);
-  float S[64];
-  cubicTaps(src, ic, S);
   float r = 0.f;
 #pragma unroll
   for (int i = 0; i < 4; i++) {
@@ -741,10 +739,9 @@ VKL_DEV float sampleCubic(const Src &src, F3 p)
   return r;
 }
 
-template <typename Src>
-VKL_DEV F3 gradientCubic(const Src &src, F3 p)
+// Its index-space gradient, from the same taps.
+VKL_DEV F3 cubicGradientFromTaps(const float *S, F3 p, I3 ic)
 {
-  const I3 ic = {floorToInt(p.x), floorToInt(p.y), floorToInt(p.z)};
   float wx[4], wy[4], wz[4], dx[4], dy[4], dz[4];
   const F3 d = {p.x - (float)ic.x, p.y - (float)ic.y, p.z - (float)ic.z};
   crWeights(d.x, wx);
@@ -753,8 +750,6 @@ VKL_DEV F3 gradientCubic(const Src &src, F3 p)
   crDerivWeights(d.x, dx);
   crDerivWeights(d.y, dy);
   crDerivWeights(d.z, dz);
-  float S[64];
-  cubicTaps(src, ic, S);
   F3 g = {0.f, 0.f, 0.f};
 #pragma unroll
   for (int i = 0; i < 4; i++) {
@@ -773,6 +768,24 @@ VKL_DEV F3 gradientCubic(const Src &src, F3 p)
     g.z += wx[i] * gy_dz;
   }
   return g;
+}
+
+template <typename Src>
+VKL_DEV float sampleCubic(const Src &src, F3 p)
+{
+  const I3 ic = {floorToInt(p.x), floorToInt(p.y), floorToInt(p.z)};
+  float S[64];
+  cubicTaps(src, ic, S);
+  return cubicFromTaps(S, p, ic);
+}
+
+template <typename Src>
+VKL_DEV F3 gradientCubic(const Src &src, F3 p)
+{
+  const I3 ic = {floorToInt(p.x), floorToInt(p.y), floorToInt(p.z)};
+  float S[64];
+  cubicTaps(src, ic, S);
+  return cubicGradientFromTaps(S, p, ic);
 }
 
 template <typename Src>

@@ -46,6 +46,12 @@ struct AttrView
   // records of 8 floats, the values the sampler reads at the cell's 8
   // trilinear corners (record order 4 dx + 2 dy + dz); null otherwise.
   const float *cells;
+  // structuredRegular with "cubicRecords": per voxel (x, y, z), x fastest, a
+  // record of 16 elements e[4 j + k] = v(x, y - 1 + j, z - 1 + k), y and z
+  // clamped into the domain. The 4x4x4 tricubic stencil of cell (x, y, z) is
+  // the 4 consecutive records of x - 1 .. x + 2 (64 elements in a row).
+  // Null otherwise.
+  const void *cubic;
 };
 
 // One inner level of the node table (levels 0..2).

@@ -444,46 +444,53 @@ void DenseVolume::commit()
   const bool bricked = S != 0;
   const uint32_t udims[3] = {(uint32_t)dims[0], (uint32_t)dims[1], (uint32_t)dims[2]};
 
+  // "cubicRecords" (bool): also store, per voxel, the 16 elements of its
+  // 4x4 (y, z) neighbourhood (AttrView::cubic; 16x the voxel memory), which
+  // the tricubic stream kernels load as whole stencils.
+  const bool cubicRecords = getBool("cubicRecords", false);
+  if (cubicRecords && tformat != VKL_TEMPORAL_FORMAT_CONSTANT)
+    throw VklError("cubicRecords is not supported for temporal data", VKL_INVALID_ARGUMENT);
+
   const std::vector<float> bg = parseBackground(*this, grid.numAttributes);
   attrViews.clear();
   for (size_t a = 0; a < attrs.size(); a++) {
     AttrView av{};
-    if (packed) {
-      const Data &d = *attrs[a];
-      std::unique_ptr<DeviceBuffer> tmp;
-      const void *src = d.base;
-      if (!(d.onDevice && d.compact())) {
+    const Data &d   = *attrs[a];
+    const size_t es = sizeOfType(d.dataType);
+    // The caller's x-fastest elements on the device: its own buffer when
+    // that is compact device memory, else a temporary upload.
+    std::unique_ptr<DeviceBuffer> tmp;
+    auto compactSource = [&]() -> const void * {
+      if (d.onDevice && d.compact()) return d.base;
+      if (!tmp) {
         std::vector<uint8_t> h = d.hostCopy();
         tmp.reset(new DeviceBuffer(h.size()));
         tmp->upload(h.data(), h.size(), st);
-        src = tmp->ptr;
       }
-      const size_t es   = sizeOfType(d.dataType);
+      return tmp->ptr;
+    };
+    if (packed) {
       DeviceBuffer *dst = newBuffer(nvox * 8 * es);
-      checkHip(launchPackCorners(src, dst->ptr, (uint32_t)es, udims, st), "corner packing");
+      checkHip(launchPackCorners(compactSource(), dst->ptr, (uint32_t)es, udims, st), "corner packing");
       checkHip(hipStreamSynchronize(st), "corner packing");
       av.data = dst->ptr;
     } else if (bricked) {
-      // Swizzle from the caller's device buffer, or from a temporary upload.
-      const Data &d = *attrs[a];
-      std::unique_ptr<DeviceBuffer> tmp;
-      const void *src = d.base;
-      if (!(d.onDevice && d.compact())) {
-        std::vector<uint8_t> h = d.hostCopy();
-        tmp.reset(new DeviceBuffer(h.size()));
-        tmp->upload(h.data(), h.size(), st);
-        src = tmp->ptr;
-      }
-      const size_t es   = sizeOfType(d.dataType);
       DeviceBuffer *dst = newBuffer((nbx * nby * nbz << S) * es);
-      checkHip(launchSwizzle(src, dst->ptr, (uint32_t)es, udims, L, (nbx * nby * nbz) << S, st), "swizzle");
+      checkHip(launchSwizzle(compactSource(), dst->ptr, (uint32_t)es, udims, L, (nbx * nby * nbz) << S, st),
+               "swizzle");
       checkHip(hipStreamSynchronize(st), "swizzle");
       av.data = dst->ptr;
     } else {
-      av.data = deviceData(*this, *attrs[a], st);
+      av.data = deviceData(*this, d, st);
+    }
+    if (cubicRecords) {
+      DeviceBuffer *dst = newBuffer(nvox * 16 * es);
+      checkHip(launchPackCubicRecords(compactSource(), dst->ptr, (uint32_t)es, udims, st), "cubic records");
+      checkHip(hipStreamSynchronize(st), "cubic records");
+      av.cubic = dst->ptr;
     }
     av.tile       = nullptr;
-    av.type       = (uint32_t)attrs[a]->dataType;
+    av.type       = (uint32_t)d.dataType;
     av.background = bg[a];
     attrViews.push_back(av);
   }
