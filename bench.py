@@ -816,8 +816,12 @@ def main():
             extra["layouts"] = variant_layouts(torch, vkl, dev, vol_t, pts, out, args.steps, args.warmup,
                                                bytes_per_launch)
         elif v == "raymarch":
-            rv = dev.structured_regular(vol_t)
+            rv = dev.structured_regular(vol_t, corner_packed=True)
             extra["raymarch"] = variant_raymarch(torch, vkl, rv, n, args.steps, args.warmup)
+            extra["raymarch"]["layout"] = "corner-packed"
+            rv.release()
+            rv = dev.structured_regular(vol_t)
+            extra["raymarch_x_fastest"] = variant_raymarch(torch, vkl, rv, n, args.steps, args.warmup)
             rv.release()
         elif v == "woodcock":
             rv = dev.structured_regular(vol_t, corner_packed=True)
