@@ -286,6 +286,10 @@ def variant_neural(torch, vkl, dev, res, steps, warmup, train_log2=28, infer_log
     approx = torch.empty_like(exact)
     ws = torch.empty(pts.shape[0] + 1, device="cuda", dtype=torch.int32)
     ms_exact = timed(torch, lambda: s.sample(pts, out=exact), steps, warmup)
+    # tricubic on the same points (the stencil's leaf cells looked up once)
+    s_c = vol.sampler(filter=vkl.VKL_FILTER_CUBIC)
+    ms_cubic = timed(torch, lambda: s_c.sample(pts, out=approx), steps, warmup)
+    s_c.release()
     # the exact sampler on the corner-packed form of the same volume
     vol_p = dev.vdb(np.full(n, 3, np.uint32), origins, np.full(n, vkl.VKL_FORMAT_DENSE_ZYX, np.uint32),
                     packed_dense=[leaves], corner_packed=True)
@@ -357,6 +361,7 @@ def variant_neural(torch, vkl, dev, res, steps, warmup, train_log2=28, infer_log
     nf.release()
     return {"volume": f"vdb shell+fBm, {res}^3 index domain, {n} float32 8^3 leaves ({n * 2048 / 2 ** 30:.2f} GiB)",
             "exact_msamples_s": round(pts.shape[0] / ms_exact / 1e3, 1),
+            "exact_cubic_msamples_s": round(pts.shape[0] / ms_cubic / 1e3, 1),
             "exact_corner_packed_msamples_s": round(pts.shape[0] / ms_exact_p / 1e3, 1),
             "exact_corner_packed_bitexact": same,
             "neural_msamples_s": by_tol[0]["neural_msamples_s"], "speedup_vs_exact": by_tol[0]["speedup_vs_exact"],

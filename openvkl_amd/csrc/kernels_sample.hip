@@ -31,7 +31,7 @@ __global__ __launch_bounds__(kBlock) void kSample(GridView g, AttrView a, uint32
   if (kFilter == kFilterNearest)
     r = sampleNearest(src, p, noff);
   else if (kFilter == kFilterCubic)
-    r = sampleCubic(src, p);
+    r = sampleCubicStream(src, p);
   else
     r = sampleLinear(src, p);
   __builtin_nontemporal_store(r, out + i * outStride);
@@ -52,7 +52,7 @@ __global__ __launch_bounds__(kBlock) void kGradient(GridView g, AttrView a, uint
     const F3 p  = xfmPoint(g.o2i, oc);
     Src src(g, a, attr);
     if (times) src.setTime(times[i]);
-    const F3 gi = (kFilter == kFilterCubic) ? gradientCubic(src, p) : gradientLinear(src, p);
+    const F3 gi = (kFilter == kFilterCubic) ? gradientCubicStream(src, p) : gradientLinear(src, p);
     r           = xfmNormal(g.o2i, gi);
   }
   out[3 * i]     = r.x;

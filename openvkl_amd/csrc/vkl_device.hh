@@ -707,6 +707,59 @@ VKL_DEV void cubicTaps(const DenseSource<T, kIdx64> &src, I3 ic, float *S)
   for (int i = 0; i < 64; i++) S[i] = src.at(ic.x + (i >> 4) - 1, ic.y + ((i >> 2) & 3) - 1, ic.z + (i & 3) - 1);
 }
 
+// vdb volumes with the flattened leaf index (unobserved, full sampling
+// depth): a stencil inside the domain touches at most 2 leaf cells per axis,
+// so their 8 index entries are read once and each tap reads its leaf
+// directly, instead of one index lookup per tap; the taps equal 64 at()
+// calls (dense leaf: its voxel; a cell no node covers: 0). A stencil that
+// leaves the domain or meets a tile / error cell takes the 64 at() calls.
+template <typename T, bool kObserve>
+VKL_DEV void cubicTapsByLeafCells(const SparseSource<T, kObserve> &src, I3 ic, float *S)
+{
+  const uint32_t ox = (uint32_t)(ic.x - 1 - src.root[0]), oy = (uint32_t)(ic.y - 1 - src.root[1]),
+                 oz = (uint32_t)(ic.z - 1 - src.root[2]);
+  if (src.flatIndex && src.inDomain(ox, oy, oz) && src.inDomain(ox + 3, oy + 3, oz + 3)) {
+    const uint32_t cx0 = ox >> 3, cy0 = oy >> 3, cz0 = oz >> 3;
+    const uint32_t cx1 = (ox + 3) >> 3, cy1 = (oy + 3) >> 3, cz1 = (oz + 3) >> 3;
+    uint32_t slot[8];
+    bool traverse = false;
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+      const uint32_t cx = (c & 4) ? cx1 : cx0, cy = (c & 2) ? cy1 : cy0, cz = (c & 1) ? cz1 : cz0;
+      slot[c]  = src.flatIndex[((uint64_t)cz * src.ldim[1] + cy) * src.ldim[0] + cx];
+      traverse = traverse || slot[c] == kLeafTraverse;
+    }
+    if (!traverse) {
+      // The cell of each tap by selects (constant register indices only).
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const uint32_t x  = ox + i;
+        const bool hx     = (x >> 3) != cx0;
+        const uint32_t t0 = hx ? slot[4] : slot[0], t1 = hx ? slot[5] : slot[1];
+        const uint32_t t2 = hx ? slot[6] : slot[2], t3 = hx ? slot[7] : slot[3];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          const uint32_t y  = oy + j;
+          const bool hy     = (y >> 3) != cy0;
+          const uint32_t u0 = hy ? t2 : t0, u1 = hy ? t3 : t1;
+#pragma unroll
+          for (int k = 0; k < 4; k++) {
+            const uint32_t z  = oz + k;
+            const uint32_t sl = (z >> 3) != cz0 ? u1 : u0;
+            S[16 * i + 4 * j + k] =
+                sl == kLeafEmpty
+                    ? 0.f
+                    : toFloat(src.dense[(uint64_t)sl * 512u + (((x & 7u) << 6) | ((y & 7u) << 3) | (z & 7u))]);
+          }
+        }
+      }
+      return;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 64; i++) S[i] = src.at(ic.x + (i >> 4) - 1, ic.y + ((i >> 2) & 3) - 1, ic.z + (i & 3) - 1);
+}
+
 // Temporal volumes: the 64 reads are a rolled loop. Each read interpolates in
 // time (a binary search for unstructured data), and 64 inlined copies of that
 // make the kernels that sample them several times larger.
@@ -786,6 +839,46 @@ VKL_DEV F3 gradientCubic(const Src &src, F3 p)
   float S[64];
   cubicTaps(src, ic, S);
   return cubicGradientFromTaps(S, p, ic);
+}
+
+// Tricubic in the stream kernels (kSample, kGradient): vdb sources gather
+// their taps by leaf cells (cubicTapsByLeafCells). The fused ray and
+// iterator kernels keep the per-tap reads of sampleCubic: inlined at their
+// several sampler call sites, the unrolled gather makes those kernels'
+// compilation take tens of minutes.
+template <typename Src>
+struct GathersByLeafCells : std::false_type
+{
+};
+template <typename T, bool kObserve>
+struct GathersByLeafCells<SparseSource<T, kObserve>> : std::true_type
+{
+};
+
+template <typename Src>
+VKL_DEV float sampleCubicStream(const Src &src, F3 p)
+{
+  if constexpr (GathersByLeafCells<Src>::value) {
+    const I3 ic = {floorToInt(p.x), floorToInt(p.y), floorToInt(p.z)};
+    float S[64];
+    cubicTapsByLeafCells(src, ic, S);
+    return cubicFromTaps(S, p, ic);
+  } else {
+    return sampleCubic(src, p);
+  }
+}
+
+template <typename Src>
+VKL_DEV F3 gradientCubicStream(const Src &src, F3 p)
+{
+  if constexpr (GathersByLeafCells<Src>::value) {
+    const I3 ic = {floorToInt(p.x), floorToInt(p.y), floorToInt(p.z)};
+    float S[64];
+    cubicTapsByLeafCells(src, ic, S);
+    return cubicGradientFromTaps(S, p, ic);
+  } else {
+    return gradientCubic(src, p);
+  }
 }
 
 template <typename Src>

@@ -195,3 +195,35 @@ def test_leaf_index_grid_on_and_off(device, index_grid):
     assert np.abs(got[ok] - want[ok]).max() <= 1e-5 * max(np.abs(want[ok]).max(), 1e-30)
     grid.set_sampler()
     vol.release()
+
+
+def test_cubic_stencil_gather_equals_per_tap_reads(device):
+    """Tricubic stencils through the leaf index grid (the <= 8 leaf cells of
+    a stencil looked up once) equal 64 per-tap traversals of the node table
+    bit for bit: samples and gradients, stencils across leaf boundaries, in
+    empty cells, under tiles of levels 1..3 and outside the domain."""
+    origins, leaves = V.sparse_shell_leaves(res=128, radius=0.3, width=0.08)
+    n = len(origins)
+    tile_orig = np.array([[256, 0, 0], [144, 8, 8], [0, 0, 4096], [136, 0, 0]], np.int32)
+    tile_lvl = np.array([2, 3, 1, 3], np.uint32)
+    lv = np.concatenate([np.full(n, 3, np.uint32), tile_lvl])
+    org = np.concatenate([origins, tile_orig])
+    fmt = np.concatenate([np.full(n, vkl.VKL_FORMAT_DENSE_ZYX, np.uint32), np.zeros(4, np.uint32)])
+    leaf_data = [[leaves[i]] for i in range(n)] + [[np.array([0.75 + k], np.float32)] for k in range(4)]
+    grid = O.Grid.vdb(lv, org, fmt, leaf_data=leaf_data)
+    box = grid.bounding_box()
+    pts = np.concatenate([V.uniform_points(150000, (-2, -2, -2), (130, 130, 130), seed=84),
+                          V.uniform_points(50000, (250, -2, -2), (390, 130, 130), seed=85),
+                          V.uniform_points(30000, box[:3] - 2, box[3:] + 2, seed=86)])
+    p = torch.from_numpy(pts).cuda()
+    got = {}
+    for index_grid in (1, 0):
+        vol = device.vdb(lv, org, fmt, leaf_data=leaf_data, leaf_index_grid=index_grid)
+        s = vol.sampler(filter=vkl.VKL_FILTER_CUBIC)
+        got[index_grid] = (s.sample(p).cpu().numpy(), s.gradient(p).cpu().numpy())
+        s.release()
+        vol.release()
+    assert_same(got[1][0], got[0][0])
+    assert_same(got[1][1], got[0][1])
+    grid.set_sampler(O.CUBIC)
+    check_cubic(grid, got[1][0][:30000], pts[:30000])
