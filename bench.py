@@ -797,6 +797,16 @@ def main():
     ms_host = (time.perf_counter() - t0) / 3 * 1e3
     host_same = bool(torch.equal(out_h.view(torch.int32), out.cpu().view(torch.int32)))
     del pts_h, out_h
+    # and on pageable host arrays (numpy)
+    pts_p = pts.cpu().numpy()
+    out_p = np.empty(npts, np.float32)
+    sampler.sample(pts_p, out=out_p)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        sampler.sample(pts_p, out=out_p)
+    ms_pageable = (time.perf_counter() - t0) / 3 * 1e3
+    host_same = host_same and bool(np.array_equal(out_p.view(np.int32), out.cpu().numpy().view(np.int32)))
+    del pts_p, out_p
     log("measuring the copy peak and the random-record gather ceiling ...")
     peak = copy_peak(torch)
     ceiling = random_line_ceiling(torch, npts)
@@ -840,9 +850,11 @@ def main():
                          "kernel_frac_of_ceiling": round(npts / (ms_event * 1e-3) / 1e9 / ceiling["gsamples_s"], 4),
                          "gsamples_s_by_footprint": by_footprint}},
         "host_pointers": {"ms": round(ms_host, 3), "msamples_s": round(npts / ms_host / 1e3, 1),
+                          "pageable_ms": round(ms_pageable, 3),
                           "bitwise_equal_to_device_call": host_same,
-                          "what": "the same vklComputeSampleN on pinned host arrays: points staged in and "
-                                  "samples out over PCIe inside the call (wall time); not the headline value"},
+                          "what": "the same vklComputeSampleN on pinned (ms) and pageable (pageable_ms) host "
+                                  "arrays: points staged in and samples out over PCIe inside the call, in "
+                                  "overlapped chunks (wall time); not the headline value"},
     }
     variants = [v for v in args.variants.split(",") if v]
     extra = {}

@@ -390,6 +390,104 @@ void sampleInto(Sampler *s, uint32_t attr, uint64_t n, const float *xyz, float *
   checkHip(launchSampleStrided(sv, a, attr, n, xyz, out, stride, d->stream, times), "sample kernel");
 }
 
+// Host-pointer calls of at least kPipelineMinPoints points on the direct
+// kernels: the points (and times) go in and the results come out in chunks of
+// kPipelineChunk points, double-buffered, with the copies in on one stream,
+// the kernels on a second and the copies out on a third, ordered by events;
+// so the copy in of one chunk overlaps the kernel of the previous one and
+// the copy out of the one before (PCIe carries both directions at once).
+// The call stays synchronous, and every point gets the same kernel
+// arithmetic as in one launch.
+constexpr uint64_t kPipelineChunk     = 1ull << 22;
+constexpr uint64_t kPipelineMinPoints = 2 * kPipelineChunk;
+
+// launch(count, xyz, times, out, stream) runs the kernels of one chunk on
+// device arrays; xyz / times / out may each be host or device memory.
+template <typename Launch>
+void pipelinedHost(Device *d, uint64_t n, const float *xyz, const float *times, float *out, uint32_t outFloats,
+                   const char *what, Launch launch)
+{
+  const bool xyzHost = !isDevicePointer(xyz), timesHost = times && !isDevicePointer(times);
+  const bool outHost = !isDevicePointer(out);
+  const size_t inBytes = kPipelineChunk * 12, tmBytes = times ? kPipelineChunk * 4 : 0;
+  const size_t outBytes = kPipelineChunk * 4 * (size_t)outFloats, slotBytes = inBytes + tmBytes + outBytes;
+  hipStream_t up = nullptr, comp = nullptr, down = nullptr;
+  hipEvent_t ready = nullptr, upDone[2] = {}, kernelDone[2] = {}, downDone[2] = {};
+  void *buf = nullptr;
+  auto finish = [&]() {
+    for (hipStream_t s : {up, comp, down})
+      if (s) (void)hipStreamSynchronize(s);
+    if (buf) {
+      (void)hipFreeAsync(buf, d->stream);
+      (void)hipStreamSynchronize(d->stream);
+    }
+    for (hipStream_t s : {up, comp, down})
+      if (s) (void)hipStreamDestroy(s);
+    for (hipEvent_t e : {ready, upDone[0], upDone[1], kernelDone[0], kernelDone[1], downDone[0], downDone[1]})
+      if (e) (void)hipEventDestroy(e);
+    buf = nullptr;
+    up = comp = down = nullptr;
+    ready = nullptr;
+  };
+  try {
+    // The buffers, and the stages, follow the work already queued on the
+    // device stream.
+    checkHip(hipMallocAsync(&buf, 2 * slotBytes, d->stream), "hipMallocAsync");
+    for (hipEvent_t *e : {&ready, &upDone[0], &upDone[1], &kernelDone[0], &kernelDone[1], &downDone[0], &downDone[1]})
+      checkHip(hipEventCreateWithFlags(e, hipEventDisableTiming), "hipEventCreate");
+    checkHip(hipEventRecord(ready, d->stream), "hipEventRecord");
+    for (hipStream_t *s : {&up, &comp, &down}) {
+      checkHip(hipStreamCreateWithFlags(s, hipStreamNonBlocking), "hipStreamCreate");
+      checkHip(hipStreamWaitEvent(*s, ready, 0), "hipStreamWaitEvent");
+    }
+    uint64_t chunk = 0;
+    for (uint64_t first = 0; first < n; first += kPipelineChunk, chunk++) {
+      const uint64_t m = std::min(kPipelineChunk, n - first);
+      const int k      = (int)(chunk & 1);
+      uint8_t *b       = static_cast<uint8_t *>(buf) + k * slotBytes;
+      const float *cx  = xyz + 3 * first;
+      const float *ct  = times ? times + first : nullptr;
+      if (xyzHost || timesHost) {
+        // This slot's inputs are free once the kernel two chunks back is done.
+        if (chunk >= 2) checkHip(hipStreamWaitEvent(up, kernelDone[k], 0), "hipStreamWaitEvent");
+        if (xyzHost) {
+          checkHip(hipMemcpyAsync(b, cx, m * 12, hipMemcpyHostToDevice, up), "hipMemcpyAsync");
+          cx = reinterpret_cast<const float *>(b);
+        }
+        if (timesHost) {
+          checkHip(hipMemcpyAsync(b + inBytes, ct, m * 4, hipMemcpyHostToDevice, up), "hipMemcpyAsync");
+          ct = reinterpret_cast<const float *>(b + inBytes);
+        }
+        checkHip(hipEventRecord(upDone[k], up), "hipEventRecord");
+        checkHip(hipStreamWaitEvent(comp, upDone[k], 0), "hipStreamWaitEvent");
+      }
+      float *co = out + (uint64_t)outFloats * first;
+      float *ko = co;
+      if (outHost) {
+        ko = reinterpret_cast<float *>(b + inBytes + tmBytes);
+        // ... and its output once the copy out two chunks back is done.
+        if (chunk >= 2) checkHip(hipStreamWaitEvent(comp, downDone[k], 0), "hipStreamWaitEvent");
+      }
+      checkHip(launch(m, cx, ct, ko, comp), what);
+      checkHip(hipEventRecord(kernelDone[k], comp), "hipEventRecord");
+      if (outHost) {
+        checkHip(hipStreamWaitEvent(down, kernelDone[k], 0), "hipStreamWaitEvent");
+        checkHip(hipMemcpyAsync(co, ko, m * 4 * (size_t)outFloats, hipMemcpyDeviceToHost, down), "hipMemcpyAsync");
+        checkHip(hipEventRecord(downDone[k], down), "hipEventRecord");
+      }
+    }
+  } catch (...) {
+    finish();
+    throw;
+  }
+  finish();
+}
+
+bool pipelines(uint64_t n, const void *xyz, const void *out)
+{
+  return n >= kPipelineMinPoints && (!isDevicePointer(xyz) || !isDevicePointer(out));
+}
+
 // times: n floats (host or device) or null (time 0); only temporal volumes
 // read them.
 void sampleN(Sampler *s, uint64_t n, const vkl_vec3f *oc, float *out, uint32_t attr, const float *times)
@@ -399,6 +497,18 @@ void sampleN(Sampler *s, uint64_t n, const vkl_vec3f *oc, float *out, uint32_t a
   (void)s->attr(attr);
   if (n == 0) return;
   const bool temporal = s->view.grid.temporalFormat != 0 && times;
+  if (pipelines(n, oc, out)) {
+    const AttrView &a    = s->attr(attr);
+    const SamplerView sv = s->viewCopy();
+    BinGeometry geo;
+    if (!binnedGeometry(s, a, n, sampleOp(sv.filter), geo)) {
+      pipelinedHost(d, n, reinterpret_cast<const float *>(oc), temporal ? times : nullptr, out, 1, "sample kernel",
+                    [&](uint64_t m, const float *x, const float *t, float *o, hipStream_t st) {
+                      return launchSampleStrided(sv, a, attr, m, x, o, 1, st, t);
+                    });
+      return;
+    }
+  }
   Staged in(oc, n * 12, d->stream, true);
   Staged tm(temporal ? times : nullptr, temporal ? n * 4 : 0, d->stream, true);
   Staged o(out, n * 4, d->stream, false);
@@ -415,13 +525,22 @@ void gradientN(Sampler *s, uint64_t n, const vkl_vec3f *oc, vkl_vec3f *out, uint
   const SamplerView sv = s->viewCopy();
   if (n == 0) return;
   const bool temporal = sv.grid.temporalFormat != 0 && times;
-  Staged in(oc, n * 12, d->stream, true);
-  Staged tm(temporal ? times : nullptr, temporal ? n * 4 : 0, d->stream, true);
-  Staged o(out, n * 12, d->stream, false);
   BinGeometry geo;
   const int gf   = sv.gradientFilter;
   const BinOp op = gf == kFilterCubic ? kBinGradientCubic : kBinGradientLinear;
-  if (gf != kFilterNearest && binnedGeometry(s, a, n, op, geo)) {
+  const bool binned = gf != kFilterNearest && binnedGeometry(s, a, n, op, geo);
+  if (!binned && pipelines(n, oc, out)) {
+    pipelinedHost(d, n, reinterpret_cast<const float *>(oc), temporal ? times : nullptr,
+                  reinterpret_cast<float *>(out), 3, "gradient kernel",
+                  [&](uint64_t m, const float *x, const float *t, float *o, hipStream_t st) {
+                    return launchGradient(sv, a, attr, m, x, o, st, t);
+                  });
+    return;
+  }
+  Staged in(oc, n * 12, d->stream, true);
+  Staged tm(temporal ? times : nullptr, temporal ? n * 4 : 0, d->stream, true);
+  Staged o(out, n * 12, d->stream, false);
+  if (binned) {
     std::lock_guard<std::mutex> lock(s->binMutex);
     const BinScratch sc = binnedScratch(s, geo, n);
     checkHip(launchBinned(sv, a, geo, op, n, in.as<float>(), o.as<float>(), 1, sc, d->stream),
