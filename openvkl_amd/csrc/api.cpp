@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstring>
 #include <limits>
+#include <thread>
 
 #include "binned.h"
 #include "kernels.h"
@@ -396,10 +397,32 @@ void sampleInto(Sampler *s, uint32_t attr, uint64_t n, const float *xyz, float *
 // the kernels on a second and the copies out on a third, ordered by events;
 // so the copy in of one chunk overlaps the kernel of the previous one and
 // the copy out of the one before (PCIe carries both directions at once).
-// The call stays synchronous, and every point gets the same kernel
+// Pageable host arrays pass through the device's page-locked staging slots,
+// filled and drained by parallel host copies while the GPU works on the other
+// slot. The call stays synchronous, and every point gets the same kernel
 // arithmetic as in one launch.
 constexpr uint64_t kPipelineChunk     = 1ull << 22;
 constexpr uint64_t kPipelineMinPoints = 2 * kPipelineChunk;
+
+// memcpy split over a few host threads (large copies only).
+void parallelCopy(void *dst, const void *src, size_t bytes)
+{
+  const unsigned hw = std::thread::hardware_concurrency();
+  const unsigned T  = bytes >= (8u << 20) ? std::max(1u, std::min(8u, hw)) : 1u;
+  if (T <= 1) {
+    std::memcpy(dst, src, bytes);
+    return;
+  }
+  const size_t part = ((bytes + T - 1) / T + 63) & ~size_t(63);
+  std::vector<std::thread> th;
+  for (unsigned t = 1; t < T && t * part < bytes; t++)
+    th.emplace_back([=]() {
+      std::memcpy(static_cast<uint8_t *>(dst) + t * part, static_cast<const uint8_t *>(src) + t * part,
+                  std::min(part, bytes - t * part));
+    });
+  std::memcpy(dst, src, std::min(part, bytes));
+  for (auto &x : th) x.join();
+}
 
 // launch(count, xyz, times, out, stream) runs the kernels of one chunk on
 // device arrays; xyz / times / out may each be host or device memory.
@@ -409,11 +432,36 @@ void pipelinedHost(Device *d, uint64_t n, const float *xyz, const float *times, 
 {
   const bool xyzHost = !isDevicePointer(xyz), timesHost = times && !isDevicePointer(times);
   const bool outHost = !isDevicePointer(out);
+  // Pageable arrays go through the page-locked staging slots.
+  const bool xyzStage = xyzHost && !isPinnedHost(xyz), timesStage = timesHost && !isPinnedHost(times);
+  const bool outStage = outHost && !isPinnedHost(out);
   const size_t inBytes = kPipelineChunk * 12, tmBytes = times ? kPipelineChunk * 4 : 0;
   const size_t outBytes = kPipelineChunk * 4 * (size_t)outFloats, slotBytes = inBytes + tmBytes + outBytes;
+  std::unique_lock<std::mutex> staging(d->stagingMutex, std::defer_lock);
+  uint8_t *pin = nullptr;
+  if (xyzStage || timesStage || outStage) {
+    staging.lock();
+    if (d->pinnedBytes < 2 * slotBytes) {
+      if (d->pinned) (void)hipHostFree(d->pinned);
+      d->pinned      = nullptr;
+      d->pinnedBytes = 0;
+      checkHip(hipHostMalloc(&d->pinned, 2 * slotBytes, hipHostMallocDefault), "hipHostMalloc");
+      d->pinnedBytes = 2 * slotBytes;
+    }
+    pin = static_cast<uint8_t *>(d->pinned);
+  }
   hipStream_t up = nullptr, comp = nullptr, down = nullptr;
   hipEvent_t ready = nullptr, upDone[2] = {}, kernelDone[2] = {}, downDone[2] = {};
   void *buf = nullptr;
+  // Results of a chunk waiting in a staging slot: (first point, count).
+  uint64_t pendFirst[2] = {0, 0}, pendCount[2] = {0, 0};
+  auto drain = [&](int k) {
+    if (!pendCount[k]) return;
+    checkHip(hipEventSynchronize(downDone[k]), "hipEventSynchronize");
+    parallelCopy(out + (uint64_t)outFloats * pendFirst[k], pin + k * slotBytes + inBytes + tmBytes,
+                 pendCount[k] * 4 * (size_t)outFloats);
+    pendCount[k] = 0;
+  };
   auto finish = [&]() {
     for (hipStream_t s : {up, comp, down})
       if (s) (void)hipStreamSynchronize(s);
@@ -445,17 +493,32 @@ void pipelinedHost(Device *d, uint64_t n, const float *xyz, const float *times, 
       const uint64_t m = std::min(kPipelineChunk, n - first);
       const int k      = (int)(chunk & 1);
       uint8_t *b       = static_cast<uint8_t *>(buf) + k * slotBytes;
+      uint8_t *h       = pin ? pin + k * slotBytes : nullptr;
       const float *cx  = xyz + 3 * first;
       const float *ct  = times ? times + first : nullptr;
       if (xyzHost || timesHost) {
-        // This slot's inputs are free once the kernel two chunks back is done.
-        if (chunk >= 2) checkHip(hipStreamWaitEvent(up, kernelDone[k], 0), "hipStreamWaitEvent");
+        // This slot's device inputs are free once the kernel two chunks back
+        // is done, and its staged inputs once their copy in is.
+        if (chunk >= 2) {
+          checkHip(hipStreamWaitEvent(up, kernelDone[k], 0), "hipStreamWaitEvent");
+          if (xyzStage || timesStage) checkHip(hipEventSynchronize(upDone[k]), "hipEventSynchronize");
+        }
         if (xyzHost) {
-          checkHip(hipMemcpyAsync(b, cx, m * 12, hipMemcpyHostToDevice, up), "hipMemcpyAsync");
+          const void *src = cx;
+          if (xyzStage) {
+            parallelCopy(h, cx, m * 12);
+            src = h;
+          }
+          checkHip(hipMemcpyAsync(b, src, m * 12, hipMemcpyHostToDevice, up), "hipMemcpyAsync");
           cx = reinterpret_cast<const float *>(b);
         }
         if (timesHost) {
-          checkHip(hipMemcpyAsync(b + inBytes, ct, m * 4, hipMemcpyHostToDevice, up), "hipMemcpyAsync");
+          const void *src = ct;
+          if (timesStage) {
+            parallelCopy(h + inBytes, ct, m * 4);
+            src = h + inBytes;
+          }
+          checkHip(hipMemcpyAsync(b + inBytes, src, m * 4, hipMemcpyHostToDevice, up), "hipMemcpyAsync");
           ct = reinterpret_cast<const float *>(b + inBytes);
         }
         checkHip(hipEventRecord(upDone[k], up), "hipEventRecord");
@@ -471,10 +534,23 @@ void pipelinedHost(Device *d, uint64_t n, const float *xyz, const float *times, 
       checkHip(launch(m, cx, ct, ko, comp), what);
       checkHip(hipEventRecord(kernelDone[k], comp), "hipEventRecord");
       if (outHost) {
+        // A staged slot still holding the results two chunks back hands them
+        // to the caller first.
+        if (outStage) drain(k);
         checkHip(hipStreamWaitEvent(down, kernelDone[k], 0), "hipStreamWaitEvent");
-        checkHip(hipMemcpyAsync(co, ko, m * 4 * (size_t)outFloats, hipMemcpyDeviceToHost, down), "hipMemcpyAsync");
+        void *dst = outStage ? static_cast<void *>(h + inBytes + tmBytes) : static_cast<void *>(co);
+        checkHip(hipMemcpyAsync(dst, ko, m * 4 * (size_t)outFloats, hipMemcpyDeviceToHost, down), "hipMemcpyAsync");
         checkHip(hipEventRecord(downDone[k], down), "hipEventRecord");
+        if (outStage) {
+          pendFirst[k] = first;
+          pendCount[k] = m;
+        }
       }
+    }
+    if (outStage) {
+      // Oldest first.
+      drain((int)(chunk & 1));
+      drain((int)((chunk + 1) & 1));
     }
   } catch (...) {
     finish();

@@ -105,6 +105,18 @@ bool isDevicePointer(const void *p)
   return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
 }
 
+bool isPinnedHost(const void *p)
+{
+  if (!p) return false;
+  hipPointerAttribute_t attr;
+  const hipError_t e = hipPointerGetAttributes(&attr, p);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();  // pageable host memory: clear the sticky error
+    return false;
+  }
+  return attr.type == hipMemoryTypeHost;
+}
+
 // ---- DeviceBuffer ------------------------------------------------------------
 
 void DeviceBuffer::allocate(size_t n)
@@ -337,6 +349,11 @@ void defaultErrorCallback(void *, VKLError e, const char *msg)
 }  // namespace
 
 Device::Device() : errorCallback(defaultErrorCallback) {}
+
+Device::~Device()
+{
+  if (pinned) (void)hipHostFree(pinned);
+}
 
 // Device params (openvkl/api/Device.cpp:135-209): logLevel, deviceIndex
 // (HIP ordinal) and stream (hipStream_t; default = the null stream).

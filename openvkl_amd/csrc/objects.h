@@ -35,6 +35,8 @@ void checkHip(hipError_t e, const char *what);
 void transformBounds(const float i2o[12], const float box[6], float out[6]);
 size_t sizeOfType(VKLDataType t);
 bool isDevicePointer(const void *p);
+// Page-locked host memory (hipHostMalloc'd or registered): DMA-able in place.
+bool isPinnedHost(const void *p);
 
 // Owning device allocation.
 struct DeviceBuffer
@@ -296,7 +298,15 @@ struct Device
   VKLLogCallback logCallback     = nullptr;
   void *logUserData              = nullptr;
 
+  // Page-locked host staging of host-pointer calls on pageable arrays (two
+  // chunk slots), allocated on first use and kept until the device is
+  // released; stagingMutex serialises such calls per device.
+  std::mutex stagingMutex;
+  void *pinned       = nullptr;
+  size_t pinnedBytes = 0;
+
   Device();
+  ~Device();
   void commit();
   void handleError(VKLError e, const std::string &msg);
   void log(int level, const std::string &msg);
