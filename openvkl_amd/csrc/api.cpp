@@ -639,6 +639,25 @@ void sampleMN(Sampler *s, uint64_t n, const vkl_vec3f *oc, float *out, uint32_t 
   if (n == 0 || m == 0) return;
   for (uint32_t k = 0; k < m; k++) (void)s->attr(attrs[k]);
   const bool temporal = s->view.grid.temporalFormat != 0 && times;
+  if (pipelines(n, oc, out)) {
+    const SamplerView sv = s->viewCopy();
+    bool binned          = false;
+    for (uint32_t k = 0; k < m && !binned; k++) {
+      BinGeometry geo;
+      binned = binnedGeometry(s, s->attr(attrs[k]), n, sampleOp(sv.filter), geo);
+    }
+    if (!binned) {
+      pipelinedHost(d, n, reinterpret_cast<const float *>(oc), temporal ? times : nullptr, out, m, "sample kernel",
+                    [&](uint64_t c, const float *x, const float *t, float *o, hipStream_t st) {
+                      for (uint32_t k = 0; k < m; k++) {
+                        const hipError_t e = launchSampleStrided(sv, s->attr(attrs[k]), attrs[k], c, x, o + k, m, st, t);
+                        if (e != hipSuccess) return e;
+                      }
+                      return hipSuccess;
+                    });
+      return;
+    }
+  }
   Staged in(oc, n * 12, d->stream, true);
   Staged tm(temporal ? times : nullptr, temporal ? n * 4 : 0, d->stream, true);
   Staged o(out, n * m * 4, d->stream, false);

@@ -59,3 +59,15 @@ def test_temporal_host_times(device):
     want = s.sample(torch.from_numpy(pts).cuda(), times=torch.from_numpy(times).cuda()).cpu().numpy()
     assert_same(s.sample(pts, times=times), want)
     s.release()
+
+
+def test_sample_m_host_arrays(device):
+    rng = np.random.default_rng(105)
+    a0 = rng.standard_normal((30, 31, 32)).astype(np.float32)
+    a1 = rng.integers(0, 255, (30, 31, 32)).astype(np.uint8)
+    vol = device.structured_regular([a0, a1])
+    pts = V.uniform_points(N, (-1, -1, -1), (32, 31, 30), seed=106)
+    s = vol.sampler(filter=vkl.VKL_FILTER_LINEAR, binned=False)
+    want = s.sample_m(torch.from_numpy(pts).cuda(), [1, 0]).cpu().numpy()
+    assert_same(s.sample_m(pts, [1, 0]), want)
+    s.release()
