@@ -133,8 +133,8 @@ def variant_config2(torch, vkl, dev, vol_t, pts, steps, warmup, U):
     x-fastest arrays (256 / 128 MiB, within the Infinity Cache) and in
     128-byte bricks (direct gathers, binned path off); half cubic; float32
     and half cubic samples and float32 cubic gradients with cubic records
-    (wavefront-cooperative stencil loads) beside the same on the
-    corner-packed records alone."""
+    and with whole cubic stencils (wavefront-cooperative stencil loads)
+    beside the same on the corner-packed records alone."""
     npts = pts.shape[0]
     out = torch.empty(npts, device="cuda")
     gout = torch.empty((npts, 3), device="cuda")
@@ -155,7 +155,11 @@ def variant_config2(torch, vkl, dev, vol_t, pts, steps, warmup, U):
              ("float32_cubic_records", vol_t, vkl.VKL_FILTER_CUBIC, False, 4, dict(packed, cubic_records=True)),
              ("float32_cubic_gradient_records", vol_t, vkl.VKL_FILTER_CUBIC, True, 4,
               dict(packed, cubic_records=True)),
-             ("half_cubic_records", half, vkl.VKL_FILTER_CUBIC, False, 2, dict(packed, cubic_records=True)))
+             ("half_cubic_records", half, vkl.VKL_FILTER_CUBIC, False, 2, dict(packed, cubic_records=True)),
+             ("float32_cubic_stencils", vol_t, vkl.VKL_FILTER_CUBIC, False, 4, dict(packed, cubic_stencils=True)),
+             ("float32_cubic_gradient_stencils", vol_t, vkl.VKL_FILTER_CUBIC, True, 4,
+              dict(packed, cubic_stencils=True)),
+             ("half_cubic_stencils", half, vkl.VKL_FILTER_CUBIC, False, 2, dict(packed, cubic_stencils=True)))
     for name, data, filt, grad, es, layout in cases:
         v = dev.structured_regular(data.contiguous(), **layout)
         sm = v.sampler(filter=filt, binned=False)

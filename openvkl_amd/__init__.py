@@ -317,14 +317,15 @@ class Device:
     def structured_regular(self, voxels, origin=(0, 0, 0), spacing=(1, 1, 1), filter=None, cell_centered=None,
                            background=None, index_origin=None, index_to_object=None, brick_shift=None,
                            corner_packed=None, dims=None, structured_steps=None, tu_indices=None, tu_times=None,
-                           data_shared=False, data_stride_factor=None, cubic_records=None):
+                           data_shared=False, data_stride_factor=None, cubic_records=None, cubic_stencils=None):
         """voxels: array [nz, ny, nx] (x fastest) or a list of them. Motion
         blur: pass dims (nx, ny, nz) and 1-D voxels with structured_steps
         values per voxel, or tu_indices / tu_times (temporallyUnstructured*;
         uint32 or uint64 indices). data_shared / data_stride_factor: how the
         voxel VKLData are made (see data()). cubic_records: also store the
         per-voxel records of the wavefront-cooperative tricubic kernels
-        ("cubicRecords", 16x the voxel memory)."""
+        ("cubicRecords", 16x the voxel memory); cubic_stencils: per cell its
+        whole stencil instead ("cubicStencils", 64x)."""
         arrs = voxels if isinstance(voxels, (list, tuple)) else [voxels]
         v = self.new_volume("structuredRegular")
         nx, ny, nz = dims if dims is not None else arrs[0].shape[::-1]
@@ -369,6 +370,8 @@ class Device:
             v.set_bool("cornerPacked", corner_packed)
         if cubic_records is not None:
             v.set_bool("cubicRecords", cubic_records)
+        if cubic_stencils is not None:
+            v.set_bool("cubicStencils", cubic_stencils)
         if index_to_object is not None:
             d = self.data(np.asarray(index_to_object, np.float32))
             v.set_data("indexToObject", d)

@@ -91,10 +91,11 @@ hipError_t launchSwizzle(const void *src, void *dst, uint32_t elemSize, const ui
 hipError_t launchPackCorners(const void *src, void *dst, uint32_t elemSize, const uint32_t dims[3],
                              hipStream_t stream);
 
-// Cubic records (16 elements per voxel, see AttrView::cubic) of an x-fastest
-// dims[0] x dims[1] x dims[2] array; dst holds 16 * voxels elements.
+// Cubic records (16 elements per voxel, see AttrView::cubic; dst holds
+// 16 * voxels elements) or, with stencils, whole stencils (64 elements per
+// cell; 64 * voxels) of an x-fastest dims[0] x dims[1] x dims[2] array.
 hipError_t launchPackCubicRecords(const void *src, void *dst, uint32_t elemSize, const uint32_t dims[3],
-                                  hipStream_t stream);
+                                  bool stencils, hipStream_t stream);
 
 // Tricubic samples (out[i * stride]) or gradients (out[3 i .. 3 i + 2]) of n
 // points of a structuredRegular volume whose attribute has cubic records.

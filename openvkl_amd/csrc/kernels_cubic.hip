@@ -39,7 +39,9 @@ constexpr uint64_t kNoStencil = ~0ull;
 // or the y / z elements a record holds clamped) take the background, as the
 // per-point path's reads do. Only points whose own y or z cell lies outside
 // the domain (no record to start from) take the per-point path.
-template <typename T, bool kIdx64, bool kGradient>
+// kWhole: the records are whole stencils (AttrView::cubicStencils), one per
+// cell; stencils whose cell lies in the domain load their own record.
+template <typename T, bool kIdx64, bool kGradient, bool kWhole>
 __global__ __launch_bounds__(StencilChunks<T>::kThreads) void kCubicRecords(GridView g, AttrView a, uint32_t attr,
                                                                              uint64_t n, const float *__restrict__ xyz,
                                                                              float *__restrict__ out,
@@ -62,11 +64,13 @@ __global__ __launch_bounds__(StencilChunks<T>::kThreads) void kCubicRecords(Grid
   const DenseSource<T, kIdx64> src(g, a, attr);
   const I3 ic       = {floorToInt(p.x), floorToInt(p.y), floorToInt(p.z)};
   const uint32_t oy = (uint32_t)(ic.y - src.ry), oz = (uint32_t)(ic.z - src.rz);
-  const bool staged = live && oy < src.ny && oz < src.nz;
-  // First stencil column and first loaded record column (nx >= 4).
+  const bool staged = live && oy < src.ny && oz < src.nz && (!kWhole || (uint32_t)(ic.x - src.rx) < src.nx);
+  // First stencil column and first loaded record column (row records:
+  // nx >= 4; whole stencils: the cell's own record).
   const int64_t sx = (int64_t)ic.x - src.rx - 1;
-  const int64_t x0 = sx < 0 ? 0 : (sx > (int64_t)src.nx - 4 ? (int64_t)src.nx - 4 : sx);
-  first[w][lane]   = staged ? (((uint64_t)oz * src.ny + oy) * src.nx + (uint64_t)x0) * K::kRecord : kNoStencil;
+  const int64_t x0 = kWhole ? sx + 1 : (sx < 0 ? 0 : (sx > (int64_t)src.nx - 4 ? (int64_t)src.nx - 4 : sx));
+  first[w][lane]   = staged ? (((uint64_t)oz * src.ny + oy) * src.nx + (uint64_t)x0) * (kWhole ? C : K::kRecord)
+                            : kNoStencil;
   __syncthreads();
   const uint4 *rec = static_cast<const uint4 *>(a.cubic);
   const uint32_t c = lane % C;
@@ -99,7 +103,8 @@ __global__ __launch_bounds__(StencilChunks<T>::kThreads) void kCubicRecords(Grid
     for (uint32_t t = 0; t < 4; t++) {
       const int64_t x   = sx + t;
       const bool vx     = x >= 0 && x < (int64_t)src.nx;
-      const uint32_t at = vx ? (uint32_t)(x - x0) : 0u;  // record within the 4 loaded
+      // record within the 4 loaded (whole stencils: the stencil's column t)
+      const uint32_t at = kWhole ? t : (vx ? (uint32_t)(x - x0) : 0u);
 #pragma unroll
       for (uint32_t cc = 0; cc < K::kRecord; cc++) {
         const uint32_t q = at * K::kRecord + cc;
@@ -157,6 +162,41 @@ __global__ __launch_bounds__(256) void kPackCubicRecords(const E *__restrict__ s
   reinterpret_cast<CubicRecord<E> *>(dst)[c] = r;
 }
 
+template <typename E>
+struct alignas(64 * sizeof(E) < 256 ? 64 * sizeof(E) : 256) StencilRecord
+{
+  E v[64];
+};
+
+// Whole stencil of cell (x, y, z): e[16 i + 4 j + k] = v(x - 1 + i, y - 1 + j,
+// z - 1 + k), coordinates clamped into the domain (clamped elements are
+// replaced by the background where the sampler reads them). One work-item
+// per cell.
+template <typename E>
+__global__ __launch_bounds__(256) void kPackCubicStencils(const E *__restrict__ src, E *__restrict__ dst, uint32_t nx,
+                                                          uint32_t ny, uint32_t nz)
+{
+  const uint64_t c = (uint64_t)blockIdx.x * 256 + threadIdx.x;
+  if (c >= (uint64_t)nx * ny * nz) return;
+  const uint32_t x = (uint32_t)(c % nx), y = (uint32_t)((c / nx) % ny), z = (uint32_t)(c / ((uint64_t)nx * ny));
+  auto clampTo = [](uint32_t v, int d, uint32_t n) {
+    const int32_t q = (int32_t)v - 1 + d;
+    return q < 0 ? 0u : (q >= (int32_t)n ? n - 1 : (uint32_t)q);
+  };
+  StencilRecord<E> r;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const uint32_t xx = clampTo(x, i, nx);
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const uint32_t yy = clampTo(y, j, ny);
+#pragma unroll
+      for (int k = 0; k < 4; k++) r.v[16 * i + 4 * j + k] = src[((uint64_t)clampTo(z, k, nz) * ny + yy) * nx + xx];
+    }
+  }
+  reinterpret_cast<StencilRecord<E> *>(dst)[c] = r;
+}
+
 namespace {
 
 template <typename T, bool kIdx64>
@@ -165,10 +205,21 @@ hipError_t launchTyped(const SamplerView &s, const AttrView &a, uint32_t attr, u
 {
   using K = StencilChunks<T>;
   const dim3 grid((unsigned)((n + K::kThreads - 1) / K::kThreads)), block(K::kThreads);
-  if (gradient)
-    hipLaunchKernelGGL((kCubicRecords<T, kIdx64, true>), grid, block, 0, st, s.grid, a, attr, n, xyz, out, stride);
-  else
-    hipLaunchKernelGGL((kCubicRecords<T, kIdx64, false>), grid, block, 0, st, s.grid, a, attr, n, xyz, out, stride);
+  if (a.cubicStencils) {
+    if (gradient)
+      hipLaunchKernelGGL((kCubicRecords<T, kIdx64, true, true>), grid, block, 0, st, s.grid, a, attr, n, xyz, out,
+                         stride);
+    else
+      hipLaunchKernelGGL((kCubicRecords<T, kIdx64, false, true>), grid, block, 0, st, s.grid, a, attr, n, xyz, out,
+                         stride);
+  } else {
+    if (gradient)
+      hipLaunchKernelGGL((kCubicRecords<T, kIdx64, true, false>), grid, block, 0, st, s.grid, a, attr, n, xyz, out,
+                         stride);
+    else
+      hipLaunchKernelGGL((kCubicRecords<T, kIdx64, false, false>), grid, block, 0, st, s.grid, a, attr, n, xyz, out,
+                         stride);
+  }
   return hipGetLastError();
 }
 
@@ -191,7 +242,8 @@ hipError_t launchCubicRecords(const SamplerView &s, const AttrView &a, uint32_t 
                               float *out, uint32_t stride, bool gradient, hipStream_t stream)
 {
   if (n == 0) return hipSuccess;
-  if (!s.grid.dense || s.grid.temporalFormat || !a.cubic || s.grid.activeSize[0] < 4) return hipErrorInvalidValue;
+  if (!s.grid.dense || s.grid.temporalFormat || !a.cubic || (!a.cubicStencils && s.grid.activeSize[0] < 4))
+    return hipErrorInvalidValue;
   switch (a.type) {
   case kTypeUchar: return launchType<uint8_t>(s, a, attr, n, xyz, out, stride, gradient, stream);
   case kTypeShort: return launchType<int16_t>(s, a, attr, n, xyz, out, stride, gradient, stream);
@@ -203,28 +255,31 @@ hipError_t launchCubicRecords(const SamplerView &s, const AttrView &a, uint32_t 
   }
 }
 
+namespace {
+
+template <typename E>
+void packTyped(bool stencils, const void *src, void *dst, const uint32_t dims[3], dim3 grid, hipStream_t stream)
+{
+  if (stencils)
+    hipLaunchKernelGGL(kPackCubicStencils<E>, grid, dim3(256), 0, stream, (const E *)src, (E *)dst, dims[0],
+                       dims[1], dims[2]);
+  else
+    hipLaunchKernelGGL(kPackCubicRecords<E>, grid, dim3(256), 0, stream, (const E *)src, (E *)dst, dims[0], dims[1],
+                       dims[2]);
+}
+
+}  // namespace
+
 hipError_t launchPackCubicRecords(const void *src, void *dst, uint32_t elemSize, const uint32_t dims[3],
-                                  hipStream_t stream)
+                                  bool stencils, hipStream_t stream)
 {
   const uint64_t voxels = (uint64_t)dims[0] * dims[1] * dims[2];
-  const dim3 grid((unsigned)((voxels + 255) / 256)), block(256);
+  const dim3 grid((unsigned)((voxels + 255) / 256));
   switch (elemSize) {
-  case 1:
-    hipLaunchKernelGGL(kPackCubicRecords<uint8_t>, grid, block, 0, stream, (const uint8_t *)src, (uint8_t *)dst,
-                       dims[0], dims[1], dims[2]);
-    break;
-  case 2:
-    hipLaunchKernelGGL(kPackCubicRecords<uint16_t>, grid, block, 0, stream, (const uint16_t *)src, (uint16_t *)dst,
-                       dims[0], dims[1], dims[2]);
-    break;
-  case 4:
-    hipLaunchKernelGGL(kPackCubicRecords<uint32_t>, grid, block, 0, stream, (const uint32_t *)src, (uint32_t *)dst,
-                       dims[0], dims[1], dims[2]);
-    break;
-  case 8:
-    hipLaunchKernelGGL(kPackCubicRecords<uint64_t>, grid, block, 0, stream, (const uint64_t *)src, (uint64_t *)dst,
-                       dims[0], dims[1], dims[2]);
-    break;
+  case 1: packTyped<uint8_t>(stencils, src, dst, dims, grid, stream); break;
+  case 2: packTyped<uint16_t>(stencils, src, dst, dims, grid, stream); break;
+  case 4: packTyped<uint32_t>(stencils, src, dst, dims, grid, stream); break;
+  case 8: packTyped<uint64_t>(stencils, src, dst, dims, grid, stream); break;
   default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

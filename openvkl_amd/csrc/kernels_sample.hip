@@ -178,7 +178,7 @@ hipError_t dispatch(const SamplerView &s, uint32_t attr, const AttrView &a, uint
 {
   // Tricubic on volumes with cubic records: wavefront-cooperative stencil
   // loads (kernels_cubic.hip), the same samples.
-  if (s.grid.dense && !s.grid.temporalFormat && a.cubic && s.grid.activeSize[0] >= 4 &&
+  if (s.grid.dense && !s.grid.temporalFormat && a.cubic && (a.cubicStencils || s.grid.activeSize[0] >= 4) &&
       (gradient ? s.gradientFilter : s.filter) == kFilterCubic)
     return launchCubicRecords(s, a, attr, n, xyz, out, stride, gradient, st);
   if (s.grid.dense) {

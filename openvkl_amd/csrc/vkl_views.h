@@ -50,8 +50,11 @@ struct AttrView
   // record of 16 elements e[4 j + k] = v(x, y - 1 + j, z - 1 + k), y and z
   // clamped into the domain. The 4x4x4 tricubic stencil of cell (x, y, z) is
   // the 4 consecutive records of x - 1 .. x + 2 (64 elements in a row).
-  // Null otherwise.
+  // With "cubicStencils" (cubicStencils = 1) instead: per cell (x, y, z), x
+  // fastest, its whole stencil e[16 i + 4 j + k] = v(x - 1 + i, y - 1 + j,
+  // z - 1 + k), coordinates clamped into the domain. Null otherwise.
   const void *cubic;
+  uint32_t cubicStencils;
 };
 
 // One inner level of the node table (levels 0..2).

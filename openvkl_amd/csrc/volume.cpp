@@ -447,9 +447,15 @@ void DenseVolume::commit()
   // "cubicRecords" (bool): also store, per voxel, the 16 elements of its
   // 4x4 (y, z) neighbourhood (AttrView::cubic; 16x the voxel memory), which
   // the tricubic stream kernels load as whole stencils.
-  const bool cubicRecords = getBool("cubicRecords", false);
-  if (cubicRecords && tformat != VKL_TEMPORAL_FORMAT_CONSTANT)
-    throw VklError("cubicRecords is not supported for temporal data", VKL_INVALID_ARGUMENT);
+  // "cubicStencils" (bool): instead, per cell its whole 4x4x4 stencil (64x
+  // the voxel memory; a float32 stencil spans exactly two cache lines, a
+  // half one one line).
+  const bool cubicRecords  = getBool("cubicRecords", false);
+  const bool cubicStencils = getBool("cubicStencils", false);
+  if ((cubicRecords || cubicStencils) && tformat != VKL_TEMPORAL_FORMAT_CONSTANT)
+    throw VklError("cubicRecords and cubicStencils are not supported for temporal data", VKL_INVALID_ARGUMENT);
+  if (cubicRecords && cubicStencils)
+    throw VklError("cubicRecords and cubicStencils cannot be combined", VKL_INVALID_ARGUMENT);
 
   const std::vector<float> bg = parseBackground(*this, grid.numAttributes);
   attrViews.clear();
@@ -483,11 +489,13 @@ void DenseVolume::commit()
     } else {
       av.data = deviceData(*this, d, st);
     }
-    if (cubicRecords) {
-      DeviceBuffer *dst = newBuffer(nvox * 16 * es);
-      checkHip(launchPackCubicRecords(compactSource(), dst->ptr, (uint32_t)es, udims, st), "cubic records");
+    if (cubicRecords || cubicStencils) {
+      DeviceBuffer *dst = newBuffer(nvox * (cubicStencils ? 64 : 16) * es);
+      checkHip(launchPackCubicRecords(compactSource(), dst->ptr, (uint32_t)es, udims, cubicStencils, st),
+               "cubic records");
       checkHip(hipStreamSynchronize(st), "cubic records");
-      av.cubic = dst->ptr;
+      av.cubic         = dst->ptr;
+      av.cubicStencils = cubicStencils ? 1u : 0u;
     }
     av.tile       = nullptr;
     av.type       = (uint32_t)d.dataType;

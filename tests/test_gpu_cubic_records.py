@@ -1,6 +1,7 @@
 """Tricubic sampling of structuredRegular volumes with cubic records
-("cubicRecords": per voxel the 16 elements of its 4x4 (y, z) neighbourhood,
-loaded a whole stencil at a time by the lanes of a wavefront together).
+("cubicRecords": per voxel the 16 elements of its 4x4 (y, z) neighbourhood;
+"cubicStencils": per cell its whole stencil), loaded a whole stencil at a
+time by the lanes of a wavefront together.
 
 Bar: samples and gradients bit-identical to the per-point tricubic path of
 the same volume without records (the same 64 taps, the same arithmetic), for
@@ -20,6 +21,8 @@ pytestmark = pytest.mark.gpu
 torch = pytest.importorskip("torch")
 
 CUBIC = vkl.VKL_FILTER_CUBIC
+KINDS = [dict(cubic_records=True), dict(cubic_stencils=True)]
+KIND_IDS = ["records", "stencils"]
 
 
 def random_volume(dtype, shape, seed):
@@ -30,11 +33,12 @@ def random_volume(dtype, shape, seed):
     return rng.integers(info.min, info.max, shape, endpoint=True).astype(dtype)
 
 
+@pytest.mark.parametrize("kind", KINDS, ids=KIND_IDS)
 @pytest.mark.parametrize("dtype", [np.float32, np.float16, np.uint8, np.int16, np.uint16, np.float64])
-def test_records_match_per_point_path_and_oracle(device, dtype):
+def test_records_match_per_point_path_and_oracle(device, dtype, kind):
     data = random_volume(dtype, (33, 26, 41), 61)
     plain = device.structured_regular(data)
-    rec = device.structured_regular(data, cubic_records=True)
+    rec = device.structured_regular(data, **kind)
     grid = O.Grid.dense(data)
     grid.set_sampler(O.CUBIC)
     # points over the domain and a voxel beyond it on every side: interior
@@ -50,13 +54,14 @@ def test_records_match_per_point_path_and_oracle(device, dtype):
     check_cubic(grid, g[:20000].cpu().numpy(), sub[:20000], gradient=True)
 
 
+@pytest.mark.parametrize("kind", KINDS, ids=KIND_IDS)
 @pytest.mark.parametrize("layout", [dict(corner_packed=True), dict(brick_shift=(2, 2, 1))])
-def test_records_over_other_base_layouts(device, layout):
+def test_records_over_other_base_layouts(device, layout, kind):
     """The records sit beside the volume's own layout, which the per-point
     fallback reads."""
     data = random_volume(np.float32, (29, 37, 45), 67)
     plain = device.structured_regular(data)
-    rec = device.structured_regular(data, cubic_records=True, **layout)
+    rec = device.structured_regular(data, **kind, **layout)
     pts = torch.from_numpy(V.uniform_points(200000, (-1.5, -1.5, -1.5), (45.5, 37.5, 29.5))).cuda()
     sp, sr = plain.sampler(filter=CUBIC), rec.sampler(filter=CUBIC)
     assert_same(sr.sample(pts).cpu().numpy(), sp.sample(pts).cpu().numpy())
@@ -67,13 +72,14 @@ def test_records_over_other_base_layouts(device, layout):
                     plain.sampler(filter=f).sample(pts).cpu().numpy())
 
 
-def test_records_transform_cell_centered_background(device):
+@pytest.mark.parametrize("kind", KINDS, ids=KIND_IDS)
+def test_records_transform_cell_centered_background(device, kind):
     rng = np.random.default_rng(71)
     data = rng.standard_normal((17, 18, 19)).astype(np.float32)
     i2o = np.array([0.9, 0.1, 0.0, -0.2, 1.1, 0.05, 0.0, 0.3, 0.7, 4.0, -2.0, 1.0], np.float32)  # column-major
     kw = dict(cell_centered=True, background=-3.5, index_to_object=i2o)
     plain = device.structured_regular(data, **kw)
-    rec = device.structured_regular(data, cubic_records=True, **kw)
+    rec = device.structured_regular(data, **kind, **kw)
     rm = np.array([i2o[0], i2o[3], i2o[6], i2o[1], i2o[4], i2o[7], i2o[2], i2o[5], i2o[8], i2o[9], i2o[10], i2o[11]],
                   np.float32)
     grid = O.Grid.dense(data, i2o=rm, cell_centered=True, background=[-3.5])
@@ -87,7 +93,8 @@ def test_records_transform_cell_centered_background(device):
     check_cubic(grid, got, pts)
 
 
-def test_records_sample_m_scalar_and_wide(device):
+@pytest.mark.parametrize("kind", KINDS, ids=KIND_IDS)
+def test_records_sample_m_scalar_and_wide(device, kind):
     """SampleM writes strided outputs through the same kernel; the scalar and
     wide calls equal the stream call."""
     rng = np.random.default_rng(73)
@@ -95,7 +102,7 @@ def test_records_sample_m_scalar_and_wide(device):
     a1 = rng.integers(0, 255, (20, 21, 22)).astype(np.uint8)
     a2 = rng.standard_normal((20, 21, 22)).astype(np.float16)
     plain = device.structured_regular([a0, a1, a2])
-    rec = device.structured_regular([a0, a1, a2], cubic_records=True)
+    rec = device.structured_regular([a0, a1, a2], **kind)
     pts = V.uniform_points(30000, (-1, -1, -1), (22, 21, 20))
     sp, sr = plain.sampler(filter=CUBIC), rec.sampler(filter=CUBIC)
     assert_same(sr.sample_m(pts, [2, 0, 1]), sp.sample_m(pts, [2, 0, 1]))
@@ -134,7 +141,8 @@ def test_records_rejected_for_temporal_data(device):
     assert code == vkl.VKL_INVALID_ARGUMENT and "cubicRecords" in msg
 
 
-def test_full_size_512_cubic_records(device):
+@pytest.mark.parametrize("kind", KINDS, ids=KIND_IDS)
+def test_full_size_512_cubic_records(device, kind):
     """512^3 float32, corner-packed with cubic records (the benchmark's
     cubic volume, 8 GiB of records: element offsets past 2^32 bytes): equal
     to the per-point path on 2^22 points, oracle bar on a subsample."""
@@ -142,7 +150,7 @@ def test_full_size_512_cubic_records(device):
     rng = np.random.default_rng(83)
     data = rng.random((n, n, n), dtype=np.float32)
     t = torch.from_numpy(data).cuda()
-    rec = device.structured_regular(t, corner_packed=True, cubic_records=True)
+    rec = device.structured_regular(t, corner_packed=True, **kind)
     sr = rec.sampler(filter=CUBIC)
     pts = torch.rand((1 << 22, 3), device="cuda", dtype=torch.float32) * 513.0 - 1.0
     got = sr.sample(pts)
@@ -161,7 +169,8 @@ def test_full_size_512_cubic_records(device):
     check_cubic(grid, got[::512].cpu().numpy(), sub)
 
 
-def test_records_beside_64bit_base_layout(device):
+@pytest.mark.parametrize("kind", KINDS, ids=KIND_IDS)
+def test_records_beside_64bit_base_layout(device, kind):
     """uint8 corner-packed 820^3 (4.4e9 elements: the per-point path uses
     64-bit offsets) with cubic records (8.8 GB): equal to the per-point path,
     including stencils on the far faces, oracle bar on a subsample."""
@@ -173,7 +182,7 @@ def test_records_beside_64bit_base_layout(device):
     pts = V.uniform_points(200000, (-1, -1, -1), (n, n, n), seed=99)
     pts[:40000] = V.uniform_points(40000, (n - 6, n - 6, n - 6), (n + 0.5, n + 0.5, n + 0.5), seed=100)
     p = torch.from_numpy(pts).cuda()
-    rec = device.structured_regular(t, corner_packed=True, cubic_records=True)
+    rec = device.structured_regular(t, corner_packed=True, **kind)
     sr = rec.sampler(filter=CUBIC)
     got = sr.sample(p)
     gg = sr.gradient(p)
@@ -188,3 +197,35 @@ def test_records_beside_64bit_base_layout(device):
     grid = O.Grid.dense(data)
     grid.set_sampler(O.CUBIC)
     check_cubic(grid, got[:20000].cpu().numpy(), pts[:20000])
+
+
+@pytest.mark.parametrize("kind", KINDS, ids=KIND_IDS)
+def test_records_tiny_volumes(device, kind):
+    """Volumes narrower than a stencil: whole stencils still load their
+    records; row records (which need 4 columns) leave them to the per-point
+    kernel."""
+    rng = np.random.default_rng(107)
+    for shape in ((6, 5, 3), (4, 2, 7), (1, 1, 1)):
+        data = rng.standard_normal(shape).astype(np.float32)
+        plain = device.structured_regular(data)
+        rec = device.structured_regular(data, **kind)
+        nz, ny, nx = shape
+        pts = V.uniform_points(20000, (-1.5, -1.5, -1.5), (nx + 0.5, ny + 0.5, nz + 0.5), seed=108)
+        sp, sr = plain.sampler(filter=CUBIC), rec.sampler(filter=CUBIC)
+        assert_same(sr.sample(pts), sp.sample(pts))
+        assert_same(sr.gradient(pts), sp.gradient(pts))
+        sp.release(), sr.release(), plain.release(), rec.release()
+
+
+def test_records_and_stencils_rejected_together(device):
+    v = device.new_volume("structuredRegular")
+    v.set_vec3i("dimensions", 4, 4, 4)
+    d = device.data(np.zeros(64, np.float32))
+    v.set_data("data", d)
+    d.release()
+    v.set_bool("cubicRecords", True)
+    v.set_bool("cubicStencils", True)
+    vkl.lib().vklCommit(v.h)
+    code, msg = device.last_error()
+    device.errors.clear()
+    assert code == vkl.VKL_INVALID_ARGUMENT and "cannot be combined" in msg
