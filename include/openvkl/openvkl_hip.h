@@ -176,7 +176,9 @@ OPENVKL_INTERFACE void vklGetSampleBatchPlanLaunch(VKLSampleBatchPlan plan, unsi
 /* Kernel launches one execution of the plan makes: one per group of batches.
  * Batches are grouped per voxel source type and filter, or, where the
  * autotuner measured it faster, with all structuredRegular volumes of 32-bit
- * voxel offsets and one filter in one group whatever their element types. */
+ * voxel offsets and one filter in one group whatever their element types;
+ * each tricubic batch on a volume with cubicRecords / cubicStencils is a
+ * launch of its own (the stencil-cooperative kernel of vklComputeSampleN). */
 OPENVKL_INTERFACE unsigned int vklGetSampleBatchPlanLaunchCount(VKLSampleBatchPlan plan);
 OPENVKL_INTERFACE void vklReleaseSampleBatchPlan(VKLSampleBatchPlan plan);
 

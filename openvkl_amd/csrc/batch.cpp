@@ -17,6 +17,11 @@
 // points rounded to a power of two) and, if OPENVKL_HIP_TUNE_CACHE names a
 // file, persisted there as text lines "signature blocks threads tile merged".
 // Without autotuning the per-specialisation grouping is used.
+//
+// Tricubic batches on structuredRegular volumes with cubic records do not
+// join the groups: each runs the stencil-cooperative kernel of
+// vklComputeSampleN (its lanes load whole stencils together), captured in the
+// same graph after the group launches.
 
 #include <hip/hip_runtime.h>
 
@@ -29,6 +34,7 @@
 #include <sstream>
 
 #include "batch.h"
+#include "kernels.h"
 #include "objects.h"
 #include "openvkl/openvkl_hip.h"
 
@@ -45,6 +51,7 @@ struct SampleBatchPlan
   uint32_t numItems = 0;
   uint64_t total    = 0;
   BatchLaunch launch{};
+  std::vector<BatchItem> direct;  // tricubic batches on cubic records
   hipGraph_t graph        = nullptr;
   hipGraphExec_t graphExec = nullptr;
   std::string signature;
@@ -99,10 +106,23 @@ std::vector<BatchLaunch> candidates(int cus, bool canMerge)
   return c;
 }
 
+// Batches the cubic-records kernel serves (as the stream call would).
+bool onCubicRecords(const BatchItem &it)
+{
+  const GridView &g = it.sampler.grid;
+  return it.sampler.filter == kFilterCubic && it.attr.cubic && g.dense && !g.temporalFormat &&
+         (it.attr.cubicStencils || g.activeSize[0] >= 4);
+}
+
 hipError_t launchPlan(const SampleBatchPlan &p, const BatchLaunch &c, hipStream_t st)
 {
   const std::vector<BatchGroup> &g = p.groups[c.merged];
-  return launchSampleBatch(g.data(), (uint32_t)g.size(), c, st);
+  hipError_t e = g.empty() ? hipSuccess : launchSampleBatch(g.data(), (uint32_t)g.size(), c, st);
+  for (const BatchItem &it : p.direct) {
+    if (e != hipSuccess) break;
+    e = launchCubicRecords(it.sampler, it.attr, it.attrIndex, it.count, it.xyz, it.out, 1, false, st);
+  }
+  return e;
 }
 
 BatchLaunch tune(SampleBatchPlan &p, hipStream_t st)
@@ -183,14 +203,23 @@ OPENVKL_INTERFACE VKLSampleBatchPlan vklNewSampleBatchPlan(unsigned int numBatch
         throw VklError("sample batch: point and sample arrays must be device memory");
       }
     }
+    // Tricubic batches on cubic records run on their own (see the top).
+    std::vector<unsigned> grouped;
+    for (unsigned i = 0; i < numBatches; i++) {
+      if (onCubicRecords(all[i])) {
+        if (all[i].count) p->direct.push_back(all[i]);
+      } else {
+        grouped.push_back(i);
+      }
+    }
     std::ostringstream sig;
-    sig << "b" << numBatches;
+    sig << "b" << numBatches << "d" << p->direct.size();
     for (int merged = 0; merged < 2; merged++) {
       // Group the batches (order kept within a group): per kernel
       // specialisation, or with the structuredRegular 32-bit-offset batches
       // of a filter in one group.
       std::map<uint32_t, std::vector<unsigned>> byKey;
-      for (unsigned i = 0; i < numBatches; i++) {
+      for (unsigned i : grouped) {
         const BatchItem &it = all[i];
         const int f         = it.sampler.filter;
         byKey[merged ? groupKey(it.sampler.grid, it.attr, f) : sourceKey(it.sampler.grid, it.attr, f)].push_back(i);
@@ -290,7 +319,7 @@ OPENVKL_INTERFACE unsigned int vklGetSampleBatchPlanLaunchCount(VKLSampleBatchPl
 {
   SampleBatchPlan *p = reinterpret_cast<SampleBatchPlan *>(plan);
   if (!p) return 0;
-  unsigned int n = 0;
+  unsigned int n = (unsigned int)p->direct.size();
   for (const BatchGroup &g : p->groups[p->launch.merged]) n += (g.total && g.numItems) ? 1u : 0u;
   return n;
 }

@@ -112,19 +112,31 @@ def test_records_sample_m_scalar_and_wide(device, kind):
         assert_same(sr.sample_wide(pts[:w], width=w), ref[:w])
 
 
-def test_records_in_batch_plan_equal_stream_call(device):
-    """Batch plans sample cubic batches per point; the stream call on a
-    volume with records gives the same values."""
+@pytest.mark.parametrize("autotune", [0, 1, 2])
+def test_records_in_batch_plan_equal_stream_call(device, autotune):
+    """Batch plans run tricubic batches on cubic records through the
+    stencil-cooperative kernel and the rest through the persistent batch
+    kernel: every batch equals its stream call bit for bit, an empty batch
+    included, and each cubic-records batch is one launch of its own."""
     data = random_volume(np.float32, (40, 40, 40), 79)
     rec = device.structured_regular(data, cubic_records=True)
-    s = rec.sampler(filter=CUBIC)
-    pts = torch.from_numpy(V.uniform_points(100000, (-1, -1, -1), (40, 40, 40))).cuda()
-    out = torch.empty(100000, device="cuda")
-    plan = vkl.SampleBatchPlan(device, [s], [pts], [out], autotune=False)
+    whole = device.structured_regular(data.astype(np.float16), cubic_stencils=True)
+    plain = device.structured_regular(data, corner_packed=True)
+    samplers = [rec.sampler(filter=CUBIC), whole.sampler(filter=CUBIC), plain.sampler(filter=CUBIC),
+                rec.sampler(filter=vkl.VKL_FILTER_LINEAR), rec.sampler(filter=CUBIC)]
+    counts = [100000, 70000, 50000, 30000, 0]
+    pts = [torch.from_numpy(V.uniform_points(max(c, 1), (-1, -1, -1), (40, 40, 40), seed=90 + k)).cuda()[:c]
+           for k, c in enumerate(counts)]
+    outs = [torch.full((c,), -7.0, device="cuda") for c in counts]
+    plan = vkl.SampleBatchPlan(device, samplers, pts, outs, autotune=autotune)
     plan.execute()
     torch.cuda.synchronize()
+    assert plan.launch_count() >= 2 + 1  # two records batches + at least one group
     plan.release()
-    assert_same(out.cpu().numpy(), s.sample(pts).cpu().numpy())
+    for s, p, o in zip(samplers, pts, outs):
+        if p.shape[0]:
+            assert_same(o.cpu().numpy(), s.sample(p).cpu().numpy())
+        s.release()
 
 
 def test_records_rejected_for_temporal_data(device):
