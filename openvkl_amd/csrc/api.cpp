@@ -1649,7 +1649,7 @@ OPENVKL_INTERFACE void vklIterateIntervalsBatch(const VKLIntervalIteratorContext
   Staged out(intervals, (uint64_t)numRays * maxIntervalsPerRay * sizeof(VKLInterval), st, false);
   Staged cnt(numIntervals, numRays * 4ull, st, false);
   const IteratorContextView hv = ctx->refreshSampler();
-  checkHip(launchIntervalBatch(hv.maxIteratorDepth, ctx->deviceView.as<IteratorContextView>(), numRays, o.as<float>(), d.as<float>(), t.as<float>(),
+  checkHip(launchIntervalBatch(hv.maxIteratorDepth, denseQueryOf(hv.sampler.grid), ctx->deviceView.as<IteratorContextView>(), numRays, o.as<float>(), d.as<float>(), t.as<float>(),
                                out.as<IntervalOut>(), maxIntervalsPerRay, cnt.as<uint32_t>(), st),
            "interval batch");
   out.copyOut();

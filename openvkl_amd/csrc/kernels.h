@@ -136,9 +136,17 @@ hipError_t launchHitNext(const IteratorContextView &hostView, const IteratorCont
 // Batched iteration: one ray per thread; ray r writes up to maxOut results at
 // out[r*maxOut], count[r] = number written. org / dir: 3 floats per ray;
 // tRanges: 2 floats per ray (null = [0, inf)); times may be null.
-hipError_t launchIntervalBatch(uint32_t depth, const IteratorContextView *c, uint32_t numRays, const float *org, const float *dir,
-                               const float *tRanges, IntervalOut *out, uint32_t maxOut, uint32_t *count,
-                               hipStream_t stream);
+// denseQuery: the context's queries use the dense node numbering
+// (denseQueryOf); the kernel is specialised on it.
+hipError_t launchIntervalBatch(uint32_t depth, bool denseQuery, const IteratorContextView *c, uint32_t numRays,
+                               const float *org, const float *dir, const float *tRanges, IntervalOut *out,
+                               uint32_t maxOut, uint32_t *count, hipStream_t stream);
+// Whether iterator queries on g use the dense node numbering (a
+// structuredRegular volume within one level-1 node, VdbQueryVoxelDense.ih:48-57).
+inline bool denseQueryOf(const GridView &g)
+{
+  return g.dense != 0 && g.activeSize[0] <= kRes[1] && g.activeSize[1] <= kRes[1] && g.activeSize[2] <= kRes[1];
+}
 // Hit batches run a persistent schedule: `lanes` (batchLaneCount) lanes take
 // rays from the work counter `work`, which launchSetWork must set to `lanes`
 // on the same stream before each batch launch.

@@ -225,7 +225,7 @@ hipError_t launchHitNext(const IteratorContextView &hostView, const IteratorCont
 // depth 2 (82 VGPRs, 5 waves per SIMD) asking for 6 waves costs a 12-byte
 // spill and measured 11 % faster on the 4096^3 vdb; at depth 3 (100 VGPRs,
 // 4 waves) 5 waves cost a 20-byte spill and measured 6 % faster.
-template <int kDepth>
+template <int kDepth, int kQuery>
 __global__ __launch_bounds__(kIterBlock) __attribute__((amdgpu_waves_per_eu(kDepth <= 2 ? 6 : 5)))
 void kIntervalBatch(const IteratorContextView *__restrict__ cp, uint32_t numRays,
                                                               const float *__restrict__ org,
@@ -243,7 +243,7 @@ void kIntervalBatch(const IteratorContextView *__restrict__ cp, uint32_t numRays
   uint32_t n = 0;
   IntervalOut o;
   IntervalOut *dst = out + (uint64_t)r * maxOut;
-  while (n < maxOut && intervalNext<false, kDepth>(c, st, o)) dst[n++] = o;
+  while (n < maxOut && intervalNext<false, kDepth, kQuery>(c, st, o)) dst[n++] = o;
   count[r] = n;
 }
 
@@ -257,18 +257,45 @@ hipError_t launchSetWork(uint32_t *work, uint32_t value, hipStream_t st)
 
 uint32_t batchLaneCount(uint32_t numRays, int cus) { return batchLanes(numRays, cus); }
 
-hipError_t launchIntervalBatch(uint32_t depth, const IteratorContextView *c, uint32_t numRays, const float *org,
-                               const float *dir, const float *tRanges, IntervalOut *out, uint32_t maxOut,
-                               uint32_t *count, hipStream_t st)
+namespace {
+
+template <int kQuery>
+void intervalBatchDepth(uint32_t depth, dim3 grid, dim3 block, hipStream_t st, const IteratorContextView *c,
+                        uint32_t numRays, const float *org, const float *dir, const float *tRanges, IntervalOut *out,
+                        uint32_t maxOut, uint32_t *count)
+{
+  switch (depth) {
+  case 0:
+    hipLaunchKernelGGL((kIntervalBatch<0, kQuery>), grid, block, 0, st, c, numRays, org, dir, tRanges, out, maxOut,
+                       count);
+    break;
+  case 1:
+    hipLaunchKernelGGL((kIntervalBatch<1, kQuery>), grid, block, 0, st, c, numRays, org, dir, tRanges, out, maxOut,
+                       count);
+    break;
+  case 2:
+    hipLaunchKernelGGL((kIntervalBatch<2, kQuery>), grid, block, 0, st, c, numRays, org, dir, tRanges, out, maxOut,
+                       count);
+    break;
+  default:
+    hipLaunchKernelGGL((kIntervalBatch<3, kQuery>), grid, block, 0, st, c, numRays, org, dir, tRanges, out, maxOut,
+                       count);
+    break;
+  }
+}
+
+}  // namespace
+
+hipError_t launchIntervalBatch(uint32_t depth, bool denseQuery, const IteratorContextView *c, uint32_t numRays,
+                               const float *org, const float *dir, const float *tRanges, IntervalOut *out,
+                               uint32_t maxOut, uint32_t *count, hipStream_t st)
 {
   if (numRays == 0) return hipSuccess;
   const dim3 grid((numRays + kIterBlock - 1) / kIterBlock), block(kIterBlock);
-  switch (depth) {
-  case 0: hipLaunchKernelGGL(kIntervalBatch<0>, grid, block, 0, st, c, numRays, org, dir, tRanges, out, maxOut, count); break;
-  case 1: hipLaunchKernelGGL(kIntervalBatch<1>, grid, block, 0, st, c, numRays, org, dir, tRanges, out, maxOut, count); break;
-  case 2: hipLaunchKernelGGL(kIntervalBatch<2>, grid, block, 0, st, c, numRays, org, dir, tRanges, out, maxOut, count); break;
-  default: hipLaunchKernelGGL(kIntervalBatch<3>, grid, block, 0, st, c, numRays, org, dir, tRanges, out, maxOut, count); break;
-  }
+  if (denseQuery)
+    intervalBatchDepth<1>(depth, grid, block, st, c, numRays, org, dir, tRanges, out, maxOut, count);
+  else
+    intervalBatchDepth<0>(depth, grid, block, st, c, numRays, org, dir, tRanges, out, maxOut, count);
   return hipGetLastError();
 }
 

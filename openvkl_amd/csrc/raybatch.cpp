@@ -124,7 +124,7 @@ OPENVKL_INTERFACE VKLRayBatchPlan vklNewRayBatchPlan(unsigned int numBatches, co
                                   counts[i], work + i, lanes, b),
                    "ray batch hits");
         } else {
-          checkHip(launchIntervalBatch(hv.maxIteratorDepth, dv + i, numRays[i], o, d, tr,
+          checkHip(launchIntervalBatch(hv.maxIteratorDepth, denseQueryOf(hv.sampler.grid), dv + i, numRays[i], o, d, tr,
                                        static_cast<IntervalOut *>(results[i]), maxPerRay[i], counts[i], b),
                    "ray batch intervals");
         }

@@ -1260,16 +1260,21 @@ VKL_DEV void intervalInit(const IteratorContextView &c, IntervalState &it, F3 or
   ddaInit<kDepth>(it.dda, org, dir, miss ? 0.f : ti.lower, c.maxIteratorDepth);
 }
 
-// VdbIterator_iterateIntervalInternal, VdbIterator.ispc:96-159.
-template <bool kNodeCache = false, int kDepth = -1>
+// Whether a context's queries use the dense node numbering: dense node
+// indices assume one level-1 node (VdbQueryVoxelDense.ih:48-57); larger
+// volumes use the general query, which stays inside the node table.
+VKL_DEV bool denseQuery(const GridView &g)
+{
+  return g.dense != 0 && g.activeSize[0] <= kRes[1] && g.activeSize[1] <= kRes[1] && g.activeSize[2] <= kRes[1];
+}
+
+// VdbIterator_iterateIntervalInternal, VdbIterator.ispc:96-159. kQuery: 0
+// the general query, 1 the dense one, -1 chosen per context at run time.
+template <bool kNodeCache = false, int kDepth = -1, int kQuery = -1>
 VKL_DEV bool intervalNext(const IteratorContextView &c, IntervalState &it, IntervalOut &out)
 {
   if (!(it.dda.t <= it.tMax)) return false;
-  // Dense node indices assume one level-1 node (VdbQueryVoxelDense.ih:48-57);
-  // larger volumes use the general query, which stays inside the node table.
-  const GridView &g = c.sampler.grid;
-  const bool dense  = g.dense != 0 && g.activeSize[0] <= kRes[1] && g.activeSize[1] <= kRes[1] &&
-                     g.activeSize[2] <= kRes[1];
+  const bool dense = kQuery >= 0 ? kQuery == 1 : denseQuery(c.sampler.grid);
   VoxelDesc v{0, true, {0.f, 0.f}};
   // Every step advances t by a positive tDelta; a ray whose step makes no
   // progress (infinite direction components) is ended after kMaxDdaSteps.
