@@ -38,7 +38,7 @@ constexpr int kMaxShadedHits = 8;
 // As for the hit batches: 3 waves per SIMD (168 VGPRs instead of 204 at 2
 // waves, a ~50-byte spill) made a 1080p frame 35 % faster. Cubic filters
 // keep the compiler's choice.
-template <typename Src, int kFilter>
+template <typename Src, int kFilter, int kQuery>
 __global__ __launch_bounds__(kIterBlock) __attribute__((amdgpu_waves_per_eu(kFilter == kFilterCubic ? 1 : 3)))
 void kHitRender(const IteratorContextView *__restrict__ cp, AttrView a,
                                                           HitRenderArgs m)
@@ -72,7 +72,7 @@ void kHitRender(const IteratorContextView *__restrict__ cp, AttrView a,
       primary = false;
     } else {
       HitOut h;
-      const bool got = hitNext(c, smp, st, h);
+      const bool got = hitNext<decltype(smp), kQuery>(c, smp, st, h);
       if (primary) {
         if (got) {
           hitT[hits][lane] = h.t;
@@ -141,18 +141,30 @@ void kHitRender(const IteratorContextView *__restrict__ cp, AttrView a,
   if (m.count) m.count[r] = hits;
 }
 
+// The kernel is specialised on the context's query kind (denseQueryOf):
+// structuredRegular volumes within one level-1 node carry no node-table
+// query.
 struct HitRenderLaunch
 {
   const IteratorContextView *c;
   AttrView a;
   HitRenderArgs m;
   hipStream_t st;
+  bool denseQuery;
+  template <typename Src, int kFilter, int kQuery>
+  hipError_t run() const
+  {
+    hipLaunchKernelGGL((kHitRender<Src, kFilter, kQuery>), dim3((m.numRays + kIterBlock - 1) / kIterBlock),
+                       dim3(kIterBlock), 0, st, c, a, m);
+    return hipGetLastError();
+  }
   template <typename Src, int kFilter>
   hipError_t operator()() const
   {
-    hipLaunchKernelGGL((kHitRender<Src, kFilter>), dim3((m.numRays + kIterBlock - 1) / kIterBlock),
-                       dim3(kIterBlock), 0, st, c, a, m);
-    return hipGetLastError();
+    if constexpr (IsDenseSource<Src>::value)
+      return denseQuery ? run<Src, kFilter, 1>() : run<Src, kFilter, 0>();
+    else
+      return run<Src, kFilter, 0>();
   }
 };
 

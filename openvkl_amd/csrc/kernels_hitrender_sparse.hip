@@ -10,7 +10,7 @@ hipError_t launchHitRenderSparse(const IteratorContextView &hostView, const Iter
   const GridView &g = hostView.sampler.grid;
   if (g.temporalFormat) return launchHitRenderTemporal(hostView, c, a, m, st);
   if (g.leafAccess) return launchHitRenderObserved(hostView, c, a, m, st);
-  return dispatchSource<kSparseKinds>(g, a, hostView.sampler.filter, HitRenderLaunch{c, a, m, st});
+  return dispatchSource<kSparseKinds>(g, a, hostView.sampler.filter, HitRenderLaunch{c, a, m, st, denseQueryOf(hostView.sampler.grid)});
 }
 
 }  // namespace vklhip

@@ -8,7 +8,7 @@ hipError_t launchHitRenderTemporal(const IteratorContextView &hostView, const It
                                    const AttrView &a, const HitRenderArgs &m, hipStream_t st)
 {
   return dispatchSource<kTemporalKinds>(hostView.sampler.grid, a, hostView.sampler.filter,
-                                        HitRenderLaunch{c, a, m, st});
+                                        HitRenderLaunch{c, a, m, st, denseQueryOf(hostView.sampler.grid)});
 }
 
 }  // namespace vklhip

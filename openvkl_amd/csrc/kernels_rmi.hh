@@ -52,11 +52,12 @@ constexpr int samplesAhead()
 
 // rgba[4r..4r+3] = composited colour and accumulated opacity of ray r.
 // structuredRegular sources: 5 waves per SIMD (96 VGPRs instead of 126 at 4
-// waves, a spill of 128 bytes) measured 4 % faster on a 1080p frame of the
-// 512^3 volume; vdb sources keep the compiler's choice (no spill), which
-// with 4 samples in flight measured 14 % faster on the 4096^3 vdb; cubic
-// filters keep the compiler's choice.
-template <typename Src, int kFilter, int kDepth>
+// waves; with the dense query alone a 12-byte spill, 128 bytes with both
+// queries) measured fastest on a 1080p frame of the 512^3 volume (6 waves:
+// 80 VGPRs and a 112-byte spill, 1-7 % slower); vdb sources keep the
+// compiler's choice (no spill), which with 4 samples in flight measured 14 %
+// faster on the 4096^3 vdb; cubic filters keep the compiler's choice.
+template <typename Src, int kFilter, int kDepth, int kQuery>
 __global__ __launch_bounds__(kIterBlock) __attribute__((
     amdgpu_waves_per_eu(kFilter == kFilterCubic || !IsDenseSource<Src>::value ? 1 : 5)))
 void kRayMarchIntervals(const IteratorContextView *__restrict__ cp,
@@ -75,7 +76,7 @@ void kRayMarchIntervals(const IteratorContextView *__restrict__ cp,
   uint32_t taken = 0;
   IntervalOut iv;
   constexpr int kAhead = samplesAhead<Src>();
-  while (alpha < 0.99f && taken < m.maxSamples && intervalNext<false, kDepth>(c, st, iv)) {
+  while (alpha < 0.99f && taken < m.maxSamples && intervalNext<false, kDepth, kQuery>(c, st, iv)) {
     const float dtNominal = iv.nominalDeltaT / m.samplingRate;
     float lo = iv.tLower, hi = minf_(iv.tLower + dtNominal, iv.tUpper);
     if constexpr (kAhead == 1) {
@@ -143,7 +144,9 @@ void kRayMarchIntervals(const IteratorContextView *__restrict__ cp,
 }
 
 // Launches the instantiation for the context's iterator depth on vdb
-// sources (the DDA keeps only the levels it walks).
+// sources (the DDA keeps only the levels it walks) and for its query kind
+// (denseQueryOf: structuredRegular volumes within one level-1 node use the
+// dense node numbering, and their kernels then carry no node-table query).
 struct RayMarchIntervalsLaunch
 {
   const IteratorContextView *c;
@@ -151,11 +154,12 @@ struct RayMarchIntervalsLaunch
   RayMarchIntervalsArgs m;
   uint32_t depth;
   hipStream_t st;
-  template <typename Src, int kFilter, int kDepth>
+  bool denseQuery;
+  template <typename Src, int kFilter, int kDepth, int kQuery>
   hipError_t run() const
   {
-    hipLaunchKernelGGL((kRayMarchIntervals<Src, kFilter, kDepth>), dim3((m.numRays + kIterBlock - 1) / kIterBlock),
-                       dim3(kIterBlock), 0, st, c, a, m);
+    hipLaunchKernelGGL((kRayMarchIntervals<Src, kFilter, kDepth, kQuery>),
+                       dim3((m.numRays + kIterBlock - 1) / kIterBlock), dim3(kIterBlock), 0, st, c, a, m);
     return hipGetLastError();
   }
   template <typename Src, int kFilter>
@@ -164,13 +168,13 @@ struct RayMarchIntervalsLaunch
     // structuredRegular sources keep the depth-generic kernel, which
     // measured faster for them (5.1 against 5.6 ms per 1080p frame).
     if constexpr (IsDenseSource<Src>::value) {
-      return run<Src, kFilter, -1>();
+      return denseQuery ? run<Src, kFilter, -1, 1>() : run<Src, kFilter, -1, 0>();
     } else {
       switch (depth) {
-      case 0: return run<Src, kFilter, 0>();
-      case 1: return run<Src, kFilter, 1>();
-      case 2: return run<Src, kFilter, 2>();
-      default: return run<Src, kFilter, 3>();
+      case 0: return run<Src, kFilter, 0, 0>();
+      case 1: return run<Src, kFilter, 1, 0>();
+      case 2: return run<Src, kFilter, 2, 0>();
+      default: return run<Src, kFilter, 3, 0>();
       }
     }
   }

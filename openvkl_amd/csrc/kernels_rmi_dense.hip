@@ -9,7 +9,7 @@ hipError_t launchRayMarchIntervals(const IteratorContextView &hostView, const It
   if (m.numRays == 0) return hipSuccess;
   const GridView &g = hostView.sampler.grid;
   if (!g.dense || g.temporalFormat) return launchRayMarchIntervalsSparse(hostView, c, a, m, st);
-  return dispatchSource<kDenseKinds>(g, a, hostView.sampler.filter, RayMarchIntervalsLaunch{c, a, m, hostView.maxIteratorDepth, st});
+  return dispatchSource<kDenseKinds>(g, a, hostView.sampler.filter, RayMarchIntervalsLaunch{c, a, m, hostView.maxIteratorDepth, st, denseQueryOf(hostView.sampler.grid)});
 }
 
 }  // namespace vklhip

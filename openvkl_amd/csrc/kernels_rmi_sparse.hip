@@ -7,7 +7,7 @@ hipError_t launchRayMarchIntervalsSparse(const IteratorContextView &hostView, co
                                          const AttrView &a, const RayMarchIntervalsArgs &m, hipStream_t st)
 {
   return dispatchSource<kSparseKinds | kTemporalKinds | kSparseObservedKinds>(
-      hostView.sampler.grid, a, hostView.sampler.filter, RayMarchIntervalsLaunch{c, a, m, hostView.maxIteratorDepth, st});
+      hostView.sampler.grid, a, hostView.sampler.filter, RayMarchIntervalsLaunch{c, a, m, hostView.maxIteratorDepth, st, denseQueryOf(hostView.sampler.grid)});
 }
 
 }  // namespace vklhip

@@ -1414,7 +1414,7 @@ VKL_DEV void hitInit(const IteratorContextView &c, HitState &h, F3 origin, F3 di
 }
 
 // DefaultHitIterator_iterateHit, DefaultIterator.ispc:168-258.
-template <typename S>
+template <typename S, int kQuery = -1>
 VKL_DEV bool hitNext(const IteratorContextView &c, const S &smp, HitState &h, HitOut &hit)
 {
   if (c.numValues == 0) return false;
@@ -1426,7 +1426,7 @@ VKL_DEV bool hitNext(const IteratorContextView &c, const S &smp, HitState &h, Hi
     bool have       = !need;
     if (need) {
       IntervalOut iv;
-      have = intervalNext<true, hitDepthOf<typename S::Source>()>(c, h.iv, iv);
+      have = intervalNext<true, hitDepthOf<typename S::Source>(), kQuery>(c, h.iv, iv);
       if (have) {
         h.curLower         = iv.tLower;
         h.curUpper         = iv.tUpper;
