@@ -96,7 +96,7 @@ __global__ __launch_bounds__(1) void kHitNext(const IteratorContextView *__restr
 // waves per SIMD (128 VGPRs instead of ~155 at 3 waves) costs a ~30-byte
 // spill and measured 13 % faster on the 4096^3 vdb. Cubic filters keep the
 // compiler's choice.
-template <typename Src, int kFilter>
+template <typename Src, int kFilter, int kQuery>
 __global__ __launch_bounds__(kIterBlock) __attribute__((amdgpu_waves_per_eu(kFilter == kFilterCubic ? 1 : 4)))
 void kHitBatch(const IteratorContextView *__restrict__ cp, AttrView a,
                                                          uint32_t numRays, const float *__restrict__ org,
@@ -123,7 +123,7 @@ void kHitBatch(const IteratorContextView *__restrict__ cp, AttrView a,
     }
     if (q.has) {
       HitOut o;
-      if (n < maxOut && hitNext(c, smp, st, o)) {
+      if (n < maxOut && hitNext<decltype(smp), kQuery>(c, smp, st, o)) {
         out[(uint64_t)q.r * maxOut + n] = o;
         n++;
       } else {
@@ -163,11 +163,14 @@ struct HitBatchLaunch
   uint32_t *work;
   uint32_t lanes;
   hipStream_t st;
+  // vdb sources run the node-table query only (kQuery 0); structuredRegular
+  // ones choose the query per context at run time.
   template <typename Src, int kFilter>
   hipError_t operator()() const
   {
-    hipLaunchKernelGGL((kHitBatch<Src, kFilter>), dim3(lanes / kIterBlock), dim3(kIterBlock), 0, st, c, a, numRays,
-                       org, dir, tRanges, times, out, maxOut, count, work);
+    constexpr int kQuery = IsDenseSource<Src>::value ? -1 : 0;
+    hipLaunchKernelGGL((kHitBatch<Src, kFilter, kQuery>), dim3(lanes / kIterBlock), dim3(kIterBlock), 0, st, c, a,
+                       numRays, org, dir, tRanges, times, out, maxOut, count, work);
     return hipGetLastError();
   }
 };
