@@ -89,7 +89,7 @@ def variant_layouts(torch, vkl, dev, vol_t, pts, out, steps, warmup, alg_bytes):
     return res
 
 
-def variant_config1(torch, vkl, dev, steps, warmup, cpu=False):
+def variant_dense128(torch, vkl, dev, steps, warmup, cpu=False):
     """The reference's own CPU-runnable case: structuredRegular 128^3 float32
     (the same wavelet + noise generator), 2^20 uniform random points,
     linear, vklComputeSampleN on the x-fastest volume (8 MiB: cache
@@ -126,7 +126,7 @@ def variant_config1(torch, vkl, dev, steps, warmup, cpu=False):
     return res
 
 
-def variant_config2(torch, vkl, dev, vol_t, pts, steps, warmup, U):
+def variant_dense512_matrix(torch, vkl, dev, vol_t, pts, steps, warmup, U):
     """The rest of the 512^3 / 2^26-point matrix on corner-packed volumes:
     half and uint8 (raw codes; the scale is applied by the caller) linear,
     float32 cubic, float32 linear gradient; half / uint8 linear on their raw
@@ -378,7 +378,7 @@ def variant_neural(torch, vkl, dev, res, steps, warmup, train_log2=28, infer_log
 
 
 def variant_raymarch(torch, vkl, vol, n, steps, warmup, width=1920, height=1080, march=512):
-    """Config-2 ray marching: 1920x1080 orthographic rays along +z through the
+    """Ray marching: 1920x1080 orthographic rays along +z through the
     512^3 volume, 512 fixed steps each (samples at step midpoints), fused in
     one kernel (vklComputeRayMarchN)."""
     s = vol.sampler(filter=vkl.VKL_FILTER_LINEAR)
@@ -431,7 +431,7 @@ def variant_raymarch(torch, vkl, vol, n, steps, warmup, width=1920, height=1080,
 
 
 def variant_woodcock(torch, vkl, vol, n, steps, warmup, width=1920, height=1080, sigma=0.5):
-    """Config-2 free-flight sampling: 1920x1080 rays along +z through the
+    """Free-flight sampling: 1920x1080 rays along +z through the
     512^3 volume, Woodcock tracking with majorant `sigma` per voxel length and
     an opacity ramp 0 -> 0.05 over the volume's value range
     (vklComputeWoodcockN; TEA states continue across steps)."""
@@ -714,8 +714,8 @@ def main():
                     help="device voxel layout: the caller's x-fastest array (with --brick) or corner-packed cells")
     ap.add_argument("--binned", choices=("auto", "on", "off"), default="auto",
                     help="spatially binned sampling (auto: on for large volumes and batches)")
-    ap.add_argument("--variants", default="config1,layouts,config2,raymarch,woodcock,batched,neural,iterators",
-                    help="comma list of extra measurements: config1, layouts, config2, raymarch, woodcock, batched, "
+    ap.add_argument("--variants", default="dense128,layouts,dense512_matrix,raymarch,woodcock,batched,neural,iterators",
+                    help="comma list of extra measurements: dense128, layouts, dense512_matrix, raymarch, woodcock, batched, "
                          "neural, "
                          "iterators "
                          "('' = none)")
@@ -864,8 +864,8 @@ def main():
     extra = {}
     for v in variants:
         log(f"variant {v} ...")
-        if v == "config1":
-            extra["config1"] = variant_config1(torch, vkl, dev, args.steps, args.warmup, cpu=not args.no_cpu_baseline)
+        if v == "dense128":
+            extra["dense128"] = variant_dense128(torch, vkl, dev, args.steps, args.warmup, cpu=not args.no_cpu_baseline)
         elif v == "layouts":
             extra["layouts"] = variant_layouts(torch, vkl, dev, vol_t, pts, out, args.steps, args.warmup,
                                                bytes_per_launch)
@@ -885,8 +885,8 @@ def main():
             rv = dev.structured_regular(vol_t, corner_packed=False)
             extra["woodcock_x_fastest"] = variant_woodcock(torch, vkl, rv, n, args.steps, args.warmup)
             rv.release()
-        elif v == "config2":
-            extra["config2"] = variant_config2(torch, vkl, dev, vol_t, pts, args.steps, args.warmup, U)
+        elif v == "dense512_matrix":
+            extra["dense512_matrix"] = variant_dense512_matrix(torch, vkl, dev, vol_t, pts, args.steps, args.warmup, U)
         elif v == "batched":
             extra["batched"] = variant_batched(torch, vkl, dev, args.steps, args.warmup, packed=True)
             extra["batched_x_fastest"] = variant_batched(torch, vkl, dev, args.steps, args.warmup)

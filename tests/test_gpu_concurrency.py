@@ -38,7 +38,7 @@ def run_threads(fns):
 
 
 def test_concurrent_sampling_and_iteration(device):
-    data = V.config1_volume(96)
+    data = V.wavelet_volume(96)
     vol = device.structured_regular(data)
     s = vol.sampler()
     ctx = s.interval_context()

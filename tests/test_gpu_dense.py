@@ -48,8 +48,8 @@ def check_cubic(grid, got, pts, gradient=False, rel=1e-5):
 
 
 @pytest.fixture(scope="module")
-def config1(device):
-    data = V.config1_volume(128)
+def wavelet128(device):
+    data = V.wavelet_volume(128)
     vol = device.structured_regular(data)
     grid = O.Grid.dense(data)
     # 2^20 points in bounds plus boundary / outside probes
@@ -60,8 +60,8 @@ def config1(device):
 
 
 @pytest.mark.parametrize("filt", [vkl.VKL_FILTER_NEAREST, vkl.VKL_FILTER_LINEAR])
-def test_config1_sample_bitexact(config1, filt):
-    vol, grid, pts = config1
+def test_wavelet128_sample_bitexact(wavelet128, filt):
+    vol, grid, pts = wavelet128
     s = vol.sampler(filter=filt)
     grid.set_sampler(filt)
     want = grid.sample(pts)
@@ -70,8 +70,8 @@ def test_config1_sample_bitexact(config1, filt):
     assert_same(got, want)
 
 
-def test_config1_cubic(config1):
-    vol, grid, pts = config1
+def test_wavelet128_cubic(wavelet128):
+    vol, grid, pts = wavelet128
     s = vol.sampler(filter=vkl.VKL_FILTER_CUBIC)
     grid.set_sampler(O.CUBIC)
     p = pts[:200000]
@@ -81,17 +81,17 @@ def test_config1_cubic(config1):
 
 
 @pytest.mark.parametrize("filt", [vkl.VKL_FILTER_NEAREST, vkl.VKL_FILTER_LINEAR])
-def test_config1_gradient_bitexact(config1, filt):
-    vol, grid, pts = config1
+def test_wavelet128_gradient_bitexact(wavelet128, filt):
+    vol, grid, pts = wavelet128
     s = vol.sampler(filter=filt)
     grid.set_sampler(filt)
     p = pts[:300000]
     assert_same(s.gradient(torch.from_numpy(p).cuda()).cpu().numpy(), grid.gradient(p))
 
 
-def test_scalar_and_wide_calls_match_stream(config1):
+def test_scalar_and_wide_calls_match_stream(wavelet128):
     """sampling_utility.h:35-119: scalar, 4/8/16-wide and stream agree."""
-    vol, grid, pts = config1
+    vol, grid, pts = wavelet128
     s = vol.sampler()
     p = pts[:16]
     ref = s.sample(p)

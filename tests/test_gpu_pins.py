@@ -153,7 +153,7 @@ def test_64_32bit_addressing(device, case):
 def test_full_size_512_corner_packed(device):
     """The headline layout (corner-packed 512^3 float32, 4 GiB of cell
     records) at full size: linear samples and gradients vs the oracle."""
-    data = V.config1_volume(512)
+    data = V.wavelet_volume(512)
     vol = device.structured_regular(torch.from_numpy(data).cuda(), corner_packed=True)
     s = vol.sampler(filter=vkl.VKL_FILTER_LINEAR)
     grid = O.Grid.dense(data)

@@ -37,7 +37,7 @@ def soa(pts, w):
 @pytest.fixture(scope="module")
 def two_attr(device):
     rng = np.random.default_rng(41)
-    a0 = V.config1_volume(40)
+    a0 = V.wavelet_volume(40)
     a1 = rng.standard_normal((40, 40, 40)).astype(np.float16)
     vol = device.structured_regular([a0, a1], origin=(-2.0, 1.0, 0.5), spacing=(0.5, 0.75, 1.25))
     grid = O.Grid.dense([a0, a1], origin=(-2.0, 1.0, 0.5), spacing=(0.5, 0.75, 1.25))

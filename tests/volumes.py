@@ -41,7 +41,7 @@ def zramp(x, y, z):
     return z + 0 * x
 
 
-def config1_volume(n=128, seed=SEED):
+def wavelet_volume(n=128, seed=SEED):
     """sin(x) cos(y) sin(z) + U(0, 0.1), origin 0, spacing 1, float32."""
     rng = np.random.default_rng(seed)
     v = procedural((n, n, n), lambda x, y, z: np.sin(x) * np.cos(y) * np.sin(z))
