@@ -93,7 +93,8 @@ def variant_dense128(torch, vkl, dev, steps, warmup, cpu=False):
     """The reference's own CPU-runnable case: structuredRegular 128^3 float32
     (the same wavelet + noise generator), 2^20 uniform random points,
     linear, vklComputeSampleN on the x-fastest volume (8 MiB: cache
-    resident). With `cpu`, the oracle on every usable host core beside it."""
+    resident), and on its corner-packed form (one record per sample). With
+    `cpu`, the oracle on every usable host core beside it."""
     n, npts = 128, 1 << 20
     vol = make_volume(torch, n)
     v = dev.structured_regular(vol)
@@ -102,8 +103,17 @@ def variant_dense128(torch, vkl, dev, steps, warmup, cpu=False):
     pts = torch.rand((npts, 3), device="cuda", generator=g) * float(n - 1)
     out = torch.empty(npts, device="cuda")
     ms = timed(torch, lambda: s.sample(pts, out=out), steps, warmup)
+    vp = dev.structured_regular(vol, corner_packed=True)
+    sp = vp.sampler(filter=vkl.VKL_FILTER_LINEAR)
+    out_p = torch.empty_like(out)
+    ms_p = timed(torch, lambda: sp.sample(pts, out=out_p), steps, warmup)
+    same_p = bool(torch.equal(out_p.view(torch.int32), out.view(torch.int32)))
+    sp.release()
+    vp.release()
     res = {"volume": "structuredRegular 128^3 float32", "points": npts, "ms": round(ms, 4),
-           "msamples_s": round(npts / ms / 1e3, 1)}
+           "msamples_s": round(npts / ms / 1e3, 1),
+           "corner_packed": {"ms": round(ms_p, 4), "msamples_s": round(npts / ms_p / 1e3, 1),
+                             "bitwise_equal_to_x_fastest": same_p}}
     if cpu:
         import oracle as O
         O.use_native()
