@@ -146,6 +146,62 @@ __device__ __forceinline__ float mlpOut(const NfModel &m, const _Float16 *sH2, i
   return y;
 }
 
+// Forward pass of inference: the same binary16 H1 and H2 as mlpHidden, but
+// H2 never reaches LDS. Each lane folds its MFMA outputs (4 rows x 4 column
+// tiles, column 16c + lr) into per-row partial dot products with W3, which
+// go through the input tile's space (dead once H1 is written: 64 rows x
+// kLdP floats fit in 64 x kLdX halves) and are summed per row by the row's
+// lane. Only the fp32 summation order differs from mlpHidden + mlpOut.
+// Without the H2 tile a block holds 14 KB of LDS instead of 23 KB.
+constexpr int kLdP = 17;  // partial-sum row stride (floats)
+static_assert(kWave * kLdP * sizeof(float) <= kWave * kLdX * sizeof(_Float16), "partials must fit the input tile");
+
+__device__ __forceinline__ float mlpForward(const NfModel &m, _Float16 *sX, _Float16 *sH1, int lr, int lg, int lane)
+{
+  const _Float16 *W1T = reinterpret_cast<const _Float16 *>(m.w.W1T);
+  const _Float16 *W2T = reinterpret_cast<const _Float16 *>(m.w.W2T);
+  h8 b1[4];
+#pragma unroll
+  for (int c = 0; c < 4; c++) b1[c] = ld8(W1T + (16 * c + lr) * kNfIn + 8 * lg);
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const h8 a = ld8(sX + (16 * r + lr) * kLdX + 8 * lg);
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      f4 acc = {0.f, 0.f, 0.f, 0.f};
+      acc    = mma(a, b1[c], acc);
+#pragma unroll
+      for (int i = 0; i < 4; i++) sH1[(16 * r + 4 * lg + i) * kLdH + 16 * c + lr] = (_Float16)fmaxf(acc[i], 0.f);
+    }
+  }
+  __syncthreads();
+  float part[4][4] = {};
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    const h8 bA    = ld8(W2T + (16 * c + lr) * kNfHidden + 8 * lg);
+    const h8 bB    = ld8(W2T + (16 * c + lr) * kNfHidden + 32 + 8 * lg);
+    const float w3 = m.w.W3[16 * c + lr];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      f4 acc = {0.f, 0.f, 0.f, 0.f};
+      acc    = mma(ld8(sH1 + (16 * r + lr) * kLdH + 8 * lg), bA, acc);
+      acc    = mma(ld8(sH1 + (16 * r + lr) * kLdH + 32 + 8 * lg), bB, acc);
+#pragma unroll
+      for (int i = 0; i < 4; i++) part[r][i] += (float)(_Float16)fmaxf(acc[i], 0.f) * w3;
+    }
+  }
+  float *sP = reinterpret_cast<float *>(sX);
+#pragma unroll
+  for (int r = 0; r < 4; r++)
+#pragma unroll
+    for (int i = 0; i < 4; i++) sP[(16 * r + 4 * lg + i) * kLdP + lr] = part[r][i];
+  __syncthreads();
+  float y = 0.f;
+#pragma unroll
+  for (int j = 0; j < 16; j++) y += sP[lane * kLdP + j];
+  return y;
+}
+
 // ---- training ------------------------------------------------------------------------
 
 __device__ __forceinline__ uint64_t splitmix(uint64_t x)
@@ -502,9 +558,8 @@ __global__ __launch_bounds__(kWave) void kNfInfer(NfModel m, GridView g, uint32_
                                                    uint32_t *__restrict__ fallback,
                                                    uint32_t *__restrict__ numFallback, int useFallback, int clampBand)
 {
-  __shared__ __attribute__((aligned(16))) _Float16 sX[kWave * kLdX];
+  __shared__ __attribute__((aligned(16))) _Float16 sX[kWave * kLdX];  // X, later the W3 partial sums
   __shared__ __attribute__((aligned(16))) _Float16 sH1[kWave * kLdH];
-  __shared__ __attribute__((aligned(16))) _Float16 sH2[kWave * kLdH];
   const int lane = threadIdx.x, lr = lane & 15, lg = lane >> 4;
   const uint64_t tiles = (n + kWave - 1) / kWave;
   for (uint64_t t = blockIdx.x; t < tiles; t += gridDim.x) {
@@ -522,8 +577,7 @@ __global__ __launch_bounds__(kWave) void kNfInfer(NfModel m, GridView g, uint32_
     }
     nfEncode(m, nfUnit(m, oc), sX + lane * kLdX);
     __syncthreads();
-    mlpHidden(m, sX, sH1, sH2, lr, lg);
-    const float y = fminf(fmaxf(mlpOut(m, sH2, lane), band.lower), band.upper);
+    const float y = fminf(fmaxf(mlpForward(m, sX, sH1, lr, lg, lane), band.lower), band.upper);
     // Wave-aggregated compaction of the fallback points.
     const bool need       = valid && exact;
     const uint64_t mask   = __ballot(need);
