@@ -35,11 +35,14 @@ constexpr int kMaxShadedHits = 8;
 // Primary and shadow queries share one hit-iterator state and one hitNext
 // call site (the hit iteration is inlined once), and no second state is
 // parked, so the state stays in registers.
-// As for the hit batches: 3 waves per SIMD (168 VGPRs instead of 204 at 2
-// waves, a ~50-byte spill) made a 1080p frame 35 % faster. Cubic filters
-// keep the compiler's choice.
+// Occupancy: on structuredRegular sources 5 waves per SIMD (128 VGPRs at
+// most, 60-104 bytes of spill) measured fastest on a 1080p frame of the
+// 512^3 volume: 23.2 ms against 24.8 at 4 waves, 29.5 at 3 and 26.6 at 6.
+// Other sources keep 3 waves (168 VGPRs instead of 204 at 2, 35 % faster).
+// Cubic filters keep the compiler's choice.
 template <typename Src, int kFilter, int kQuery>
-__global__ __launch_bounds__(kIterBlock) __attribute__((amdgpu_waves_per_eu(kFilter == kFilterCubic ? 1 : 3)))
+__global__ __launch_bounds__(kIterBlock) __attribute__((
+    amdgpu_waves_per_eu(kFilter == kFilterCubic ? 1 : (IsDenseSource<Src>::value ? 5 : 3))))
 void kHitRender(const IteratorContextView *__restrict__ cp, AttrView a,
                                                           HitRenderArgs m)
 {
